@@ -1,0 +1,183 @@
+"""Benchmark: SSSP GTEPS on R-MAT scale 24 / edge factor 16 on one MI355X.
+
+A step is one SSSP query (root = max out-degree vertex, uint32 weights
+U[1,100]) over the whole graph, resident in HBM, through libcgmi's C ABI.
+GTEPS = sum of out-degrees of the reached vertices / seconds per query.
+The line also carries: the roofline of the dominant kernel (push expansion),
+the restated reference CPU engine timed on this host (cpu_baseline), and the
+other headline workloads (BFS scale 18, WCC scale 24 symmetrized, PageRank
+scale 25 / edge factor 44) in "extra".
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extra] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS (BFS/SSSP, bit-exact) learned+batched vs plain, 1 MI355X; PR HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_runs(eng, root, steps, warmup):
+    for _ in range(warmup):
+        eng.run(root, copy_out=False)
+    t0 = time.perf_counter()
+    st = None
+    for _ in range(steps):
+        _, st = eng.run(root, copy_out=False)
+    dt = (time.perf_counter() - t0) / steps
+    return dt, st
+
+
+def profile_run(cg, g, alg, root, **opt):
+    eng = cg.Engine(g, alg, profile=1, **opt)
+    eng.run(root, copy_out=False)
+    _, st = eng.run(root, copy_out=False)
+    eng.close()
+    return st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
+              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import cggraph_amd as cg
+
+    t = time.perf_counter()
+    g = cg.Graph.rmat(args.scale, args.edge_factor, seed=1, weights=(1, 100))
+    info = g.info()
+    root = info["max_outdeg_vertex"]
+    log(f"R-MAT s{args.scale} ef{args.edge_factor}: V={info['V']} E={info['E']} root={root} "
+        f"built in {time.perf_counter() - t:.2f}s")
+
+    eng = cg.Engine(g, 1, sched_mode=1)
+    dt, st = timed_runs(eng, root, args.steps, args.warmup)
+    gpu_vals = eng.result()
+    eng.close()
+    m_q = st["edges_traversed"]
+    gteps = m_q / dt / 1e9
+    log(f"SSSP: {dt * 1e3:.3f} ms/query, iterations {st['iterations']}, reached edges {m_q}, "
+        f"{gteps:.2f} GTEPS")
+
+    pst = profile_run(cg, g, 1, root, sched_mode=1)
+    kms = pst["expand_kernel_ms"] / max(pst["expand_launches"], 1)
+    alg_bytes = pst["expand_alg_bytes"] / max(pst["expand_launches"], 1)
+    achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_expand<SSSP>", "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_ms": round(kms, 5),
+                "alg_bytes_per_launch": round(alg_bytes, 1), "launches": pst["expand_launches"]}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        off, dst, w = g.export()
+        og = orc.CSR(off, dst, w)
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        vals, ms, iters = orc.multicore_run(1, og, root, threads)
+        parity = bool(np.array_equal(vals, gpu_vals))
+        cpu_m = orc.reached_edges(og, vals)
+        cpu = {"value": round(cpu_m / (ms * 1e-3) / 1e9, 4), "unit": "GTEPS", "cores": threads,
+               "kind": "port",
+               "sample": f"one SSSP query on the same R-MAT s{args.scale} graph, restated reference "
+                         f"MULTI_CORE engine (Graph_Numa + threeStage_taskSteal), {iters} iterations, "
+                         f"{ms:.0f} ms", "bit_exact_vs_gpu": parity}
+        log(f"CPU baseline: {cpu}")
+
+    extra = {}
+    if not args.no_extra:
+        extra = run_extra(cg, np, info)
+    g.close()
+
+    line = {
+        "metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"SSSP on R-MAT scale {args.scale} edge factor {args.edge_factor} "
+                               f"(a/b/c/d .57/.19/.19/.05, seed 1, self-loops and duplicates removed, "
+                               f"weights U[1,100]), root = max out-degree vertex",
+                   "V": info["V"], "E": info["E"], "root": root, "schedule": "gpu",
+                   "iterations": st["iterations"], "reached_edges": m_q},
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_extra(cg, np, info24):
+    extra = {}
+    # BFS R-MAT scale 18 (the reference's own CPU-runnable configuration)
+    g = cg.Graph.rmat(18, 16, seed=1, weights=None)
+    inf = g.info()
+    eng = cg.Engine(g, 0, sched_mode=1)
+    dt, st = timed_runs(eng, inf["max_outdeg_vertex"], 20, 3)
+    eng.close()
+    extra["bfs_s18"] = {"gteps": round(st["edges_traversed"] / dt / 1e9, 3), "ms": round(dt * 1e3, 4),
+                        "E": inf["E"], "iterations": st["iterations"], "pull_iterations": st["pull_iterations"]}
+    log(f"BFS s18: {extra['bfs_s18']}")
+    g.close()
+    # BFS on scale 24 as well
+    g = cg.Graph.rmat(24, 16, seed=1, weights=None)
+    inf = g.info()
+    eng = cg.Engine(g, 0, sched_mode=1)
+    dt, st = timed_runs(eng, inf["max_outdeg_vertex"], 5, 2)
+    eng.close()
+    extra["bfs_s24"] = {"gteps": round(st["edges_traversed"] / dt / 1e9, 3), "ms": round(dt * 1e3, 4),
+                        "iterations": st["iterations"], "pull_iterations": st["pull_iterations"]}
+    log(f"BFS s24: {extra['bfs_s24']}")
+    g.close()
+    # WCC on the symmetrized scale-24 graph
+    g = cg.Graph.rmat(24, 16, seed=1, weights=None, symmetrize=True)
+    inf = g.info()
+    eng = cg.Engine(g, 2, sched_mode=1)
+    dt, st = timed_runs(eng, 0, 5, 2)
+    eng.close()
+    extra["wcc_s24_sym"] = {"gteps": round(inf["E"] / dt / 1e9, 3), "ms": round(dt * 1e3, 4), "E": inf["E"]}
+    log(f"WCC s24: {extra['wcc_s24_sym']}")
+    g.close()
+    # PageRank on R-MAT scale 25 edge factor 44
+    t = time.perf_counter()
+    g = cg.Graph.rmat(25, 44, seed=1, weights=None)
+    inf = g.info()
+    log(f"R-MAT s25 ef44: V={inf['V']} E={inf['E']} built in {time.perf_counter() - t:.2f}s")
+    eng = cg.Engine(g, 3, profile=1)
+    eng.run(0, copy_out=False)
+    _, st = eng.run(0, copy_out=False)
+    eng.close()
+    per_it = st["expand_kernel_ms"] / max(st["iterations"], 1)
+    alg = 8.0 * inf["E"] + 8.0 * (inf["V"] + 1) + 4.0 * inf["V"]
+    gbs = alg / (per_it * 1e-3) / 1e9
+    extra["pr_s25_ef44"] = {"ms_per_iteration": round(per_it, 4), "alg_bytes_per_iteration": alg,
+                            "GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+                            "E": inf["E"], "iterations": st["iterations"]}
+    log(f"PR s25: {extra['pr_s25_ef44']}")
+    g.close()
+    return extra
+
+
+if __name__ == "__main__":
+    main()

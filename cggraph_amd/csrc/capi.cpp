@@ -1,0 +1,541 @@
+// extern "C" entry points of libcgmi (declared in include/cgmi.h), graph
+// objects, file formats and the segment partition.
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+
+namespace cg {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+const char* last_error() { return g_err.c_str(); }
+void clear_error() { g_err.clear(); }
+
+bool device_available() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return n > 0;
+}
+
+int require_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return set_error(CG_ERR_NO_DEVICE, "no HIP device is available (hipGetDeviceCount: %s, %d devices)",
+                     hipGetErrorString(e), n);
+  }
+  CG_HIP(hipSetDevice(0));
+  return CG_OK;
+}
+
+}  // namespace cg
+
+using namespace cg;
+
+int cg_graph::ensure_host() {
+  if (host_valid) return CG_OK;
+  if (!dev_valid) return set_error(CG_ERR_INTERNAL, "graph has neither a host nor a device copy");
+  try {
+    h_off.resize((size_t)V + 1);
+    h_dst.resize(E);
+    if (has_w) h_w.resize(E);
+  } catch (const std::bad_alloc&) {
+    return set_error(CG_ERR_OOM, "host copy of the graph (%llu edges)", (unsigned long long)E);
+  }
+  CG_HIP(hipMemcpy(h_off.data(), d_off.p, ((size_t)V + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (E) CG_HIP(hipMemcpy(h_dst.data(), d_dst.p, E * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (has_w && E) CG_HIP(hipMemcpy(h_w.data(), d_w.p, E * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  host_valid = true;
+  return CG_OK;
+}
+
+int cg_graph::ensure_device() {
+  if (dev_valid) return CG_OK;
+  if (!host_valid) return set_error(CG_ERR_INTERNAL, "graph has neither a host nor a device copy");
+  CG_TRY(require_device());
+  CG_TRY(d_off.ensure((size_t)V + 1));
+  CG_TRY(d_dst.ensure(E));
+  CG_HIP(hipMemcpy(d_off.p, h_off.data(), ((size_t)V + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  if (E) CG_HIP(hipMemcpy(d_dst.p, h_dst.data(), E * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (has_w) {
+    CG_TRY(d_w.ensure(E));
+    if (E) CG_HIP(hipMemcpy(d_w.p, h_w.data(), E * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  dev_valid = true;
+  return CG_OK;
+}
+
+namespace {
+
+// Host-side facts of a CSR: max-degree vertex, sortedness, symmetry.
+void host_summary(cg_graph* g) {
+  uint64_t best = 0;
+  uint32_t bv = 0;
+  for (uint32_t v = 0; v < g->V; ++v) {
+    const uint64_t d = g->h_off[v + 1] - g->h_off[v];
+    if (d > best) { best = d; bv = v; }
+  }
+  g->max_outdeg = best;
+  g->max_outdeg_vertex = bv;
+  // symmetric: every row sorted and every u->v has v->u (binary search)
+  bool sym = true;
+  for (uint32_t u = 0; u < g->V && sym; ++u)
+    for (uint64_t e = g->h_off[u]; e + 1 < g->h_off[u + 1]; ++e)
+      if (g->h_dst[e] >= g->h_dst[e + 1]) { sym = false; break; }
+  for (uint32_t u = 0; u < g->V && sym; ++u)
+    for (uint64_t e = g->h_off[u]; e < g->h_off[u + 1]; ++e) {
+      const uint32_t v = g->h_dst[e];
+      const uint32_t* b = g->h_dst.data() + g->h_off[v];
+      const uint32_t* en = g->h_dst.data() + g->h_off[v + 1];
+      if (!std::binary_search(b, en, u)) { sym = false; break; }
+    }
+  g->symmetric = sym;
+}
+
+int read_file(const char* path, void* dst, uint64_t bytes) {
+  int fd = ::open(path, O_RDONLY);
+  if (fd < 0) return set_error(CG_ERR_IO, "cannot open %s", path);
+  struct stat stt;
+  if (fstat(fd, &stt) != 0 || (uint64_t)stt.st_size != bytes) {
+    ::close(fd);
+    return set_error(CG_ERR_IO, "%s holds %lld bytes, expected %llu (load_binFile size check)", path,
+                     (long long)stt.st_size, (unsigned long long)bytes);
+  }
+  uint64_t done = 0;
+  char* p = static_cast<char*>(dst);
+  while (done < bytes) {
+    ssize_t r = ::read(fd, p + done, std::min<uint64_t>(bytes - done, 1ull << 30));
+    if (r <= 0) {
+      ::close(fd);
+      return set_error(CG_ERR_IO, "read error on %s", path);
+    }
+    done += (uint64_t)r;
+  }
+  ::close(fd);
+  return CG_OK;
+}
+
+int validate_and_adopt(cg_graph* g) {
+  if (g->h_off[0] != 0) return set_error(CG_ERR_INVALID, "offsets[0] must be 0");
+  for (uint32_t v = 0; v < g->V; ++v)
+    if (g->h_off[v + 1] < g->h_off[v]) return set_error(CG_ERR_INVALID, "offsets decrease at vertex %u", v);
+  if (g->h_off[g->V] != g->E)
+    return set_error(CG_ERR_INVALID, "offsets[V] = %llu but E = %llu", (unsigned long long)g->h_off[g->V],
+                     (unsigned long long)g->E);
+  for (uint64_t e = 0; e < g->E; ++e)
+    if (g->h_dst[e] >= g->V) return set_error(CG_ERR_INVALID, "destination %u at edge %llu is >= V", g->h_dst[e],
+                                              (unsigned long long)e);
+  if (g->has_w)
+    for (uint64_t e = 0; e < g->E; ++e)
+      if (g->h_w[e] > 0xFFFFFFFFu - CG_UNREACHED)
+        return set_error(CG_ERR_INVALID, "weight %u at edge %llu would let value + weight wrap", g->h_w[e],
+                         (unsigned long long)e);
+  if (g->zero_tail > g->V) return set_error(CG_ERR_INVALID, "zero_outdeg_tail > V");
+  for (uint32_t v = g->V - g->zero_tail; v < g->V; ++v)
+    if (g->h_off[v + 1] != g->h_off[v])
+      return set_error(CG_ERR_INVALID, "vertex %u lies in the declared zero-out-degree tail but has edges", v);
+  g->host_valid = true;
+  host_summary(g);
+  return CG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_options_default(cg_options* o) {
+  if (!o) return set_error(CG_ERR_INVALID, "options pointer is NULL");
+  std::memset(o, 0, sizeof(*o));
+  o->segment_size = 1024;
+  o->heavy_active_vertices = 16384;
+  o->heavy_active_edges = 25000000ull;
+  o->bin_wave = 32;
+  o->bin_chunk = 4096;
+  o->chunk_edges = 2048;
+  o->sched_mode = CG_SCHED_GPU;
+  o->forced_choice = 0;
+  o->cpu_threads = 0;
+  o->pr_iterations = 20;
+  o->pr_damping = 0.85f;
+  o->profile = 0;
+  o->trace_path = nullptr;
+  o->mlp_path = nullptr;
+  o->fault_flags = 0;
+  return CG_OK;
+}
+
+int cg_graph_from_csr(uint32_t V, uint64_t E, const void* offsets, int offsets_u64, const uint32_t* dst,
+                      const uint32_t* w, uint32_t zero_outdeg_tail, cg_graph** out) {
+  clear_error();
+  if (!out || !offsets || (E && !dst)) return set_error(CG_ERR_INVALID, "NULL argument");
+  if (!offsets_u64 && E > 0xFFFFFFFFull) return set_error(CG_ERR_INVALID, "uint32 offsets cannot hold E");
+  std::unique_ptr<cg_graph> g(new (std::nothrow) cg_graph());
+  if (!g) return set_error(CG_ERR_OOM, "graph object");
+  g->V = V;
+  g->E = E;
+  g->zero_tail = zero_outdeg_tail;
+  g->has_w = w != nullptr;
+  try {
+    g->h_off.resize((size_t)V + 1);
+    if (offsets_u64) std::memcpy(g->h_off.data(), offsets, ((size_t)V + 1) * sizeof(uint64_t));
+    else {
+      const uint32_t* o32 = static_cast<const uint32_t*>(offsets);
+      for (size_t i = 0; i <= V; ++i) g->h_off[i] = o32[i];
+    }
+    g->h_dst.assign(dst, dst + E);
+    if (w) g->h_w.assign(w, w + E);
+  } catch (const std::bad_alloc&) {
+    return set_error(CG_ERR_OOM, "host copy of the graph");
+  }
+  CG_TRY(validate_and_adopt(g.get()));
+  *out = g.release();
+  return CG_OK;
+}
+
+int cg_graph_load_files(const char* offset_file, int offsets_u64, const char* dst_file, const char* w_file,
+                        uint32_t V, uint64_t E, cg_graph** out) {
+  clear_error();
+  if (!out || !offset_file || !dst_file) return set_error(CG_ERR_INVALID, "NULL argument");
+  std::unique_ptr<cg_graph> g(new (std::nothrow) cg_graph());
+  if (!g) return set_error(CG_ERR_OOM, "graph object");
+  g->V = V;
+  g->E = E;
+  g->has_w = w_file != nullptr;
+  try {
+    g->h_off.resize((size_t)V + 1);
+    g->h_dst.resize(E);
+    if (w_file) g->h_w.resize(E);
+  } catch (const std::bad_alloc&) {
+    return set_error(CG_ERR_OOM, "host copy of the graph");
+  }
+  if (offsets_u64) {
+    CG_TRY(read_file(offset_file, g->h_off.data(), ((uint64_t)V + 1) * 8));
+  } else {
+    std::vector<uint32_t> o32((size_t)V + 1);
+    CG_TRY(read_file(offset_file, o32.data(), ((uint64_t)V + 1) * 4));
+    for (size_t i = 0; i <= V; ++i) g->h_off[i] = o32[i];
+  }
+  CG_TRY(read_file(dst_file, g->h_dst.data(), E * 4));
+  if (w_file) CG_TRY(read_file(w_file, g->h_w.data(), E * 4));
+  CG_TRY(validate_and_adopt(g.get()));
+  *out = g.release();
+  return CG_OK;
+}
+
+int cg_graph_generate_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                           uint32_t wlo, uint32_t whi, int symmetrize, cg_graph** out) {
+  clear_error();
+  if (!out) return set_error(CG_ERR_INVALID, "NULL argument");
+  std::unique_ptr<cg_graph> g(new (std::nothrow) cg_graph());
+  if (!g) return set_error(CG_ERR_OOM, "graph object");
+  CG_TRY(gen_rmat(scale, edge_factor, a, b, c, seed, wlo, whi, symmetrize, g.get()));
+  *out = g.release();
+  return CG_OK;
+}
+
+int cg_graph_generate_grid(uint32_t n, double drop, uint64_t seed, uint32_t wlo, uint32_t whi, cg_graph** out) {
+  clear_error();
+  if (!out) return set_error(CG_ERR_INVALID, "NULL argument");
+  std::unique_ptr<cg_graph> g(new (std::nothrow) cg_graph());
+  if (!g) return set_error(CG_ERR_OOM, "graph object");
+  CG_TRY(gen_grid(n, drop, seed, wlo, whi, g.get()));
+  *out = g.release();
+  return CG_OK;
+}
+
+int cg_graph_info(const cg_graph* g, uint32_t* V, uint64_t* E, int* has_w, int* symmetric, uint32_t* mv) {
+  if (!g) return set_error(CG_ERR_INVALID, "NULL graph");
+  if (V) *V = g->V;
+  if (E) *E = g->E;
+  if (has_w) *has_w = g->has_w;
+  if (symmetric) *symmetric = g->symmetric;
+  if (mv) *mv = g->max_outdeg_vertex;
+  return CG_OK;
+}
+
+int cg_graph_export(const cg_graph* cg, uint64_t* offsets, uint32_t* dst, uint32_t* w) {
+  if (!cg) return set_error(CG_ERR_INVALID, "NULL graph");
+  cg_graph* g = const_cast<cg_graph*>(cg);
+  CG_TRY(g->ensure_host());
+  if (offsets) std::memcpy(offsets, g->h_off.data(), ((size_t)g->V + 1) * sizeof(uint64_t));
+  if (dst && g->E) std::memcpy(dst, g->h_dst.data(), g->E * sizeof(uint32_t));
+  if (w && g->has_w && g->E) std::memcpy(w, g->h_w.data(), g->E * sizeof(uint32_t));
+  return CG_OK;
+}
+
+int cg_graph_build_transpose(cg_graph* g) {
+  clear_error();
+  if (!g) return set_error(CG_ERR_INVALID, "NULL graph");
+  CG_TRY(g->ensure_device());
+  return build_transpose(g);
+}
+
+int cg_graph_device_csr(const cg_graph* cg, const uint64_t** offsets, const uint32_t** dst, const uint32_t** w) {
+  if (!cg) return set_error(CG_ERR_INVALID, "NULL graph");
+  cg_graph* g = const_cast<cg_graph*>(cg);
+  CG_TRY(g->ensure_device());
+  if (offsets) *offsets = g->d_off.p;
+  if (dst) *dst = g->d_dst.p;
+  if (w) *w = g->has_w ? g->d_w.p : nullptr;
+  return CG_OK;
+}
+
+void cg_graph_destroy(cg_graph* g) { delete g; }
+
+// TwoLevelPartition (my_project/CGgraph/Two-level_partition.cuh:213-476):
+// 1024-vertex segments over V' = V - zero tail, sorted by average degree (then
+// id), cut into 4 buckets at average degree 17 / 33 / 257 (lower_bound), and
+// charged against this GPU's HBM in ascending-degree order.  The average is
+// taken in floating point (the reference divides integers first, :253).
+int cg_partition(const cg_graph* cg, int algorithm, cg_partition_info* info) {
+  clear_error();
+  if (!cg || !info) return set_error(CG_ERR_INVALID, "NULL argument");
+  cg_graph* g = const_cast<cg_graph*>(cg);
+  CG_TRY(g->ensure_host());
+  const uint64_t seg = 1024;
+  const uint64_t Vp = (uint64_t)g->V - g->zero_tail;
+  const uint64_t nseg = (Vp + seg - 1) / seg;
+  struct S { double deg; uint32_t id; uint64_t edges; };
+  std::vector<S> segs(nseg);
+  for (uint64_t s = 0; s < nseg; ++s) {
+    const uint64_t a = s * seg, b = std::min<uint64_t>(Vp, a + seg);
+    const uint64_t edges = g->h_off[b] - g->h_off[a];
+    segs[s] = S{(double)edges / (double)(b - a), (uint32_t)s, edges};
+  }
+  std::sort(segs.begin(), segs.end(), [](const S& x, const S& y) {
+    return x.deg < y.deg || (x.deg == y.deg && x.id < y.id);
+  });
+  auto lb = [&](double t) {
+    return (uint64_t)(std::lower_bound(segs.begin(), segs.end(), t,
+                                       [](const S& x, double v) { return x.deg < v; }) - segs.begin());
+  };
+  info->segment_num = nseg;
+  info->bucket_start[0] = 0;
+  info->bucket_start[1] = lb(17.0);
+  info->bucket_start[2] = lb(33.0);
+  info->bucket_start[3] = lb(257.0);
+  info->bucket_start[4] = nseg;
+  int64_t budget = 0;
+  if (device_available()) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = (int64_t)tot;
+    else (void)hipGetLastError();
+  }
+  const uint64_t fixed = 4ull * g->V + 8ull * (Vp + 1) + (1ull << 30);
+  uint64_t used = fixed;
+  bool all = budget > 0;
+  for (const S& s : segs) {
+    const uint64_t b = s.edges * (algorithm == CG_SSSP ? 8 : 4);
+    if ((int64_t)(used + b) > budget) { all = false; break; }
+    used += b;
+  }
+  info->can_hold_all = all ? 1 : 0;
+  info->device_bytes = used;
+  return CG_OK;
+}
+
+int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use_device_num, const cg_options* opt,
+                     cg_engine** out) {
+  clear_error();
+  if (!cg || !out) return set_error(CG_ERR_INVALID, "NULL argument");
+  if (algorithm < CG_BFS || algorithm > CG_PR) return set_error(CG_ERR_INVALID, "unknown algorithm %d", algorithm);
+  if (engine_type != CG_COORDINATION && engine_type != CG_SINGLE_GPU)
+    return set_error(CG_ERR_UNSUPPORTED,
+                     "engine type %d is not provided; this library runs COORDINATION (0) and SINGLE_GPU (4)",
+                     engine_type);
+  if (use_device_num < 1)
+    return set_error(CG_ERR_NO_DEVICE, "use_device_num = %d: the engine needs one GPU", use_device_num);
+  if (use_device_num > 1)
+    return set_error(CG_ERR_UNSUPPORTED, "use_device_num = %d: this build drives one GPU", use_device_num);
+  CG_TRY(require_device());
+  cg_graph* g = const_cast<cg_graph*>(cg);
+  if (algorithm == CG_SSSP && !g->has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
+  CG_TRY(g->ensure_device());
+  std::unique_ptr<cg_engine> e(new (std::nothrow) cg_engine());
+  if (!e) return set_error(CG_ERR_OOM, "engine object");
+  e->g = g;
+  e->alg = algorithm;
+  e->engine_type = engine_type;
+  e->use_device = use_device_num;
+  if (opt) e->opt = *opt;
+  else cg_options_default(&e->opt);
+  if (engine_type == CG_SINGLE_GPU) e->opt.sched_mode = CG_SCHED_GPU;
+  if (e->opt.chunk_edges == 0 || e->opt.bin_wave == 0 || e->opt.bin_chunk < e->opt.bin_wave)
+    return set_error(CG_ERR_INVALID, "degree tiers need 0 < bin_wave <= bin_chunk and chunk_edges > 0");
+  if (e->opt.sched_mode == CG_SCHED_FORCED && (e->opt.forced_choice < 0 || e->opt.forced_choice >= CG_NUM_CHOICES))
+    return set_error(CG_ERR_INVALID, "forced_choice %d outside [0, 16)", e->opt.forced_choice);
+  CG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  CG_HIP(hipEventCreate(&e->ev0));
+  CG_HIP(hipEventCreate(&e->ev1));
+  CG_HIP(hipEventCreateWithFlags(&e->ev_copy, hipEventDisableTiming));
+  const uint64_t V = g->V;
+  const uint64_t words = (V + 63) / 64 + 1;
+  if (algorithm == CG_PR) {
+    CG_TRY(build_transpose(g));
+    CG_TRY(e->rank.ensure(V));
+    CG_TRY(e->rank2.ensure(V));
+    CG_TRY(e->contrib.ensure(V));
+    CG_TRY(e->contrib2.ensure(V));
+    DevGraph dg{g->d_off.p, g->d_dst.p, nullptr, g->d_in_off.p, g->d_in_src.p, g->V};
+    CG_TRY(pr_plan_create(dg, &e->plan, e->stream));
+  } else {
+    CG_TRY(e->val.ensure(V));
+    CG_TRY(e->bm[0].ensure(words));
+    CG_TRY(e->bm[1].ensure(words));
+    CG_TRY(e->q_small.ensure(V));
+    CG_TRY(e->q_med.ensure(V));
+    uint32_t min_chunk = e->opt.chunk_edges;
+    CG_TRY(e->q_chunk.ensure(V + g->E / min_chunk + 1));
+    CG_TRY(e->st.ensure(2));
+    CG_TRY(e->hst.ensure(3));
+    CG_TRY(e->acc.ensure(1));
+  }
+  if (e->opt.trace_path) {
+    e->trace = std::fopen(e->opt.trace_path, "a");
+    if (!e->trace) return set_error(CG_ERR_IO, "cannot open trace file %s", e->opt.trace_path);
+  }
+  *out = e.release();
+  return CG_OK;
+}
+
+int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
+  clear_error();
+  if (!e) return set_error(CG_ERR_INVALID, "NULL engine");
+  cg_graph& g = *e->g;
+  if (e->alg != CG_CC && e->alg != CG_PR && root >= g.V)
+    return set_error(CG_ERR_INVALID, "root %u >= V = %u", root, g.V);
+  CG_HIP(hipSetDevice(0));
+  e->result_valid = false;
+  cg_run_stats rs{};
+  int st = CG_OK;
+  if (e->alg == CG_PR) {
+    DevGraph dg{g.d_off.p, g.d_dst.p, nullptr, g.d_in_off.p, g.d_in_src.p, g.V};
+    const auto t0 = std::chrono::steady_clock::now();
+    double kms = 0;
+    uint32_t launches = 0;
+    st = pr_run(dg, e->plan, e->opt.pr_iterations, e->opt.pr_damping, e->rank.p, e->rank2.p, e->contrib.p,
+                e->contrib2.p, e->stream, e->opt.profile, &kms, &launches);
+    rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    rs.iterations = e->opt.pr_iterations;
+    rs.heavy_iterations = rs.iterations;
+    rs.expand_launches = launches;
+    rs.expand_kernel_ms = kms;
+    rs.edges_relaxed = g.E * (uint64_t)e->opt.pr_iterations;
+    rs.edges_traversed = g.E * (uint64_t)e->opt.pr_iterations;
+    rs.expand_alg_bytes = (double)e->opt.pr_iterations *
+                          (8.0 * (double)g.E + 8.0 * ((double)g.V + 1) + 4.0 * (double)g.V);
+  } else if (e->alg == CG_CC && g.symmetric && e->opt.sched_mode != CG_SCHED_HEURISTIC &&
+             e->opt.sched_mode != CG_SCHED_FORCED) {
+    DevGraph dg{g.d_off.p, g.d_dst.p, nullptr, nullptr, nullptr, g.V};
+    const auto t0 = std::chrono::steady_clock::now();
+    if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, e->stream));
+    st = wcc_hooking(dg, e->val.p, 2, e->stream, nullptr);
+    if (st == CG_OK && e->opt.profile) CG_HIP(hipEventRecord(e->ev1, e->stream));
+    if (st == CG_OK) CG_HIP(hipStreamSynchronize(e->stream));
+    rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st == CG_OK && e->opt.profile) {
+      float ms = 0;
+      CG_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+      rs.expand_kernel_ms = ms;
+    }
+    rs.iterations = 1;
+    rs.heavy_iterations = 1;
+    rs.expand_launches = 1;
+    rs.edges_traversed = g.E;
+    rs.edges_relaxed = g.E;
+    rs.expand_alg_bytes = 8.0 * ((double)g.V + 1) + 4.0 * (double)g.E + 8.0 * (double)g.V;
+  } else {
+    st = engine_run(e, root, &rs);
+  }
+  if (st != CG_OK) return st;
+  e->result_valid = true;
+  e->last_root = root;
+  if (stats) *stats = rs;
+  if (out) return cg_engine_result(e, out);
+  return CG_OK;
+}
+
+int cg_engine_result(cg_engine* e, void* out) {
+  if (!e || !out) return set_error(CG_ERR_INVALID, "NULL argument");
+  if (!e->result_valid) return set_error(CG_ERR_INVALID, "no completed run to read");
+  const void* src = e->alg == CG_PR ? (const void*)e->rank.p : (const void*)e->val.p;
+  CG_HIP(hipMemcpy(out, src, (size_t)e->g->V * 4, hipMemcpyDeviceToHost));
+  return CG_OK;
+}
+
+int cg_engine_result_device(cg_engine* e, void** dev_ptr) {
+  if (!e || !dev_ptr) return set_error(CG_ERR_INVALID, "NULL argument");
+  *dev_ptr = e->alg == CG_PR ? (void*)e->rank.p : (void*)e->val.p;
+  return CG_OK;
+}
+
+void cg_engine_destroy(cg_engine* e) { delete e; }
+
+int cg_push_step_device(int algorithm, const cg_graph* cg, uint32_t* val, const uint64_t* front_in,
+                        uint64_t* front_out, void* stream) {
+  clear_error();
+  if (!cg || !val || !front_in || !front_out) return set_error(CG_ERR_INVALID, "NULL argument");
+  if (algorithm == CG_PR) return set_error(CG_ERR_UNSUPPORTED, "PageRank has no push step");
+  cg_graph* g = const_cast<cg_graph*>(cg);
+  if (algorithm == CG_SSSP && !g->has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
+  CG_TRY(g->ensure_device());
+  DevGraph dg{g->d_off.p, g->d_dst.p, g->has_w ? g->d_w.p : nullptr, nullptr, nullptr, g->V};
+  return push_step_from_bitmap(algorithm, dg, val, front_in, front_out, static_cast<hipStream_t>(stream));
+}
+
+int cg_check_result_file(const char* path, const uint32_t* values, uint32_t V, const uint32_t* old2new,
+                         uint64_t* first_bad) {
+  clear_error();
+  if (!path || !values) return set_error(CG_ERR_INVALID, "NULL argument");
+  std::vector<uint32_t> ref(V);
+  CG_TRY(read_file(path, ref.data(), (uint64_t)V * 4));
+  for (uint32_t i = 0; i < V; ++i) {
+    const uint32_t mine = old2new ? values[old2new[i]] : values[i];
+    if (mine != ref[i]) {
+      if (first_bad) *first_bad = i;
+      return set_error(CG_ERR_MISMATCH, "value of vertex %u is %u, the result file holds %u", i, mine, ref[i]);
+    }
+  }
+  return CG_OK;
+}
+
+int cg_save_bin_u32(const char* path, const uint32_t* values, uint64_t n) {
+  clear_error();
+  if (!path || (n && !values)) return set_error(CG_ERR_INVALID, "NULL argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return set_error(CG_ERR_IO, "cannot create %s", path);
+  const size_t wr = n ? std::fwrite(values, 4, n, f) : 0;
+  std::fclose(f);
+  if (wr != n) return set_error(CG_ERR_IO, "short write to %s", path);
+  return CG_OK;
+}
+
+const char* cg_last_error(void) { return last_error(); }
+const char* cg_version(void) { return "cgmi 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// Shared definitions of libcgmi: status handling, device buffers and the
+// graph object.  Every translation unit of the library includes this file.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cgmi.h"
+
+namespace cg {
+
+// Per-thread last error (cg_last_error).
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+const char* last_error();
+void clear_error();
+
+#define CG_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t cg_e_ = (call);                                                            \
+    if (cg_e_ != hipSuccess)                                                              \
+      return ::cg::set_error(CG_ERR_HIP, "%s failed: %s (%s:%d)", #call,                  \
+                             hipGetErrorString(cg_e_), __FILE__, __LINE__);               \
+  } while (0)
+
+#define CG_TRY(expr)                 \
+  do {                               \
+    int cg_s_ = (expr);              \
+    if (cg_s_ != CG_OK) return cg_s_; \
+  } while (0)
+
+constexpr uint32_t kUnreached = CG_UNREACHED;
+constexpr int kWave = 64;  // gfx950 wavefront width
+
+// Owning device allocation.  Freed with hipFree on destruction.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // (Re)allocates when the current allocation is smaller than `count`.
+  int ensure(size_t count) {
+    if (count <= n && p) return CG_OK;
+    reset();
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return set_error(CG_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    n = count;
+    return CG_OK;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Pinned host allocation (hipHostMalloc; never hipHostRegister).
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int ensure(size_t count) {
+    if (count <= n && p) return CG_OK;
+    reset();
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return set_error(CG_ERR_OOM, "hipHostMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    n = count;
+    return CG_OK;
+  }
+};
+
+// Returns CG_OK when a GPU is present and selects device 0.
+int require_device();
+bool device_available();
+
+}  // namespace cg
+
+// The graph object behind cg_graph*.  The host copy is filled lazily for
+// graphs built on the GPU (the host team and cg_graph_export need it).
+struct cg_graph {
+  uint32_t V = 0;
+  uint64_t E = 0;
+  uint32_t zero_tail = 0;   // vertices [V - zero_tail, V) have out-degree 0
+  bool symmetric = false;   // every edge u->v has v->u (set by generators / checked on load)
+  bool has_w = false;
+  uint32_t max_outdeg_vertex = 0;
+  uint64_t max_outdeg = 0;
+  // host CSR
+  bool host_valid = false;
+  std::vector<uint64_t> h_off;
+  std::vector<uint32_t> h_dst, h_w;
+  // device CSR (HBM)
+  bool dev_valid = false;
+  cg::DevBuf<uint64_t> d_off;
+  cg::DevBuf<uint32_t> d_dst, d_w;
+  // transposed graph (in-edges sorted by source), built on demand
+  bool dev_t_valid = false;
+  cg::DevBuf<uint64_t> d_in_off;
+  cg::DevBuf<uint32_t> d_in_src, d_in_w;
+
+  int ensure_host();    // D2H when only the device copy exists
+  int ensure_device();  // H2D when only the host copy exists
+};

@@ -1,0 +1,65 @@
+// The engine object behind cg_engine*.
+#pragma once
+
+#include <cstdio>
+#include <vector>
+
+#include "host_team.hpp"
+#include "kernels.hpp"
+#include "sched.hpp"
+
+struct cg_engine {
+  cg_graph* g = nullptr;
+  int alg = CG_BFS;
+  int engine_type = CG_COORDINATION;
+  int use_device = 1;
+  cg_options opt{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
+
+  // values (uint32 for BFS/SSSP/CC, float for PR) and frontier state
+  cg::DevBuf<uint32_t> val;
+  cg::DevBuf<uint64_t> bm[2];
+  cg::DevBuf<uint32_t> q_small, q_med;
+  cg::DevBuf<uint64_t> q_chunk;
+  cg::DevBuf<cg::DevStats> st;       // two slots, alternating per iteration
+  cg::PinnedBuf<cg::DevStats> hst;
+  cg::DevBuf<unsigned long long> acc;
+  cg::DevBuf<float> sched_probs;      // learned-scheduler output (16 floats)
+
+  // host-team traffic
+  cg::DevBuf<uint32_t> d_share_vals, d_cand_id, d_cand_val;
+  cg::PinnedBuf<uint32_t> p_ids, p_vals, p_cand_id, p_cand_val;
+  cg::PinnedBuf<uint64_t> p_chunks;
+  std::vector<uint32_t> cand_id, cand_val;
+  std::vector<uint64_t> chunk_lo, chunk_hi;
+  cg::HostTeam team;
+  bool team_ready = false;
+
+  // PageRank
+  cg::DevBuf<float> rank, rank2, contrib, contrib2;
+  cg::PrPlan* plan = nullptr;
+
+  // learned scheduler
+  std::vector<float> mlp;   // 6032 parameters
+  bool mlp_valid = false;
+  cg::DevBuf<float> d_mlp;
+
+  // measured throughputs (edges/s) for the host/GPU split
+  double rate_gpu = 2.0e10, rate_cpu = 2.0e8;
+
+  FILE* trace = nullptr;
+  bool result_valid = false;
+  uint32_t last_root = 0;
+  int cur = 0;  // bitmap holding the result's last frontier
+
+  ~cg_engine();
+};
+
+namespace cg {
+int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs);
+int mlp_load(cg_engine* e);
+// Learned-scheduler inference on the device, fused into the stats epilogue.
+int launch_stats_mlp(const float* params, const DevStats* st, uint32_t V, uint64_t E,
+                     uint64_t visited, uint32_t iter, float* probs, hipStream_t s);
+}  // namespace cg

@@ -1,0 +1,396 @@
+// Graph construction on the GPU: the synthetic workloads (R-MAT, lattice),
+// edge-list -> CSR (sort, drop self-loops and duplicates), the transposed
+// graph for pull steps, and the max-out-degree root.
+//
+// Keys are (src << bits) | dst with bits = ceil(log2 V), sorted with
+// rocPRIM's radix sort over 2*bits bits only.  The generators are counter
+// based: edge i of the R-MAT stream depends on (seed, i) alone, so the GPU and
+// the CPU oracle produce identical edge lists.  The reference builds its CSR
+// on the host from packed 12-byte edge records (Basic/Graph/graphBinReader.hpp:
+// 89-326) and requires strictly ascending neighbours (my_project/CGgraph/
+// sortNbr.hpp:84-91); the CSR built here has that property by construction.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "kernels.hpp"
+
+namespace cg {
+
+namespace {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+constexpr int kBlock = 256;
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint32_t prob_threshold(double p) {
+  double t = p * 4294967296.0;
+  if (t <= 0.0) return 0;
+  if (t >= 4294967295.0) return 0xFFFFFFFFu;
+  return (uint32_t)t;
+}
+
+__device__ __forceinline__ uint32_t edge_weight(uint64_t wkey, uint32_t u, uint32_t v, uint32_t lo,
+                                                uint64_t span, int symmetric) {
+  if (symmetric && u > v) { uint32_t t = u; u = v; v = t; }
+  const uint64_t h = mix64(wkey + ((((uint64_t)u) << 32) | v) * kGolden);
+  return lo + (uint32_t)(((h >> 32) * span) >> 32);
+}
+
+__global__ void k_rmat_keys(uint64_t* __restrict__ keys, uint64_t m, uint32_t scale, uint32_t tA,
+                            uint32_t tAB, uint32_t tABC, uint64_t key, int sym) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t s = 0, d = 0;
+    uint64_t r = 0;
+    for (uint32_t l = 0; l < scale; ++l) {
+      if ((l & 1) == 0) r = mix64(key + (i * 16 + (l >> 1) + 1) * kGolden);
+      const uint32_t x = (uint32_t)(r >> ((l & 1) * 32));
+      const uint32_t q = x < tA ? 0u : (x < tAB ? 1u : (x < tABC ? 2u : 3u));
+      s = (s << 1) | (q >> 1);
+      d = (d << 1) | (q & 1);
+    }
+    if (sym) {
+      keys[2 * i] = ((uint64_t)s << scale) | d;
+      keys[2 * i + 1] = ((uint64_t)d << scale) | s;
+    } else {
+      keys[i] = ((uint64_t)s << scale) | d;
+    }
+  }
+}
+
+// Undirected lattice edge e (horizontal edges first); a dropped edge becomes
+// the self-loop 0 -> 0, which the CSR build removes.
+__global__ void k_grid_keys(uint64_t* __restrict__ keys, uint32_t n, uint64_t key, uint32_t tdrop,
+                            uint32_t bits) {
+  const uint64_t nh = (uint64_t)n * (n - 1);
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < 2 * nh;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u, v;
+    if (e < nh) {
+      const uint32_t r = (uint32_t)(e / (n - 1)), c = (uint32_t)(e % (n - 1));
+      u = r * n + c;
+      v = u + 1;
+    } else {
+      const uint64_t f = e - nh;
+      const uint32_t r = (uint32_t)(f / n), c = (uint32_t)(f % n);
+      u = r * n + c;
+      v = u + n;
+    }
+    const uint32_t x = (uint32_t)(mix64(key + (e + 1) * kGolden) >> 32);
+    if (x < tdrop) { u = 0; v = 0; }
+    keys[2 * e] = ((uint64_t)u << bits) | v;
+    keys[2 * e + 1] = ((uint64_t)v << bits) | u;
+  }
+}
+
+__global__ void k_mark(const uint64_t* __restrict__ keys, uint64_t n, uint32_t bits,
+                       uint32_t* __restrict__ flags) {
+  const uint64_t mask = (bits >= 32) ? 0xFFFFFFFFull : ((1ull << bits) - 1);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    const bool self = (k >> bits) == (k & mask);
+    const bool dup = i > 0 && keys[i - 1] == k;
+    flags[i] = (!self && !dup) ? 1u : 0u;
+  }
+}
+
+__global__ void k_scatter(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
+                          const uint64_t* __restrict__ pos, uint64_t n, uint32_t bits,
+                          uint64_t* __restrict__ kept, uint32_t* __restrict__ dst) {
+  const uint64_t mask = (bits >= 32) ? 0xFFFFFFFFull : ((1ull << bits) - 1);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (flags[i]) {
+      const uint64_t p = pos[i];
+      kept[p] = keys[i];
+      dst[p] = (uint32_t)(keys[i] & mask);
+    }
+  }
+}
+
+// off[v] = first position whose key has source >= v (lower bound), v in [0, V].
+__global__ void k_offsets(const uint64_t* __restrict__ kept, uint64_t E, uint32_t V, uint32_t bits,
+                          uint64_t* __restrict__ off) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v <= V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t target = v << bits;
+    uint64_t lo = 0, hi = E;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (kept[mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+    off[v] = lo;
+  }
+}
+
+__global__ void k_weights(const uint64_t* __restrict__ kept, uint64_t E, uint32_t bits,
+                          uint64_t wkey, uint32_t lo, uint64_t span, int symmetric,
+                          uint32_t* __restrict__ w) {
+  const uint64_t mask = (bits >= 32) ? 0xFFFFFFFFull : ((1ull << bits) - 1);
+  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < E;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = kept[p];
+    w[p] = edge_weight(wkey, (uint32_t)(k >> bits), (uint32_t)(k & mask), lo, span, symmetric);
+  }
+}
+
+// Source of every edge without a search: each non-empty vertex writes its id
+// at its first edge, then an inclusive max-scan carries it across the row.
+__global__ void k_row_heads(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ head) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    if (off[v + 1] > off[v]) head[off[v]] = (uint32_t)v;
+}
+
+__global__ void k_transpose_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                 uint64_t E, uint32_t bits, uint64_t* __restrict__ keys) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    keys[e] = ((uint64_t)dst[e] << bits) | src[e];
+}
+
+__global__ void k_low_bits(const uint64_t* __restrict__ keys, uint64_t n, uint32_t bits,
+                           uint32_t* __restrict__ out) {
+  const uint64_t mask = (bits >= 32) ? 0xFFFFFFFFull : ((1ull << bits) - 1);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)(keys[i] & mask);
+}
+
+__global__ void k_max_deg(const uint64_t* __restrict__ off, uint32_t V,
+                          unsigned long long* __restrict__ best) {
+  unsigned long long b = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t deg = off[v + 1] - off[v];
+    // larger degree wins; among equal degrees the smaller id wins
+    const unsigned long long k = ((unsigned long long)deg << 32) | (0xFFFFFFFFull - v);
+    if (k > b) b = k;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    unsigned long long o = __shfl_xor(b, d, 64);
+    if (o > b) b = o;
+  }
+  if (__lane_id() == 0 && b) atomicMax(best, b);
+}
+
+unsigned grid_of(uint64_t n) {
+  uint64_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  return (unsigned)(b < 65536 ? b : 65536);
+}
+
+uint32_t bits_for(uint64_t V) {
+  uint32_t b = 1;
+  while ((1ull << b) < V) ++b;
+  return b;
+}
+
+#define LAUNCH_OK()                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess)                                                               \
+      return set_error(CG_ERR_HIP, "kernel launch failed: %s (%s:%d)",                  \
+                       hipGetErrorString(e_), __FILE__, __LINE__);                      \
+  } while (0)
+
+// Sorts keys[0..n) (double buffer with alt), drops self-loops and
+// duplicates, and fills the graph's device CSR.  On return keys/alt are free
+// to reuse.
+int csr_from_keys(DevBuf<uint64_t>& keys, DevBuf<uint64_t>& alt, uint64_t n, uint32_t V,
+                  uint32_t bits, cg_graph* g, uint64_t wkey, uint32_t wlo, uint32_t whi,
+                  int sym_weights, hipStream_t s) {
+  rocprim::double_buffer<uint64_t> db(keys.p, alt.p);
+  size_t tmp_bytes = 0;
+  CG_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, n, 0, 2 * bits, s));
+  DevBuf<uint8_t> tmp;
+  CG_TRY(tmp.ensure(tmp_bytes));
+  CG_HIP(rocprim::radix_sort_keys(tmp.p, tmp_bytes, db, n, 0, 2 * bits, s));
+  uint64_t* sorted = db.current();
+  uint64_t* other = db.alternate();
+
+  DevBuf<uint32_t> flags;
+  DevBuf<uint64_t> pos;
+  CG_TRY(flags.ensure(n));
+  CG_TRY(pos.ensure(n));
+  hipLaunchKernelGGL(k_mark, dim3(grid_of(n)), dim3(kBlock), 0, s, sorted, n, bits, flags.p);
+  LAUNCH_OK();
+  size_t scan_bytes = 0;
+  CG_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, flags.p, pos.p, (uint64_t)0, n,
+                                 rocprim::plus<uint64_t>(), s));
+  DevBuf<uint8_t> scan_tmp;
+  CG_TRY(scan_tmp.ensure(scan_bytes));
+  CG_HIP(rocprim::exclusive_scan(scan_tmp.p, scan_bytes, flags.p, pos.p, (uint64_t)0, n,
+                                 rocprim::plus<uint64_t>(), s));
+  uint64_t last_pos = 0;
+  uint32_t last_flag = 0;
+  if (n) {
+    CG_HIP(hipMemcpyAsync(&last_pos, pos.p + n - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    CG_HIP(hipMemcpyAsync(&last_flag, flags.p + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  const uint64_t E = n ? last_pos + last_flag : 0;
+  g->V = V;
+  g->E = E;
+  CG_TRY(g->d_off.ensure((uint64_t)V + 1));
+  CG_TRY(g->d_dst.ensure(E));
+  hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(kBlock), 0, s, sorted, flags.p, pos.p, n, bits,
+                     other, g->d_dst.p);
+  LAUNCH_OK();
+  flags.reset();
+  pos.reset();
+  hipLaunchKernelGGL(k_offsets, dim3(grid_of((uint64_t)V + 1)), dim3(kBlock), 0, s, other, E, V, bits,
+                     g->d_off.p);
+  LAUNCH_OK();
+  g->has_w = whi != 0;
+  if (g->has_w) {
+    CG_TRY(g->d_w.ensure(E));
+    hipLaunchKernelGGL(k_weights, dim3(grid_of(E)), dim3(kBlock), 0, s, other, E, bits, wkey, wlo,
+                       (uint64_t)(whi - wlo) + 1, sym_weights, g->d_w.p);
+    LAUNCH_OK();
+  }
+  CG_HIP(hipStreamSynchronize(s));
+  g->dev_valid = true;
+  g->host_valid = false;
+  return CG_OK;
+}
+
+}  // namespace
+
+int gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+             uint32_t wlo, uint32_t whi, int symmetrize, cg_graph* g) {
+  if (scale < 1 || scale > 31) return set_error(CG_ERR_INVALID, "R-MAT scale %u outside [1, 31]", scale);
+  if (whi && wlo > whi) return set_error(CG_ERR_INVALID, "weight range [%u, %u] is empty", wlo, whi);
+  if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+    return set_error(CG_ERR_INVALID, "R-MAT probabilities a=%g b=%g c=%g are not a distribution", a, b, c);
+  CG_TRY(require_device());
+  const uint64_t m = ((uint64_t)1 << scale) * edge_factor;
+  const uint64_t n = symmetrize ? 2 * m : m;
+  hipStream_t s = nullptr;
+  DevBuf<uint64_t> keys, alt;
+  CG_TRY(keys.ensure(n));
+  CG_TRY(alt.ensure(n));
+  const uint64_t key = mix64(seed * kGolden + 0x5851F42D4C957F2Dull);
+  hipLaunchKernelGGL(k_rmat_keys, dim3(grid_of(m)), dim3(kBlock), 0, s, keys.p, m, scale,
+                     prob_threshold(a), prob_threshold(a + b), prob_threshold(a + b + c), key,
+                     symmetrize);
+  LAUNCH_OK();
+  const uint64_t wkey = mix64(seed * kGolden + 0x2545F4914F6CDD1Dull);
+  CG_TRY(csr_from_keys(keys, alt, n, 1u << scale, scale, g, wkey, wlo, whi, 0, s));
+  g->symmetric = symmetrize != 0;
+  g->zero_tail = 0;
+  return device_max_outdeg(g);
+}
+
+int gen_grid(uint32_t n, double drop, uint64_t seed, uint32_t wlo, uint32_t whi, cg_graph* g) {
+  if (n < 2 || (uint64_t)n * n > 0xFFFFFFFFull)
+    return set_error(CG_ERR_INVALID, "grid side %u outside [2, 65535]", n);
+  if (whi && wlo > whi) return set_error(CG_ERR_INVALID, "weight range [%u, %u] is empty", wlo, whi);
+  CG_TRY(require_device());
+  const uint64_t V = (uint64_t)n * n;
+  const uint32_t bits = bits_for(V);
+  const uint64_t keys_n = 4ull * n * (n - 1);
+  hipStream_t s = nullptr;
+  DevBuf<uint64_t> keys, alt;
+  CG_TRY(keys.ensure(keys_n));
+  CG_TRY(alt.ensure(keys_n));
+  const uint64_t key = mix64(seed * kGolden + 0x94D049BB133111EBull);
+  hipLaunchKernelGGL(k_grid_keys, dim3(grid_of(keys_n / 2)), dim3(kBlock), 0, s, keys.p, n, key,
+                     prob_threshold(drop), bits);
+  LAUNCH_OK();
+  const uint64_t wkey = mix64(seed * kGolden + 0x2545F4914F6CDD1Dull);
+  CG_TRY(csr_from_keys(keys, alt, keys_n, (uint32_t)V, bits, g, wkey, wlo, whi, 1, s));
+  g->symmetric = true;
+  g->zero_tail = 0;
+  return device_max_outdeg(g);
+}
+
+int build_transpose(cg_graph* g) {
+  if (g->dev_t_valid) return CG_OK;
+  CG_TRY(g->ensure_device());
+  hipStream_t s = nullptr;
+  const uint64_t E = g->E;
+  const uint32_t V = g->V;
+  const uint32_t bits = bits_for(V);
+  CG_TRY(g->d_in_off.ensure((uint64_t)V + 1));
+  CG_TRY(g->d_in_src.ensure(E));
+  if (E == 0) {
+    CG_TRY(launch_fill_u64(g->d_in_off.p, 0, (uint64_t)V + 1, s));
+    CG_HIP(hipStreamSynchronize(s));
+    g->dev_t_valid = true;
+    return CG_OK;
+  }
+  DevBuf<uint64_t> keys, alt;
+  CG_TRY(keys.ensure(E));
+  CG_TRY(alt.ensure(E));
+  {
+    DevBuf<uint32_t> head, src;
+    CG_TRY(head.ensure(E));
+    CG_TRY(src.ensure(E));
+    CG_HIP(hipMemsetAsync(head.p, 0, E * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_row_heads, dim3(grid_of(V)), dim3(kBlock), 0, s, g->d_off.p, V, head.p);
+    LAUNCH_OK();
+    size_t sb = 0;
+    CG_HIP(rocprim::inclusive_scan(nullptr, sb, head.p, src.p, E, rocprim::maximum<uint32_t>(), s));
+    DevBuf<uint8_t> st;
+    CG_TRY(st.ensure(sb));
+    CG_HIP(rocprim::inclusive_scan(st.p, sb, head.p, src.p, E, rocprim::maximum<uint32_t>(), s));
+    hipLaunchKernelGGL(k_transpose_keys, dim3(grid_of(E)), dim3(kBlock), 0, s, src.p, g->d_dst.p, E,
+                       bits, keys.p);
+    LAUNCH_OK();
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  rocprim::double_buffer<uint64_t> db(keys.p, alt.p);
+  size_t tmp_bytes = 0;
+  DevBuf<uint8_t> tmp;
+  if (g->has_w) {
+    DevBuf<uint32_t> walt;
+    CG_TRY(g->d_in_w.ensure(E));
+    CG_TRY(walt.ensure(E));
+    CG_HIP(hipMemcpyAsync(g->d_in_w.p, g->d_w.p, E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    rocprim::double_buffer<uint32_t> vb(g->d_in_w.p, walt.p);
+    CG_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, db, vb, E, 0, 2 * bits, s));
+    CG_TRY(tmp.ensure(tmp_bytes));
+    CG_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, db, vb, E, 0, 2 * bits, s));
+    if (vb.current() != g->d_in_w.p)
+      CG_HIP(hipMemcpyAsync(g->d_in_w.p, vb.current(), E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    CG_HIP(hipStreamSynchronize(s));
+  } else {
+    CG_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, E, 0, 2 * bits, s));
+    CG_TRY(tmp.ensure(tmp_bytes));
+    CG_HIP(rocprim::radix_sort_keys(tmp.p, tmp_bytes, db, E, 0, 2 * bits, s));
+  }
+  uint64_t* sorted = db.current();
+  hipLaunchKernelGGL(k_offsets, dim3(grid_of((uint64_t)V + 1)), dim3(kBlock), 0, s, sorted, E, V, bits,
+                     g->d_in_off.p);
+  LAUNCH_OK();
+  hipLaunchKernelGGL(k_low_bits, dim3(grid_of(E)), dim3(kBlock), 0, s, sorted, E, bits, g->d_in_src.p);
+  LAUNCH_OK();
+  CG_HIP(hipStreamSynchronize(s));
+  g->dev_t_valid = true;
+  return CG_OK;
+}
+
+int device_max_outdeg(cg_graph* g) {
+  DevBuf<unsigned long long> best;
+  CG_TRY(best.ensure(1));
+  CG_HIP(hipMemset(best.p, 0, sizeof(unsigned long long)));
+  if (g->V) {
+    hipLaunchKernelGGL(k_max_deg, dim3(grid_of(g->V)), dim3(kBlock), 0, 0, g->d_off.p, g->V, best.p);
+    LAUNCH_OK();
+  }
+  unsigned long long b = 0;
+  CG_HIP(hipMemcpy(&b, best.p, sizeof(b), hipMemcpyDeviceToHost));
+  g->max_outdeg = b >> 32;
+  g->max_outdeg_vertex = b ? (uint32_t)(0xFFFFFFFFull - (b & 0xFFFFFFFFull)) : 0;
+  return CG_OK;
+}
+
+}  // namespace cg

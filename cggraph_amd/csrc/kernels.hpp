@@ -1,0 +1,103 @@
+// Host-side launchers of the HIP kernels.  Kernels live in the .hip files;
+// engine.cpp and capi.cpp only see these declarations.
+#pragma once
+
+#include "common.hpp"
+
+namespace cg {
+
+// Counters the frontier kernels accumulate on the device; the host reads the
+// whole struct back once per iteration (one small D2H copy).
+struct DevStats {
+  uint32_t n_small;            // frontier vertices with 0 < degree < bin_wave
+  uint32_t n_med;              // bin_wave <= degree < bin_chunk (one wave each)
+  uint32_t n_chunk;            // edge chunks of vertices with degree >= bin_chunk
+  uint32_t n_active;           // frontier vertices (popcount of the bitmap)
+  unsigned long long m_active; // sum of out-degrees over the frontier
+  uint32_t hist[8];            // frontier vertices by floor(log4(degree)), last bin open
+  uint32_t n_new;              // vertices reached for the first time by the expansion
+  uint32_t n_zero_deg;         // frontier vertices with out-degree 0
+  unsigned long long m_new;    // sum of out-degrees of the newly reached vertices (pull)
+  uint32_t n_cand_applied;     // host candidates that improved a value
+  uint32_t pad[3];
+};
+static_assert(sizeof(DevStats) % 8 == 0, "DevStats layout");
+
+struct DevGraph {
+  const uint64_t* off;
+  const uint32_t* dst;
+  const uint32_t* w;
+  const uint64_t* in_off;
+  const uint32_t* in_src;
+  uint32_t V;
+};
+
+struct Bins {
+  uint32_t bin_wave;
+  uint32_t bin_chunk;
+  uint32_t chunk_edges;
+};
+
+// ---- traversal (traverse.hip) ----
+int launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+int launch_iota_u32(uint32_t* p, uint64_t n, hipStream_t s);
+int launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
+// Sets bits [0, n) of a bitmap of `words` words (the rest cleared).
+int launch_bitmap_prefix(uint64_t* bm, uint64_t words, uint64_t n, hipStream_t s);
+int launch_set_root(uint32_t* val, uint64_t* bm, uint32_t root, hipStream_t s);
+// Frontier statistics of `in` (n_active, m_active, hist, n_zero_deg); zeroes
+// `zero_me` (the bitmap the coming expansion writes).
+int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
+                 DevStats* st, hipStream_t s);
+// Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
+int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
+                   uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
+                   hipStream_t s);
+// Upper bound on the expansion's work items for a frontier of n_active
+// vertices and m_active edges.
+uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);
+// Push expansion over the queues of launch_compact (lengths read on the
+// device; at most small_limit small-tier vertices, the rest being the host
+// team's share).
+int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
+                       const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
+                       uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
+                       hipStream_t s);
+// Bottom-up BFS step: unreached vertices look for a parent in `in`.
+int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
+                    DevStats* st, hipStream_t s);
+// Applies host-produced (vertex, value) candidates with atomic min and marks
+// improved vertices in `out`.
+int launch_apply_candidates(const uint32_t* ids, const uint32_t* vals, uint32_t n, uint32_t* val,
+                            uint64_t* out, DevStats* st, hipStream_t s);
+// Gathers (id, value) of the frontier vertices listed in `ids`.
+int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, uint32_t* out,
+                         hipStream_t s);
+// Sum of out-degrees of vertices whose value != CG_UNREACHED (TEPS numerator).
+int launch_reached_edges(const DevGraph& g, const uint32_t* val, unsigned long long* acc,
+                         hipStream_t s);
+// One push step straight from a bitmap (cg_push_step_device): compaction and
+// expansion with scratch queues owned by the caller.
+int push_step_from_bitmap(int alg, const DevGraph& g, uint32_t* val, const uint64_t* in,
+                          uint64_t* out, hipStream_t s);
+
+// ---- WCC by hooking (wcc.hip) ----
+int wcc_hooking(const DevGraph& g, uint32_t* parent, uint32_t sample_rounds, hipStream_t s,
+                uint64_t* edges_relaxed);
+
+// ---- PageRank (pagerank.hip) ----
+struct PrPlan;  // per-graph vertex tiers for the pull kernel
+int pr_plan_create(const DevGraph& g, PrPlan** plan, hipStream_t s);
+void pr_plan_destroy(PrPlan* plan);
+int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* rank, float* tmp_rank,
+           float* contrib, float* contrib2, hipStream_t s, int profile, double* kernel_ms,
+           uint32_t* launches);
+
+// ---- generators and CSR construction on the GPU (gen.hip) ----
+int gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+             uint32_t wlo, uint32_t whi, int symmetrize, cg_graph* g);
+int gen_grid(uint32_t n, double drop, uint64_t seed, uint32_t wlo, uint32_t whi, cg_graph* g);
+int build_transpose(cg_graph* g);
+int device_max_outdeg(cg_graph* g);
+
+}  // namespace cg

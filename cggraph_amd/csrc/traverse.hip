@@ -1,0 +1,532 @@
+// Frontier kernels of the traversal engine, written for gfx950 (wave64).
+//
+// One iteration of a push step is two launches:
+//   k_compact  one lane per 64-bit frontier word.  Active vertices are split by
+//              out-degree into three work queues -- small (edge-balanced inside
+//              a wavefront), medium (one wavefront per vertex) and chunked (a
+//              vertex of degree >= bin_chunk becomes ceil(deg / chunk_edges)
+//              chunks, one wavefront each).  Queue slots are allocated with one
+//              wave-level exclusive scan and one atomic per queue per wavefront.
+//              The kernel also accumulates the frontier statistics the
+//              schedulers read, and zeroes the bitmap the expansion will write.
+//   k_expand   one wavefront per work item; every edge relaxes
+//              value[dst] with a strict-less pre-check and atomic min; an
+//              improvement sets the destination's bit in the next frontier.
+// The relaxation is the reference's edge operation (paths relative to
+// CGgraph/my_project/CGgraph/): device_algorithm.cuh:51-54 (BFS: value+1),
+// :253-257 (SSSP: value+w), host_algorithm.hpp:424-429 (CC: value).  The
+// degree tiers replace WarpDonate::schedule (device_warpDonate.cuh:43-388),
+// whose CTA / warp / half-warp tiers assume 32-lane warps.
+#include "kernels.hpp"
+
+namespace cg {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Exclusive prefix sum over the 64 lanes; every lane must be active.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t* total) {
+  const uint32_t lane = lane_id();
+  uint32_t v = x;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, kWave);
+    if (lane >= (uint32_t)d) v += t;
+  }
+  *total = __shfl(v, kWave - 1, kWave);
+  return v - x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+__device__ __forceinline__ uint32_t hist_bin(uint64_t deg) {
+  uint32_t l = 63 - __clzll(deg + 1);  // floor(log2(deg + 1))
+  l >>= 1;
+  return l > 7 ? 7 : l;
+}
+
+enum { ALG_BFS = CG_BFS, ALG_SSSP = CG_SSSP, ALG_CC = CG_CC };
+
+// Edge operation shared by every push kernel.  Returns 1 when this update
+// reached the vertex for the first time.
+template <int ALG>
+__device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_t* __restrict__ dst,
+                                          const uint32_t* __restrict__ w, uint32_t* __restrict__ val,
+                                          uint64_t* __restrict__ out) {
+  const uint32_t d = dst[e];
+  uint32_t cand;
+  if constexpr (ALG == ALG_BFS) cand = sv + 1;
+  else if constexpr (ALG == ALG_SSSP) cand = sv + w[e];
+  else cand = sv;
+  if (cand < val[d]) {
+    const uint32_t old = atomicMin(val + d, cand);
+    if (cand < old) {
+      atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
+      return old == kUnreached ? 1u : 0u;
+    }
+  }
+  return 0u;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_iota_u32(uint32_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+
+__global__ void k_bitmap_prefix(uint64_t* __restrict__ bm, uint64_t words, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t lo = i * 64;
+    uint64_t word = 0;
+    if (lo + 64 <= n) word = ~0ull;
+    else if (lo < n) word = (1ull << (n - lo)) - 1;
+    bm[i] = word;
+  }
+}
+
+__global__ void k_set_root(uint32_t* __restrict__ val, uint64_t* __restrict__ bm, uint32_t root) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    val[root] = 0;
+    bm[root >> 6] |= 1ull << (root & 63);
+  }
+}
+
+// Frontier statistics: popcount, out-degree sum, log-degree histogram.  Also
+// zeroes the bitmap the coming expansion writes.  One lane per frontier word.
+__global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ in,
+                                                  uint64_t* __restrict__ zero_me, uint64_t words,
+                                                  DevGraph g, DevStats* __restrict__ st) {
+  const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  uint64_t word = 0;
+  if (wi < words) {
+    word = in[wi];
+    if (zero_me) zero_me[wi] = 0;
+  }
+  if (__ballot(word != 0) == 0) return;  // wave-uniform exit for empty stretches
+  uint32_t c_zero = 0;
+  uint64_t m = 0;
+  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint64_t t = word; t; t &= t - 1) {
+    const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
+    const uint64_t deg = g.off[v + 1] - g.off[v];
+    m += deg;
+    h[hist_bin(deg)]++;
+    c_zero += deg == 0;
+  }
+  const uint32_t act = wave_sum_u32((uint32_t)__popcll(word));
+  const uint64_t msum = wave_sum_u64(m);
+  const uint32_t zsum = wave_sum_u32(c_zero);
+  uint32_t hs[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) hs[k] = wave_sum_u32(h[k]);
+  if (lane_id() == 0) {
+    atomicAdd(&st->n_active, act);
+    atomicAdd(&st->m_active, (unsigned long long)msum);
+    if (zsum) atomicAdd(&st->n_zero_deg, zsum);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (hs[k]) atomicAdd(&st->hist[k], hs[k]);
+  }
+}
+
+// Splits the frontier into the three work queues.  Slots come from one
+// wave-level exclusive scan and one atomic per queue per wavefront.
+__global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
+                                                    DevGraph g, Bins bins,
+                                                    uint32_t* __restrict__ q_small,
+                                                    uint32_t* __restrict__ q_med,
+                                                    uint64_t* __restrict__ q_chunk,
+                                                    DevStats* __restrict__ st) {
+  const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  const uint64_t word = wi < words ? in[wi] : 0;
+  if (__ballot(word != 0) == 0) return;
+  uint32_t c_small = 0, c_med = 0, c_chunk = 0;
+  for (uint64_t t = word; t; t &= t - 1) {
+    const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
+    const uint64_t deg = g.off[v + 1] - g.off[v];
+    if (deg == 0) continue;
+    if (deg < bins.bin_wave) c_small++;
+    else if (deg < bins.bin_chunk) c_med++;
+    else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+  }
+  uint32_t t_small, t_med, t_chunk;
+  const uint32_t o_small = wave_excl_scan(c_small, &t_small);
+  const uint32_t o_med = wave_excl_scan(c_med, &t_med);
+  const uint32_t o_chunk = wave_excl_scan(c_chunk, &t_chunk);
+  uint32_t b_small = 0, b_med = 0, b_chunk = 0;
+  if (lane_id() == 0) {
+    if (t_small) b_small = atomicAdd(&st->n_small, t_small);
+    if (t_med) b_med = atomicAdd(&st->n_med, t_med);
+    if (t_chunk) b_chunk = atomicAdd(&st->n_chunk, t_chunk);
+  }
+  b_small = __shfl(b_small, 0, kWave) + o_small;
+  b_med = __shfl(b_med, 0, kWave) + o_med;
+  b_chunk = __shfl(b_chunk, 0, kWave) + o_chunk;
+  for (uint64_t t = word; t; t &= t - 1) {
+    const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
+    const uint64_t deg = g.off[v + 1] - g.off[v];
+    if (deg == 0) continue;
+    if (deg < bins.bin_wave) q_small[b_small++] = v;
+    else if (deg < bins.bin_chunk) q_med[b_med++] = v;
+    else {
+      const uint32_t nch = (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+      for (uint32_t c = 0; c < nch; ++c) q_chunk[b_chunk++] = ((uint64_t)v << 32) | c;
+    }
+  }
+}
+
+template <int ALG>
+__global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restrict__ val,
+                                                   uint64_t* __restrict__ out,
+                                                   const uint32_t* __restrict__ q_small,
+                                                   const uint32_t* __restrict__ q_med,
+                                                   const uint64_t* __restrict__ q_chunk,
+                                                   uint32_t small_limit, uint32_t chunk_edges,
+                                                   DevStats* __restrict__ st) {
+  // The grid is sized from an upper bound; the queue lengths come from the
+  // compaction and surplus wavefronts leave at once.
+  const uint32_t n_chunk = st->n_chunk, n_med = st->n_med;
+  const uint32_t n_small = st->n_small < small_limit ? st->n_small : small_limit;
+  const uint32_t lane = lane_id();
+  const uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+  uint32_t n_new = 0;
+  if (item < n_chunk) {
+    // a slice of a high-degree vertex: lanes stride its edges
+    const uint64_t pk = q_chunk[item];
+    const uint32_t v = (uint32_t)(pk >> 32);
+    const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
+    const uint64_t end = g.off[v + 1];
+    const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
+    const uint32_t sv = val[v];
+    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG>(sv, k, g.dst, g.w, val, out);
+  } else if (item < (uint64_t)n_chunk + n_med) {
+    const uint32_t v = q_med[item - n_chunk];
+    const uint64_t s = g.off[v], e = g.off[v + 1];
+    const uint32_t sv = val[v];
+    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG>(sv, k, g.dst, g.w, val, out);
+  } else {
+    // 64 low-degree vertices per wavefront, their edges spread evenly over
+    // the lanes: lane j of round r takes edge r*64+j of the concatenated
+    // adjacency lists and finds its owner by binary search over the exclusive
+    // degree prefix held one entry per lane.
+    if (item >= (uint64_t)n_chunk + n_med + (n_small + kWave - 1) / kWave) return;
+    const uint64_t base = (item - n_chunk - n_med) * kWave;
+    const uint32_t cnt = (n_small - base) < (uint64_t)kWave ? (uint32_t)(n_small - base) : kWave;
+    uint32_t deg = 0, sv = 0;
+    uint64_t s = 0;
+    if (lane < cnt) {
+      const uint32_t v = q_small[base + lane];
+      s = g.off[v];
+      deg = (uint32_t)(g.off[v + 1] - s);
+      sv = val[v];
+    }
+    uint32_t total;
+    const uint32_t excl = wave_excl_scan(deg, &total);
+    for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+      const uint32_t j = j0 + lane;
+      uint32_t lo = 0;
+#pragma unroll
+      for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+        const uint32_t pm = __shfl(excl, (int)(lo + step), kWave);
+        if (pm <= j) lo += step;
+      }
+      const uint64_t os = __shfl(s, (int)lo, kWave);
+      const uint32_t osv = __shfl(sv, (int)lo, kWave);
+      const uint32_t oex = __shfl(excl, (int)lo, kWave);
+      if (j < total) n_new += relax<ALG>(osv, os + (j - oex), g.dst, g.w, val, out);
+    }
+  }
+  if constexpr (ALG != ALG_CC) {
+    const uint32_t tot = wave_sum_u32(n_new);
+    if (lane == 0 && tot) atomicAdd(&st->n_new, tot);
+  }
+}
+
+// Bottom-up BFS (Beamer et al., SC'12): every unreached vertex scans its
+// in-edges for a parent in the current frontier.  One lane per vertex, so a
+// wavefront owns one 64-bit word of the next frontier and writes it whole.
+__global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __restrict__ val,
+                                                     const uint64_t* __restrict__ in,
+                                                     uint64_t* __restrict__ out,
+                                                     DevStats* __restrict__ st) {
+  const uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  bool found = false;
+  uint64_t mdeg = 0;
+  if (v < g.V && val[v] == kUnreached) {
+    const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
+    for (uint64_t k = s; k < e; ++k) {
+      const uint32_t u = g.in_src[k];
+      if ((in[u >> 6] >> (u & 63)) & 1ull) {
+        val[v] = val[u] + 1;
+        found = true;
+        mdeg = g.off[v + 1] - g.off[v];
+        break;
+      }
+    }
+  }
+  const uint64_t mask = __ballot(found);
+  const uint64_t words = ((uint64_t)g.V + 63) / 64;
+  if (lane_id() == 0 && (v >> 6) < words) out[v >> 6] = mask;
+  if (mask) {
+    const uint64_t ms = wave_sum_u64(mdeg);
+    if (lane_id() == 0) {
+      atomicAdd(&st->n_new, (uint32_t)__popcll(mask));
+      atomicAdd(&st->m_new, (unsigned long long)ms);
+    }
+  }
+}
+
+__global__ void k_apply_candidates(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ vals,
+                                   uint32_t n, uint32_t* __restrict__ val, uint64_t* __restrict__ out,
+                                   DevStats* __restrict__ st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t applied = 0;
+  if (i < n) {
+    const uint32_t d = ids[i], c = vals[i];
+    if (c < val[d]) {
+      const uint32_t old = atomicMin(val + d, c);
+      if (c < old) {
+        atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
+        applied = 1;
+      }
+    }
+  }
+  const uint32_t tot = wave_sum_u32(applied);
+  if (lane_id() == 0 && tot) atomicAdd(&st->n_cand_applied, tot);
+}
+
+__global__ void k_gather_values(const uint32_t* __restrict__ ids, uint32_t n,
+                                const uint32_t* __restrict__ val, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = val[ids[i]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_reached_edges(DevGraph g, const uint32_t* __restrict__ val,
+                                                          unsigned long long* __restrict__ acc) {
+  uint64_t sum = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x; v < g.V;
+       v += (uint64_t)gridDim.x * kBlock)
+    if (val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
+  sum = wave_sum_u64(sum);
+  if (lane_id() == 0 && sum) atomicAdd(acc, (unsigned long long)sum);
+}
+
+// Allocation-free push step straight from a bitmap: one lane per vertex, so a
+// wavefront covers one frontier word; its active vertices' edges are spread
+// over the lanes as in the small-degree tier of k_expand.
+template <int ALG>
+__global__ __launch_bounds__(kBlock) void k_push_bitmap(DevGraph g, uint32_t* __restrict__ val,
+                                                        const uint64_t* __restrict__ in,
+                                                        uint64_t* __restrict__ out) {
+  const uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint64_t words = ((uint64_t)g.V + 63) / 64;
+  const uint64_t word = (v >> 6) < words ? in[v >> 6] : 0;
+  if (word == 0) return;  // word is wave-uniform
+  uint32_t deg = 0, sv = 0;
+  uint64_t s = 0;
+  if (v < g.V && ((word >> lane) & 1ull)) {
+    s = g.off[v];
+    deg = (uint32_t)(g.off[v + 1] - s);
+    sv = val[v];
+  }
+  uint32_t total;
+  const uint32_t excl = wave_excl_scan(deg, &total);
+  for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+    const uint32_t j = j0 + lane;
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+      const uint32_t pm = __shfl(excl, (int)(lo + step), kWave);
+      if (pm <= j) lo += step;
+    }
+    const uint64_t os = __shfl(s, (int)lo, kWave);
+    const uint32_t osv = __shfl(sv, (int)lo, kWave);
+    const uint32_t oex = __shfl(excl, (int)lo, kWave);
+    if (j < total) (void)relax<ALG>(osv, os + (j - oex), g.dst, g.w, val, out);
+  }
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block = kBlock, unsigned cap = 1u << 20) {
+  uint64_t b = (n + block - 1) / block;
+  if (b == 0) b = 1;
+  return (unsigned)(b < cap ? b : cap);
+}
+
+}  // namespace
+
+#define CG_LAUNCH_CHECK()                                                               \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess)                                                               \
+      return set_error(CG_ERR_HIP, "kernel launch failed: %s (%s:%d)",                  \
+                       hipGetErrorString(e_), __FILE__, __LINE__);                      \
+  } while (0)
+
+int launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
+  if (!n) return CG_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, kBlock, 65536)), dim3(kBlock), 0, s, p, value, n);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s) {
+  if (!n) return CG_OK;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, kBlock, 65536)), dim3(kBlock), 0, s, p, value, n);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_iota_u32(uint32_t* p, uint64_t n, hipStream_t s) {
+  if (!n) return CG_OK;
+  hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, kBlock, 65536)), dim3(kBlock), 0, s, p, n);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_bitmap_prefix(uint64_t* bm, uint64_t words, uint64_t n, hipStream_t s) {
+  if (!words) return CG_OK;
+  hipLaunchKernelGGL(k_bitmap_prefix, dim3(grid_for(words, kBlock, 65536)), dim3(kBlock), 0, s, bm,
+                     words, n);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_set_root(uint32_t* val, uint64_t* bm, uint32_t root, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_root, dim3(1), dim3(64), 0, s, val, bm, root);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
+                 DevStats* st, hipStream_t s) {
+  if (!words) return CG_OK;
+  const uint64_t blocks = (words + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, st);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
+                   uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
+                   hipStream_t s) {
+  if (!words) return CG_OK;
+  const uint64_t blocks = (words + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, words, g, bins,
+                     q_small, q_med, q_chunk, st);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins) {
+  // chunks <= vertices + edges / chunk_edges; medium <= vertices; small groups
+  return n_active + m_active / bins.chunk_edges + n_active + n_active / kWave + 1;
+}
+
+int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
+                       const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
+                       uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
+                       hipStream_t s) {
+  if (!item_bound) return CG_OK;
+  const uint64_t blocks = (item_bound + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (blocks >= (1ull << 31)) return set_error(CG_ERR_INTERNAL, "expansion grid too large");
+  dim3 grid((unsigned)blocks), block(kBlock);
+  switch (alg) {
+    case CG_BFS:
+      hipLaunchKernelGGL(k_expand<ALG_BFS>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
+                         small_limit, bins.chunk_edges, st);
+      break;
+    case CG_SSSP:
+      hipLaunchKernelGGL(k_expand<ALG_SSSP>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
+                         small_limit, bins.chunk_edges, st);
+      break;
+    case CG_CC:
+      hipLaunchKernelGGL(k_expand<ALG_CC>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
+                         small_limit, bins.chunk_edges, st);
+      break;
+    default:
+      return set_error(CG_ERR_UNSUPPORTED, "push expansion: algorithm %d", alg);
+  }
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
+                    DevStats* st, hipStream_t s) {
+  if (!g.in_off || !g.in_src) return set_error(CG_ERR_INTERNAL, "pull step without in-edges");
+  const uint64_t threads = ((uint64_t)g.V + 63) / 64 * 64;
+  const uint64_t blocks = (threads + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_pull_bfs, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, st);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_apply_candidates(const uint32_t* ids, const uint32_t* vals, uint32_t n, uint32_t* val,
+                            uint64_t* out, DevStats* st, hipStream_t s) {
+  if (!n) return CG_OK;
+  hipLaunchKernelGGL(k_apply_candidates, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ids,
+                     vals, n, val, out, st);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, uint32_t* out,
+                         hipStream_t s) {
+  if (!n) return CG_OK;
+  hipLaunchKernelGGL(k_gather_values, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ids, n,
+                     val, out);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_reached_edges(const DevGraph& g, const uint32_t* val, unsigned long long* acc,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_reached_edges, dim3(grid_for(g.V, kBlock, 8192)), dim3(kBlock), 0, s, g, val,
+                     acc);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int push_step_from_bitmap(int alg, const DevGraph& g, uint32_t* val, const uint64_t* in,
+                          uint64_t* out, hipStream_t s) {
+  const uint64_t threads = ((uint64_t)g.V + 63) / 64 * 64;
+  if (!threads) return CG_OK;
+  dim3 grid((unsigned)((threads + kBlock - 1) / kBlock)), block(kBlock);
+  switch (alg) {
+    case CG_BFS: hipLaunchKernelGGL(k_push_bitmap<ALG_BFS>, grid, block, 0, s, g, val, in, out); break;
+    case CG_SSSP: hipLaunchKernelGGL(k_push_bitmap<ALG_SSSP>, grid, block, 0, s, g, val, in, out); break;
+    case CG_CC: hipLaunchKernelGGL(k_push_bitmap<ALG_CC>, grid, block, 0, s, g, val, in, out); break;
+    default: return set_error(CG_ERR_UNSUPPORTED, "push step: algorithm %d", alg);
+  }
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+}  // namespace cg

@@ -1,0 +1,143 @@
+// WCC by min-label hooking on a symmetric graph (the north-star "CC").
+//
+// The reference computes CC by pushing labels: value[i] = i, then every
+// active vertex lowers its neighbours' labels with write_min / atomicMin
+// (my_project/CGgraph/host_algorithm.hpp:412-433, device_algorithm.cuh:430-465).
+// On a symmetric graph the fixed point is the minimum vertex id of each
+// connected component.  This file reaches the same labels by hooking:
+//   parent[v] = v; for every edge (u, v) link the two trees by pointing the
+//   larger root at the smaller one (CAS); finally compress every path.
+// parent[x] <= x holds throughout, so each tree's root is its minimum id and
+// the final labels equal the label-propagation fixed point bit for bit.
+// The edge order follows Afforest (Sutton, Ben-Nun, Bar, IPDPS'18): link the
+// first `sample_rounds` neighbours of every vertex, compress, find the most
+// frequent root by sampling, then let only vertices outside that component
+// link their remaining edges -- an edge between the large component and
+// another vertex is seen from the other side, because the graph is symmetric.
+#include <algorithm>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace cg {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Link the trees of u and v (Afforest's Link): climb towards the roots; when
+// the larger candidate is a root, CAS it onto the smaller one.
+__device__ void link(uint32_t u, uint32_t v, uint32_t* __restrict__ parent) {
+  uint32_t p1 = ld_relaxed(parent + u);
+  uint32_t p2 = ld_relaxed(parent + v);
+  while (p1 != p2) {
+    const uint32_t high = p1 > p2 ? p1 : p2;
+    const uint32_t low = p1 + p2 - high;
+    const uint32_t p_high = ld_relaxed(parent + high);
+    if (p_high == low) break;
+    if (p_high == high) {
+      uint32_t expected = high;
+      if (__hip_atomic_compare_exchange_strong(parent + high, &expected, low, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        break;
+    }
+    p1 = ld_relaxed(parent + ld_relaxed(parent + high));
+    p2 = ld_relaxed(parent + low);
+  }
+}
+
+__global__ void k_link_round(DevGraph g, uint32_t r, uint32_t* __restrict__ parent) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < g.V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = g.off[v], e = g.off[v + 1];
+    if (s + r < e) link((uint32_t)v, g.dst[s + r], parent);
+  }
+}
+
+__global__ void k_compress(uint32_t V, uint32_t* __restrict__ parent) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t p = ld_relaxed(parent + v);
+    uint32_t gp = ld_relaxed(parent + p);
+    while (p != gp) {
+      __hip_atomic_store(parent + v, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      p = gp;
+      gp = ld_relaxed(parent + p);
+    }
+  }
+}
+
+__global__ void k_sample_roots(const uint32_t* __restrict__ parent, uint32_t V, uint64_t seed,
+                               uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    out[i] = parent[(uint32_t)(z % V)];
+  }
+}
+
+// Remaining edges of vertices outside the large component.  One wavefront
+// per vertex so hub rows spread over 64 lanes.
+__global__ __launch_bounds__(kBlock) void k_link_rest(DevGraph g, uint32_t r, uint32_t skip,
+                                                      uint32_t* __restrict__ parent) {
+  const uint32_t lane = __lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t v = blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64; v < g.V; v += waves) {
+    if (ld_relaxed(parent + v) == skip) continue;  // wave-uniform
+    const uint64_t s = g.off[v] + r, e = g.off[v + 1];
+    for (uint64_t k = s + lane; k < e; k += 64) link((uint32_t)v, g.dst[k], parent);
+  }
+}
+
+unsigned grid_of(uint64_t n) {
+  uint64_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  return (unsigned)(b < 65536 ? b : 65536);
+}
+
+}  // namespace
+
+int wcc_hooking(const DevGraph& g, uint32_t* parent, uint32_t sample_rounds, hipStream_t s,
+                uint64_t* edges_relaxed) {
+  const uint32_t V = g.V;
+  if (edges_relaxed) *edges_relaxed = 0;
+  if (V == 0) return CG_OK;
+  CG_TRY(launch_iota_u32(parent, V, s));
+  for (uint32_t r = 0; r < sample_rounds; ++r) {
+    hipLaunchKernelGGL(k_link_round, dim3(grid_of(V)), dim3(kBlock), 0, s, g, r, parent);
+    hipLaunchKernelGGL(k_compress, dim3(grid_of(V)), dim3(kBlock), 0, s, V, parent);
+  }
+  // most frequent root among 1024 samples
+  constexpr uint32_t kSamples = 1024;
+  DevBuf<uint32_t> samp;
+  CG_TRY(samp.ensure(kSamples));
+  hipLaunchKernelGGL(k_sample_roots, dim3(kSamples / kBlock), dim3(kBlock), 0, s, parent, V,
+                     (uint64_t)0x5EEDull, kSamples, samp.p);
+  std::vector<uint32_t> h(kSamples);
+  CG_HIP(hipMemcpyAsync(h.data(), samp.p, kSamples * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  std::unordered_map<uint32_t, uint32_t> cnt;
+  uint32_t best = h[0], bc = 0;
+  for (uint32_t x : h) {
+    const uint32_t c = ++cnt[x];
+    if (c > bc || (c == bc && x < best)) { bc = c; best = x; }
+  }
+  const uint64_t waves_grid = std::min<uint64_t>(((uint64_t)V + 3) / 4, 65536);
+  hipLaunchKernelGGL(k_link_rest, dim3((unsigned)(waves_grid ? waves_grid : 1)), dim3(kBlock), 0, s, g,
+                     sample_rounds, best, parent);
+  hipLaunchKernelGGL(k_compress, dim3(grid_of(V)), dim3(kBlock), 0, s, V, parent);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(CG_ERR_HIP, "WCC kernels: %s", hipGetErrorString(e));
+  if (edges_relaxed) *edges_relaxed = 0;
+  return CG_OK;
+}
+
+}  // namespace cg

@@ -1,0 +1,196 @@
+"""GPU parity of the traversal engine against the CPU oracle (bit-exact).
+
+Every run goes through the C ABI (libcgmi.so).  Sizes stay where the oracle
+finishes in seconds; the full-size R-MAT cases live in test_gpu_fullsize.py.
+"""
+import numpy as np
+import pytest
+
+import cggraph_amd as cg
+from cggraph_amd import _lib
+
+pytestmark = pytest.mark.gpu
+U = 999_999_999
+BFS, SSSP, CC, PR = 0, 1, 2, 3
+
+
+def gpu_graph(o):
+    return cg.Graph.from_csr(o.offsets, o.dest, o.weight)
+
+
+def run(g, alg, root=0, **opt):
+    e = cg.Engine(g, alg, **opt)
+    try:
+        out, st = e.run(root)
+    finally:
+        e.close()
+    return out, st
+
+
+def sym_of(oracle, g):
+    rows = np.repeat(np.arange(g.V), np.diff(g.offsets).astype(np.int64))
+    return oracle.csr_build(g.V, rows, g.dest, symmetrize=True)
+
+
+@pytest.mark.parametrize("name", ["toy_0", "toy_1"])
+def test_toys_all_roots(oracle, golden, name):
+    t = golden["graphs"][name]
+    og = oracle.CSR(t["offsets"], t["dest"], t["weight"])
+    g = gpu_graph(og)
+    for r in range(t["V"]):
+        for mode in (_lib.CG_SCHED_GPU, _lib.CG_SCHED_HEURISTIC):
+            b, _ = run(g, BFS, r, sched_mode=mode)
+            s, _ = run(g, SSSP, r, sched_mode=mode)
+            assert b.tolist() == oracle.bfs(og, r).tolist(), (name, r, mode)
+            assert s.tolist() == oracle.sssp(og, r).tolist(), (name, r, mode)
+            if str(r) in t["bfs"]:
+                assert b.tolist() == t["bfs"][str(r)]
+                assert s.tolist() == t["sssp"][str(r)]
+    c, _ = run(g, CC)
+    assert c.tolist() == t["cc_directed"]
+    gs = gpu_graph(sym_of(oracle, og))
+    assert gs.info()["symmetric"]
+    c, _ = run(gs, CC)
+    assert c.tolist() == t["cc_symmetric"]
+
+
+def random_graph(oracle, seed, V, m, hub=0):
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, V, m).astype(np.uint32)
+    dst = rng.integers(0, V, m).astype(np.uint32)
+    if hub:
+        src = np.concatenate([src, np.zeros(hub, np.uint32)])
+        dst = np.concatenate([dst, rng.integers(0, V, hub).astype(np.uint32)])
+    g = oracle.csr_build(V, src, dst)
+    g.weight = rng.integers(1, 101, g.E).astype(np.uint32)
+    return g
+
+
+@pytest.mark.parametrize("seed,V,m,hub", [(0, 1000, 5000, 0), (1, 4099, 30000, 9000),
+                                          (2, 70000, 400000, 60000), (3, 5, 0, 0)])
+@pytest.mark.parametrize("mode", [_lib.CG_SCHED_GPU, _lib.CG_SCHED_HEURISTIC])
+def test_random_graphs(oracle, seed, V, m, hub, mode):
+    og = random_graph(oracle, seed, V, m, hub)
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    b, st = run(g, BFS, root, sched_mode=mode)
+    assert np.array_equal(b, oracle.bfs(og, root))
+    assert st["edges_traversed"] == oracle.reached_edges(og, b)
+    s, _ = run(g, SSSP, root, sched_mode=mode)
+    assert np.array_equal(s, oracle.sssp(og, root))
+    c, _ = run(g, CC, sched_mode=mode)
+    assert np.array_equal(c, oracle.cc_directed(og))
+
+
+@pytest.mark.parametrize("choice", range(16))
+def test_forced_choices(oracle, choice):
+    """Every one of the 16 schedule choices reaches the same fixed point."""
+    og = random_graph(oracle, 11, 20000, 150000, hub=20000)
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    for alg, ref in ((BFS, oracle.bfs(og, root)), (SSSP, oracle.sssp(og, root)),
+                     (CC, oracle.cc_directed(og))):
+        got, _ = run(g, alg, root, sched_mode=_lib.CG_SCHED_FORCED, forced_choice=choice)
+        assert np.array_equal(got, ref), (alg, choice)
+
+
+@pytest.mark.parametrize("scale,ef", [(10, 16), (14, 16), (16, 8)])
+def test_rmat_small(oracle, scale, ef):
+    og = oracle.rmat_graph(scale, ef, seed=1)
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    assert g.info()["max_outdeg_vertex"] == root
+    b, _ = run(g, BFS, root)
+    assert np.array_equal(b, oracle.bfs(og, root))
+    s, _ = run(g, SSSP, root)
+    assert np.array_equal(s, oracle.sssp(og, root))
+    sym = sym_of(oracle, og)
+    c, _ = run(gpu_graph(sym), CC)
+    assert np.array_equal(c, oracle.cc_undirected(sym))
+
+
+def test_gpu_generator_matches_oracle(oracle):
+    for scale, ef, sym in ((12, 16, False), (13, 8, True)):
+        og = oracle.rmat_graph(scale, ef, seed=5, symmetrize=sym)
+        g = cg.Graph.rmat(scale, ef, seed=5, symmetrize=sym)
+        off, dst, w = g.export()
+        assert np.array_equal(off, og.offsets) and np.array_equal(dst, og.dest)
+        assert np.array_equal(w, og.weight)
+        assert g.info()["symmetric"] == sym
+    og = oracle.grid_graph(37, 0.1, seed=2)
+    g = cg.Graph.grid(37, 0.1, seed=2)
+    off, dst, w = g.export()
+    assert np.array_equal(off, og.offsets) and np.array_equal(dst, og.dest)
+    assert np.array_equal(w, og.weight)
+
+
+def test_grid_sssp(oracle):
+    og = oracle.grid_graph(200, 0.1, seed=4)
+    g = cg.Graph.grid(200, 0.1, seed=4)
+    for root in (0, 12345, 39999):
+        s, _ = run(g, SSSP, root)
+        assert np.array_equal(s, oracle.sssp(og, root))
+        b, _ = run(g, BFS, root)
+        assert np.array_equal(b, oracle.bfs(og, root))
+    c, _ = run(g, CC)
+    assert np.array_equal(c, oracle.cc_undirected(og))
+
+
+def test_edge_cases(oracle):
+    # no edges at all
+    og = oracle.csr_build(100, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    og.weight = np.zeros(0, np.uint32)
+    g = gpu_graph(og)
+    b, st = run(g, BFS, 7)
+    assert b[7] == 0 and np.all(np.delete(b, 7) == U) and st["iterations"] == 1
+    c, _ = run(g, CC)
+    assert np.array_equal(c, np.arange(100))
+    # root with zero out-degree, V not a multiple of 64
+    og = random_graph(oracle, 9, 131, 300)
+    zero = [v for v in range(og.V) if og.offsets[v + 1] == og.offsets[v]]
+    g = gpu_graph(og)
+    for r in zero[:3] + [130]:
+        s, _ = run(g, SSSP, r)
+        assert np.array_equal(s, oracle.sssp(og, r))
+    # star with a hub far above every tier threshold
+    n = 300001
+    og = oracle.csr_build(n, np.zeros(n - 1, np.uint32), np.arange(1, n, dtype=np.uint32))
+    og.weight = np.full(og.E, 3, np.uint32)
+    g = gpu_graph(og)
+    s, _ = run(g, SSSP, 0)
+    assert s[0] == 0 and np.all(s[1:] == 3)
+    with pytest.raises(cg.CgError):
+        run(g, BFS, n)  # root out of range
+
+
+def test_pagerank_l1_and_determinism(oracle):
+    og = oracle.rmat_graph(16, 16, seed=1, weights=None)
+    g = cg.Graph.from_csr(og.offsets, og.dest)
+    e = cg.Engine(g, PR)
+    r1, st = e.run()
+    r2, _ = e.run()
+    e.close()
+    assert np.array_equal(r1.view(np.uint32), r2.view(np.uint32))
+    ref = oracle.pagerank64(og, 20, 0.85)
+    l1 = float(np.abs(r1.astype(np.float64) - ref).sum())
+    assert l1 <= 1e-6, l1
+    assert st["iterations"] == 20
+
+
+def test_error_paths(oracle):
+    og = random_graph(oracle, 1, 50, 100)
+    g = cg.Graph.from_csr(og.offsets, og.dest)  # no weights
+    with pytest.raises(cg.CgError) as ei:
+        cg.Engine(g, SSSP)
+    assert ei.value.code == _lib.CG_ERR_INVALID
+    with pytest.raises(cg.CgError) as ei:
+        cg.Engine(g, BFS, engine_type=3)
+    assert ei.value.code == _lib.CG_ERR_UNSUPPORTED
+    gw = gpu_graph(og)
+    e = cg.Engine(gw, BFS, sched_mode=_lib.CG_SCHED_GPU, fault_flags=_lib.CG_FAULT_HIP_ERROR)
+    with pytest.raises(cg.CgError) as ei:
+        e.run(0)
+    assert ei.value.code == _lib.CG_ERR_HIP
+    with pytest.raises(cg.CgError):
+        e.result()  # a failed run leaves no result behind
+    e.close()
