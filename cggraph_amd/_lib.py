@@ -23,7 +23,8 @@ CG_UNREACHED = 999_999_999
 CG_NUM_CHOICES = 16
 CG_NUM_FEATURES = 12
 
-CG_SCHED_HEURISTIC, CG_SCHED_GPU, CG_SCHED_LEARNED, CG_SCHED_FORCED = 0, 1, 2, 3
+CG_SCHED_HEURISTIC, CG_SCHED_GPU, CG_SCHED_LEARNED, CG_SCHED_FORCED, CG_SCHED_PROBE = 0, 1, 2, 3, 4
+MLP_PARAMS = 6032
 CG_FAULT_HIP_ERROR, CG_FAULT_MLP_NAN, CG_FAULT_SLOW_GPU = 1, 2, 4
 
 

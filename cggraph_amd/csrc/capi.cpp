@@ -384,9 +384,16 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   e->use_device = use_device_num;
   if (opt) e->opt = *opt;
   else cg_options_default(&e->opt);
+  // keep private copies of the caller's strings
+  if (e->opt.trace_path) { e->trace_path = e->opt.trace_path; e->opt.trace_path = e->trace_path.c_str(); }
+  if (e->opt.mlp_path) { e->mlp_path = e->opt.mlp_path; e->opt.mlp_path = e->mlp_path.c_str(); }
   if (engine_type == CG_SINGLE_GPU) e->opt.sched_mode = CG_SCHED_GPU;
   if (e->opt.chunk_edges == 0 || e->opt.bin_wave == 0 || e->opt.bin_chunk < e->opt.bin_wave)
     return set_error(CG_ERR_INVALID, "degree tiers need 0 < bin_wave <= bin_chunk and chunk_edges > 0");
+  if (e->opt.sched_mode < CG_SCHED_HEURISTIC || e->opt.sched_mode > CG_SCHED_PROBE)
+    return set_error(CG_ERR_INVALID, "unknown scheduler mode %d", e->opt.sched_mode);
+  if (e->opt.sched_mode == CG_SCHED_PROBE && !e->opt.trace_path)
+    return set_error(CG_ERR_INVALID, "probe mode needs a trace_path");
   if (e->opt.sched_mode == CG_SCHED_FORCED && (e->opt.forced_choice < 0 || e->opt.forced_choice >= CG_NUM_CHOICES))
     return set_error(CG_ERR_INVALID, "forced_choice %d outside [0, 16)", e->opt.forced_choice);
   CG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
@@ -413,7 +420,10 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->q_chunk.ensure(V + g->E / min_chunk + 1));
     CG_TRY(e->st.ensure(2));
     CG_TRY(e->hst.ensure(3));
-    CG_TRY(e->acc.ensure(1));
+    CG_TRY(e->acc.ensure(kAccSlots));
+    CG_TRY(e->parts.ensure(stats_parts_needed(words)));
+    CG_TRY(e->stripes.ensure(kStripes * kStripeStride));
+    CG_TRY(e->run_state.ensure(1));
   }
   if (e->opt.trace_path) {
     e->trace = std::fopen(e->opt.trace_path, "a");
@@ -449,8 +459,11 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
     rs.edges_traversed = g.E * (uint64_t)e->opt.pr_iterations;
     rs.expand_alg_bytes = (double)e->opt.pr_iterations *
                           (8.0 * (double)g.E + 8.0 * ((double)g.V + 1) + 4.0 * (double)g.V);
-  } else if (e->alg == CG_CC && g.symmetric && e->opt.sched_mode != CG_SCHED_HEURISTIC &&
-             e->opt.sched_mode != CG_SCHED_FORCED) {
+  } else if (e->alg == CG_CC && g.symmetric &&
+             (e->opt.sched_mode == CG_SCHED_GPU || e->opt.sched_mode == CG_SCHED_LEARNED)) {
+    // Symmetric input: min-label hooking reaches the label-propagation fixed
+    // point directly; the per-iteration schedulers apply to the frontier
+    // engine (directed input, or the heuristic / forced / probe modes).
     DevGraph dg{g.d_off.p, g.d_dst.p, nullptr, nullptr, nullptr, g.V};
     const auto t0 = std::chrono::steady_clock::now();
     if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, e->stream));

@@ -10,16 +10,17 @@
 // Here the values and both frontier bitmaps live in HBM for the whole run.
 // Per iteration:
 //   1. k_stats reads the frontier bitmap (popcount, edge sum, degree
-//      histogram) and zeroes the next bitmap; the stats come back in one
-//      small copy, which is the only synchronisation of the iteration.
+//      histogram; in learned mode it also runs the scheduler network) and
+//      zeroes the next bitmap; the stats come back in one small copy, which is
+//      the only synchronisation of a GPU iteration.
 //   2. the scheduler picks push / pull, degree tiers and the side;
 //   3. GPU side: k_compact + k_expand (push) or k_pull_bfs (pull);
 //      host side: the frontier's (vertex, value) pairs go to the host team,
 //      whose candidates come back and are applied with atomic min; in a
 //      co-executed iteration the host takes the tail of the small-degree queue
 //      while the GPU expands the rest.
-// Because every path applies the same monotone relaxation until no value
-// changes, every schedule ends at the same unique fixed point.
+// Every path applies the same monotone relaxation until no value changes, so
+// every schedule ends at the same unique fixed point.
 #include "engine.hpp"
 
 #include <omp.h>
@@ -68,6 +69,7 @@ struct IterRecord {
   uint32_t iter;
   int choice;
   int host_share;  // 0 none, 1 host only, 2 co-executed
+  int learned;     // 1 when the network's choice was taken
   double us;
   DevStats s;
   float feat[CG_NUM_FEATURES];
@@ -77,35 +79,189 @@ void write_trace(cg_engine* e, const IterRecord& r, uint32_t root) {
   if (!e->trace) return;
   std::fprintf(e->trace,
                "{\"alg\":%d,\"V\":%u,\"E\":%llu,\"root\":%u,\"iter\":%u,\"choice\":%d,\"host\":%d,"
-               "\"us\":%.3f,\"n_active\":%u,\"m_active\":%llu,\"features\":[",
+               "\"learned\":%d,\"us\":%.3f,\"n_active\":%u,\"m_active\":%llu,\"features\":[",
                e->alg, e->g->V, (unsigned long long)e->g->E, root, r.iter, r.choice, r.host_share,
-               r.us, r.s.n_active, (unsigned long long)r.s.m_active);
+               r.learned, r.us, r.s.n_active, (unsigned long long)r.s.m_active);
   for (int k = 0; k < CG_NUM_FEATURES; ++k)
     std::fprintf(e->trace, "%s%.6g", k ? "," : "", (double)r.feat[k]);
   std::fprintf(e->trace, "]}\n");
 }
 
-// Host side of one iteration: the frontier (or its small-tier tail) goes to
-// the host team; candidates come back and are applied on the GPU.
-int host_iteration(cg_engine* e, const DevGraph& dg, uint64_t* in, uint64_t* out, DevStats* dst_st,
-                   uint32_t take_from, uint32_t n_take, cg_run_stats* rs) {
+void write_probe(cg_engine* e, const IterRecord& r, uint32_t root, const double* us) {
+  if (!e->trace) return;
+  std::fprintf(e->trace,
+               "{\"probe\":1,\"alg\":%d,\"V\":%u,\"E\":%llu,\"root\":%u,\"iter\":%u,\"n_active\":%u,"
+               "\"m_active\":%llu,\"features\":[",
+               e->alg, e->g->V, (unsigned long long)e->g->E, root, r.iter, r.s.n_active,
+               (unsigned long long)r.s.m_active);
+  for (int k = 0; k < CG_NUM_FEATURES; ++k)
+    std::fprintf(e->trace, "%s%.6g", k ? "," : "", (double)r.feat[k]);
+  std::fprintf(e->trace, "],\"us_all\":[");
+  for (int k = 0; k < CG_NUM_CHOICES; ++k) std::fprintf(e->trace, "%s%.3f", k ? "," : "", us[k]);
+  std::fprintf(e->trace, "]}\n");
+}
+
+// Relaxes the host team's share (ids/vals already in pinned memory) and
+// applies its candidates on the GPU.
+int host_relax_and_apply(cg_engine* e, uint32_t n, uint64_t* out, cg_run_stats* rs) {
   hipStream_t s = e->stream;
-  if (n_take) {
-    CG_TRY(e->d_share_vals.ensure(n_take));
-    CG_TRY(launch_gather_values(e->q_small.p + take_from, n_take, e->val.p, e->d_share_vals.p, s));
-    CG_TRY(e->p_ids.ensure(n_take));
-    CG_TRY(e->p_vals.ensure(n_take));
-    CG_HIP(hipMemcpyAsync(e->p_ids.p, e->q_small.p + take_from, n_take * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-    CG_HIP(hipMemcpyAsync(e->p_vals.p, e->d_share_vals.p, n_take * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-    CG_HIP(hipEventRecord(e->ev_copy, s));
+  uint64_t edges = 0;
+  const auto th = clk::now();
+  const uint64_t nc = e->team.relax(e->alg, e->g, e->p_ids.p, e->p_vals.p, nullptr, nullptr, n,
+                                    e->cand_id, e->cand_val, &edges);
+  const double sec = ms_since(th) * 1e-3;
+  if (edges && sec > 0) e->rate_cpu = 0.7 * e->rate_cpu + 0.3 * ((double)edges / sec);
+  rs->edges_relaxed += edges;
+  rs->cpu_vertices += n;
+  rs->bytes_d2h += 8ull * n;
+  if (!nc) return CG_OK;
+  if (nc > 0xFFFFFFFFull) return set_error(CG_ERR_INTERNAL, "host candidate list too long");
+  CG_TRY(e->d_cand_id.ensure(nc));
+  CG_TRY(e->d_cand_val.ensure(nc));
+  CG_HIP(hipMemcpyAsync(e->d_cand_id.p, e->cand_id.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  CG_HIP(hipMemcpyAsync(e->d_cand_val.p, e->cand_val.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  rs->bytes_h2d += 8ull * nc;
+  CG_TRY(launch_apply_candidates(e->d_cand_id.p, e->d_cand_val.p, (uint32_t)nc, e->val.p, out,
+                                 e->stripes.p, s));
+  // the candidate vectors are pageable host memory: finish the copies before reuse
+  CG_HIP(hipStreamSynchronize(s));
+  return CG_OK;
+}
+
+// Queues D2H copies of q_small[from, from + n) and their values; ev_copy
+// marks their completion.
+int queue_share_copy(cg_engine* e, uint32_t from, uint32_t n) {
+  hipStream_t s = e->stream;
+  CG_TRY(e->d_share_vals.ensure(n ? n : 1));
+  CG_TRY(e->p_ids.ensure(n ? n : 1));
+  CG_TRY(e->p_vals.ensure(n ? n : 1));
+  if (n) {
+    CG_TRY(launch_gather_values(e->q_small.p + from, n, e->val.p, e->d_share_vals.p, s));
+    CG_HIP(hipMemcpyAsync(e->p_ids.p, e->q_small.p + from, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    CG_HIP(hipMemcpyAsync(e->p_vals.p, e->d_share_vals.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   }
-  (void)in;
-  (void)out;
-  (void)dg;
-  (void)dst_st;
-  (void)rs;
+  CG_HIP(hipEventRecord(e->ev_copy, s));
+  return CG_OK;
+}
+
+bool probs_usable(const DevStats& s) {
+  if (s.choice < 0 || s.choice >= CG_NUM_CHOICES) return false;
+  float mx = 0.f, sum = 0.f;
+  for (int k = 0; k < CG_NUM_CHOICES; ++k) {
+    if (!std::isfinite(s.probs[k])) return false;
+    mx = std::max(mx, s.probs[k]);
+    sum += s.probs[k];
+  }
+  // the network defers to the heuristic when it is not confident
+  return std::fabs(sum - 1.f) < 1e-3f && mx >= 0.35f;
+}
+
+struct Step {
+  Choice ch;
+  int host_share;  // 0 GPU only, 1 host team only, 2 co-executed
+};
+
+// Executes one iteration's expansion from frontier `in` into `out`.
+int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats& sst, DevStats* dst,
+              uint64_t* in, uint64_t* out, bool* timing_pending, cg_run_stats* local) {
+  hipStream_t s = e->stream;
+  const uint64_t words = ((uint64_t)dg.V + 63) / 64;
+  const int alg = e->alg;
+  // queue lengths restart at zero for this compaction
+  CG_HIP(hipMemsetAsync(dst, 0, 3 * sizeof(uint32_t), s));
+  if (step.ch.pull) {
+    ++local->pull_iterations;
+    ++local->heavy_iterations;
+    local->gpu_vertices += sst.n_active;
+    if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
+    CG_TRY(launch_pull_bfs(dg, e->val.p, in, out, e->stripes.p, s));
+    if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); *timing_pending = true; }
+    local->edges_relaxed += sst.m_active;
+    return CG_OK;
+  }
+  if (step.host_share == 1) {
+    // LIGHT: the host team relaxes the whole frontier
+    ++local->light_iterations;
+    const Bins all_small{0xFFFFFFFFu, 0xFFFFFFFFu, e->opt.chunk_edges};
+    CG_TRY(launch_compact(in, words, dg, all_small, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, s));
+    const uint32_t n = sst.n_active - sst.n_zero_deg;
+    CG_TRY(queue_share_copy(e, 0, n));
+    CG_HIP(hipEventSynchronize(e->ev_copy));
+    return host_relax_and_apply(e, n, out, local);
+  }
+  // GPU push, optionally co-executed with the host team
+  ++local->heavy_iterations;
+  Bins bins{e->opt.bin_wave, e->opt.bin_chunk, e->opt.chunk_edges};
+  if (step.ch.tier) {
+    bins.bin_wave = kTierSets[step.ch.tier].bin_wave;
+    bins.bin_chunk = kTierSets[step.ch.tier].bin_chunk;
+  }
+  CG_TRY(launch_compact(in, words, dg, bins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, s));
+  uint32_t small_limit = 0xFFFFFFFFu, n_host = 0;
+  if (step.host_share == 2) {
+    // Split the small tier by measured throughput (the analytical split of
+    // GraphCooperation.cuh:1036-1086): host share of the edges
+    // R_cpu / (R_cpu + R_gpu), taken from the queue tail.
+    CG_HIP(hipMemcpyAsync(e->hst.p + 2, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
+    CG_HIP(hipStreamSynchronize(s));
+    const uint32_t n_small = e->hst.p[2].n_small;
+    double rg = e->rate_gpu;
+    if (e->opt.fault_flags & CG_FAULT_SLOW_GPU) rg = e->rate_cpu * 0.25;
+    const double frac = e->rate_cpu / (e->rate_cpu + rg);
+    n_host = (uint32_t)std::min<double>(n_small, std::floor(frac * n_small));
+    small_limit = n_small - n_host;
+    if (n_host) CG_TRY(queue_share_copy(e, small_limit, n_host));
+  }
+  const uint64_t bound = expand_item_bound(sst.n_active, sst.m_active, bins);
+  if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
+  const auto tg = clk::now();
+  CG_TRY(launch_expand_push(alg, dg, e->val.p, out, e->q_small.p, e->q_med.p, e->q_chunk.p, bound,
+                            small_limit, bins, dst, e->stripes.p, s));
+  if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); *timing_pending = true; }
+  ++local->expand_launches;
+  local->expand_alg_bytes += 20.0 * (sst.n_active - sst.n_zero_deg) +
+                             (alg == CG_SSSP ? 12.0 : 8.0) * (double)sst.m_active;
+  local->gpu_vertices += sst.n_active - n_host;
+  local->edges_relaxed += sst.m_active;
+  if (n_host) {
+    CG_HIP(hipEventSynchronize(e->ev_copy));
+    CG_TRY(host_relax_and_apply(e, n_host, out, local));
+    const double sec = ms_since(tg) * 1e-3;
+    if (sec > 0) e->rate_gpu = 0.7 * e->rate_gpu + 0.3 * ((double)sst.m_active / sec);
+  }
+  return CG_OK;
+}
+
+// CG_SCHED_PROBE: time every choice from the current state, restoring the
+// values, the frontier, the new-vertex counters and the host shadow after each.
+int probe_all(cg_engine* e, const DevGraph& dg, const DevStats& sst, DevStats* dst, uint64_t* in,
+              uint64_t* out, double* us) {
+  hipStream_t s = e->stream;
+  const uint32_t V = dg.V;
+  const uint64_t words = ((uint64_t)V + 63) / 64;
+  CG_TRY(e->val_snap.ensure(V ? V : 1));
+  CG_TRY(e->bm_snap.ensure(words + 1));
+  CG_HIP(hipMemcpyAsync(e->val_snap.p, e->val.p, (size_t)V * 4, hipMemcpyDeviceToDevice, s));
+  CG_HIP(hipMemcpyAsync(e->bm_snap.p, in, words * 8, hipMemcpyDeviceToDevice, s));
+  std::vector<uint32_t> shadow_snap = e->team.shadow;
+  CG_HIP(hipStreamSynchronize(s));
+  for (int c = 0; c < CG_NUM_CHOICES; ++c) {
+    Step st{decode_choice(c), 0};
+    if (e->alg != CG_BFS) st.ch.pull = 0;
+    st.host_share = (st.ch.host && !st.ch.pull) ? 1 : 0;
+    cg_run_stats scratch{};
+    bool tp = false;
+    const auto t0 = clk::now();
+    CG_TRY(exec_step(e, dg, st, sst, dst, in, out, &tp, &scratch));
+    CG_HIP(hipStreamSynchronize(s));
+    us[c] = std::chrono::duration<double, std::micro>(clk::now() - t0).count();
+    CG_HIP(hipMemcpyAsync(e->val.p, e->val_snap.p, (size_t)V * 4, hipMemcpyDeviceToDevice, s));
+    CG_HIP(hipMemcpyAsync(in, e->bm_snap.p, words * 8, hipMemcpyDeviceToDevice, s));
+    CG_HIP(hipMemsetAsync(out, 0, words * 8, s));
+    CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripes * kStripeStride * sizeof(uint32_t), s));
+    if (st.host_share) e->team.shadow = shadow_snap;
+  }
+  CG_HIP(hipStreamSynchronize(s));
   return CG_OK;
 }
 
@@ -118,31 +274,33 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   hipStream_t s = e->stream;
   const int alg = e->alg;
   const int mode = e->opt.sched_mode;
-  const bool need_pull = alg == CG_BFS && (mode == CG_SCHED_GPU || mode == CG_SCHED_LEARNED ||
-                                           mode == CG_SCHED_FORCED);
-  if (need_pull) CG_TRY(build_transpose(&g));
-  const bool host_possible = mode == CG_SCHED_HEURISTIC || mode == CG_SCHED_LEARNED ||
+  if (alg == CG_SSSP && !g.has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
+  if (alg == CG_BFS && mode != CG_SCHED_HEURISTIC) CG_TRY(build_transpose(&g));
+  const bool learned = mode == CG_SCHED_LEARNED;
+  const bool probe = mode == CG_SCHED_PROBE;
+  if (learned) CG_TRY(mlp_load(e));
+  const bool host_possible = mode == CG_SCHED_HEURISTIC || learned || probe ||
                              (mode == CG_SCHED_FORCED && decode_choice(e->opt.forced_choice).host);
   if (host_possible) CG_TRY(ensure_team(e));
+  const float* mlp_dev = (learned && e->mlp_valid) ? e->d_mlp.p : nullptr;
 
   DevGraph dg{g.d_off.p, g.d_dst.p, g.has_w ? g.d_w.p : nullptr,
               g.dev_t_valid ? g.d_in_off.p : nullptr, g.dev_t_valid ? g.d_in_src.p : nullptr, V};
-  if (alg == CG_SSSP && !g.has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
 
   cg_run_stats local{};
   const auto t0 = clk::now();
   // ---- initAlgorithm (CGgraph_engine.cuh:1259-1316) ----
   CG_HIP(hipMemsetAsync(e->bm[0].p, 0, words * sizeof(uint64_t), s));
   CG_HIP(hipMemsetAsync(e->bm[1].p, 0, words * sizeof(uint64_t), s));
-  uint64_t visited;
+  CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripes * kStripeStride * sizeof(uint32_t), s));
   if (alg == CG_CC) {
     CG_TRY(launch_iota_u32(e->val.p, V, s));
     CG_TRY(launch_bitmap_prefix(e->bm[0].p, words, (uint64_t)V - g.zero_tail, s));
-    visited = V;
+    CG_TRY(launch_init_run_state(e->run_state.p, V, s));
   } else {
     CG_TRY(launch_fill_u32(e->val.p, kUnreached, V, s));
     CG_TRY(launch_set_root(e->val.p, e->bm[0].p, root, s));
-    visited = 1;
+    CG_TRY(launch_init_run_state(e->run_state.p, 1, s));
   }
   if (host_possible) e->team.reset_values(alg, root);
 
@@ -150,7 +308,6 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   uint32_t it = 0, prev_active = 0;
   bool pull = false;
   uint64_t m_explored = 0;
-  const Bins base_bins{e->opt.bin_wave, e->opt.bin_chunk, e->opt.chunk_edges};
   bool timing_pending = false;
   IterRecord rec{};
   auto tit = clk::now();
@@ -159,172 +316,86 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     const int slot = it & 1;
     DevStats* dst = e->st.p + slot;
     CG_HIP(hipMemsetAsync(dst, 0, sizeof(DevStats), s));
-    CG_TRY(launch_stats(e->bm[cur].p, e->bm[cur ^ 1].p, words, dg, dst, s));
-    CG_HIP(hipMemcpyAsync(e->hst.p, e->st.p, 2 * sizeof(DevStats), hipMemcpyDeviceToHost, s));
+    CG_TRY(launch_stats(e->bm[cur].p, e->bm[cur ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p,
+                        e->run_state.p, mlp_dev, dst, s));
+    CG_HIP(hipMemcpyAsync(e->hst.p + slot, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     CG_HIP(hipStreamSynchronize(s));
     if (e->opt.fault_flags & CG_FAULT_HIP_ERROR)
       return set_error(CG_ERR_HIP, "injected HIP failure at iteration %u", it);
     const DevStats sst = e->hst.p[slot];
-    const DevStats prev = e->hst.p[slot ^ 1];
     if (it > 1) {
-      visited += prev.n_new;
-      if (timing_pending && e->opt.profile) {
+      if (timing_pending) {
         float ms = 0.f;
         CG_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
         local.expand_kernel_ms += ms;
         timing_pending = false;
       }
       rec.us = std::chrono::duration<double, std::micro>(clk::now() - tit).count();
-      write_trace(e, rec, root);
+      if (!probe) write_trace(e, rec, root);
     }
     tit = clk::now();
     if (sst.n_active == 0) break;
 
     // ---- schedule ----
     const uint64_t m_unexplored = g.E > m_explored ? g.E - m_explored : 0;
-    Choice ch{0, 0, 0};
-    int host_share = 0;  // 0 none, 1 host only, 2 co-executed
+    Step step{Choice{0, 0, 0}, 0};
+    int took_learned = 0;
     if (mode == CG_SCHED_HEURISTIC) {
-      const bool heavy = heuristic_heavy(e, sst);
-      host_share = heavy ? 2 : 1;
-    } else if (mode == CG_SCHED_GPU) {
-      if (alg == CG_BFS) pull = beamer_pull(pull, sst, m_unexplored, V, prev_active);
-      ch.pull = pull;
+      step.host_share = heuristic_heavy(e, sst) ? 2 : 1;
     } else if (mode == CG_SCHED_FORCED) {
-      ch = decode_choice(e->opt.forced_choice);
-      host_share = ch.host ? 1 : 0;
-    } else {  // learned: decided further below once the MLP is wired in
+      step.ch = decode_choice(e->opt.forced_choice);
+      step.host_share = step.ch.host ? 1 : 0;
+    } else {
       if (alg == CG_BFS) pull = beamer_pull(pull, sst, m_unexplored, V, prev_active);
-      ch.pull = pull;
+      step.ch.pull = pull;
+      if (learned) {
+        if (mlp_dev && probs_usable(sst)) {
+          step.ch = decode_choice(sst.choice);
+          step.host_share = step.ch.host ? 1 : 0;
+          took_learned = 1;
+        } else {
+          ++local.learned_fallbacks;
+        }
+      }
     }
-    if (alg != CG_BFS) ch.pull = 0;
-    if (ch.pull && host_share == 1) host_share = 0;  // pull steps run on the GPU
-    pull = ch.pull;
-    Bins bins = base_bins;
-    if (mode == CG_SCHED_FORCED || mode == CG_SCHED_LEARNED) {
-      bins.bin_wave = kTierSets[ch.tier].bin_wave;
-      bins.bin_chunk = kTierSets[ch.tier].bin_chunk;
-    }
+    if (alg != CG_BFS) step.ch.pull = 0;
+    if (step.ch.pull) step.host_share = 0;  // bottom-up steps run on the GPU
+    pull = step.ch.pull;
 
     rec.iter = it;
-    rec.choice = encode_choice(ch);
-    rec.host_share = host_share;
+    rec.choice = encode_choice(Choice{step.ch.pull, step.ch.tier, step.host_share == 1});
+    rec.host_share = step.host_share;
+    rec.learned = took_learned;
     rec.s = sst;
-    make_features(sst, V, g.E, visited, it, rec.feat);
+    make_features(sst, V, g.E, sst.visited, sst.iter, rec.feat);
     m_explored += sst.m_active;
     prev_active = sst.n_active;
 
     uint64_t* in = e->bm[cur].p;
     uint64_t* out = e->bm[cur ^ 1].p;
-    if (ch.pull) {
-      ++local.pull_iterations;
-      ++local.heavy_iterations;
-      local.gpu_vertices += sst.n_active;
-      if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
-      CG_TRY(launch_pull_bfs(dg, e->val.p, in, out, dst, s));
-      if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); timing_pending = true; }
-      local.edges_relaxed += sst.m_active;
-    } else if (host_share == 1) {
-      // ---- LIGHT: the host team relaxes the whole frontier ----
-      ++local.light_iterations;
-      const Bins all_small{0xFFFFFFFFu, 0xFFFFFFFFu, e->opt.chunk_edges};
-      CG_TRY(launch_compact(in, words, dg, all_small, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, s));
-      const uint32_t n = sst.n_active - sst.n_zero_deg;
-      CG_TRY(e->d_share_vals.ensure(n ? n : 1));
-      CG_TRY(launch_gather_values(e->q_small.p, n, e->val.p, e->d_share_vals.p, s));
-      CG_TRY(e->p_ids.ensure(n ? n : 1));
-      CG_TRY(e->p_vals.ensure(n ? n : 1));
-      if (n) {
-        CG_HIP(hipMemcpyAsync(e->p_ids.p, e->q_small.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        CG_HIP(hipMemcpyAsync(e->p_vals.p, e->d_share_vals.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-      }
-      CG_HIP(hipStreamSynchronize(s));
-      local.bytes_d2h += 8ull * n;
-      uint64_t edges = 0;
-      const auto th = clk::now();
-      const uint64_t nc = e->team.relax(alg, &g, e->p_ids.p, e->p_vals.p, nullptr, nullptr, n,
-                                        e->cand_id, e->cand_val, &edges);
-      const double sec = ms_since(th) * 1e-3;
-      if (edges && sec > 0) e->rate_cpu = 0.7 * e->rate_cpu + 0.3 * (edges / sec);
-      local.edges_relaxed += edges;
-      local.cpu_vertices += n;
-      if (nc) {
-        CG_TRY(e->d_cand_id.ensure(nc));
-        CG_TRY(e->d_cand_val.ensure(nc));
-        CG_HIP(hipMemcpyAsync(e->d_cand_id.p, e->cand_id.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-        CG_HIP(hipMemcpyAsync(e->d_cand_val.p, e->cand_val.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-        local.bytes_h2d += 8ull * nc;
-        CG_TRY(launch_apply_candidates(e->d_cand_id.p, e->d_cand_val.p, (uint32_t)nc, e->val.p, out, dst, s));
-        // the values just shipped and synchronised make the candidate buffers reusable
-        CG_HIP(hipStreamSynchronize(s));
-      }
-    } else {
-      // ---- GPU push, optionally co-executed with the host team ----
-      ++local.heavy_iterations;
-      CG_TRY(launch_compact(in, words, dg, bins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, s));
-      uint32_t small_limit = 0xFFFFFFFFu, n_host = 0, n_small = 0;
-      if (host_share == 2) {
-        // Split the small tier by measured throughput (analytical split in the
-        // spirit of GraphCooperation.cuh:1036-1086): host share of edges
-        // = R_cpu / (R_cpu + R_gpu), taken from the queue tail.
-        DevStats cnt;
-        CG_HIP(hipMemcpyAsync(e->hst.p + 2, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
-        CG_HIP(hipStreamSynchronize(s));
-        cnt = e->hst.p[2];
-        n_small = cnt.n_small;
-        double rg = e->rate_gpu;
-        if (e->opt.fault_flags & CG_FAULT_SLOW_GPU) rg = e->rate_cpu * 0.25;
-        const double frac = e->rate_cpu / (e->rate_cpu + rg);
-        n_host = (uint32_t)std::min<double>(n_small, std::floor(frac * n_small));
-        small_limit = n_small - n_host;
-        if (n_host) CG_TRY(host_iteration(e, dg, in, out, dst, small_limit, n_host, &local));
-      }
-      const uint64_t bound = expand_item_bound(sst.n_active, sst.m_active, bins);
-      if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
-      CG_TRY(launch_expand_push(alg, dg, e->val.p, out, e->q_small.p, e->q_med.p, e->q_chunk.p, bound,
-                                small_limit, bins, dst, s));
-      if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); timing_pending = true; }
-      ++local.expand_launches;
-      local.expand_alg_bytes += 20.0 * (sst.n_active - sst.n_zero_deg) +
-                                (alg == CG_SSSP ? 12.0 : 8.0) * (double)sst.m_active;
-      local.gpu_vertices += sst.n_active - n_host;
-      local.edges_relaxed += sst.m_active;
-      if (n_host) {
-        // host share: its (id, value) copies were queued before the expansion
-        CG_HIP(hipEventSynchronize(e->ev_copy));
-        uint64_t edges = 0;
-        const auto th = clk::now();
-        const uint64_t nc = e->team.relax(alg, &g, e->p_ids.p, e->p_vals.p, nullptr, nullptr, n_host,
-                                          e->cand_id, e->cand_val, &edges);
-        const double sec = ms_since(th) * 1e-3;
-        if (edges && sec > 0) e->rate_cpu = 0.7 * e->rate_cpu + 0.3 * (edges / sec);
-        local.cpu_vertices += n_host;
-        local.bytes_d2h += 8ull * n_host;
-        if (nc) {
-          CG_TRY(e->d_cand_id.ensure(nc));
-          CG_TRY(e->d_cand_val.ensure(nc));
-          CG_HIP(hipMemcpyAsync(e->d_cand_id.p, e->cand_id.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-          CG_HIP(hipMemcpyAsync(e->d_cand_val.p, e->cand_val.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-          local.bytes_h2d += 8ull * nc;
-          CG_TRY(launch_apply_candidates(e->d_cand_id.p, e->d_cand_val.p, (uint32_t)nc, e->val.p, out, dst, s));
-          CG_HIP(hipStreamSynchronize(s));
-        }
-      }
+    if (probe) {
+      double us[CG_NUM_CHOICES];
+      CG_TRY(probe_all(e, dg, sst, dst, in, out, us));
+      write_probe(e, rec, root, us);
     }
+    CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));
     cur ^= 1;
   }
   CG_HIP(hipStreamSynchronize(s));
   local.iterations = it - 1;
   local.ms = ms_since(t0);
   e->cur = cur;
+  if (e->trace) std::fflush(e->trace);
   // TEPS numerator, outside the timed region
   if (alg != CG_CC) {
-    CG_HIP(hipMemsetAsync(e->acc.p, 0, sizeof(unsigned long long), s));
+    CG_HIP(hipMemsetAsync(e->acc.p, 0, kAccSlots * sizeof(unsigned long long), s));
     CG_TRY(launch_reached_edges(dg, e->val.p, e->acc.p, s));
-    unsigned long long m = 0;
-    CG_HIP(hipMemcpyAsync(&m, e->acc.p, sizeof(m), hipMemcpyDeviceToHost, s));
+    unsigned long long m[kAccSlots];
+    CG_HIP(hipMemcpyAsync(m, e->acc.p, sizeof(m), hipMemcpyDeviceToHost, s));
     CG_HIP(hipStreamSynchronize(s));
-    local.edges_traversed = m;
+    unsigned long long tot = 0;
+    for (int k = 0; k < kAccSlots; ++k) tot += m[k];
+    local.edges_traversed = tot;
   } else {
     local.edges_traversed = g.E;
   }

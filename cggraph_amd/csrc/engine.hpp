@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "host_team.hpp"
@@ -24,8 +25,12 @@ struct cg_engine {
   cg::DevBuf<uint64_t> q_chunk;
   cg::DevBuf<cg::DevStats> st;       // two slots, alternating per iteration
   cg::PinnedBuf<cg::DevStats> hst;
-  cg::DevBuf<unsigned long long> acc;
-  cg::DevBuf<float> sched_probs;      // learned-scheduler output (16 floats)
+  cg::DevBuf<unsigned long long> acc;  // kAccSlots striped accumulators
+  cg::DevBuf<cg::StatsPart> parts;     // k_stats workgroup partials
+  cg::DevBuf<uint32_t> stripes;        // striped new-vertex counters
+  cg::DevBuf<cg::RunState> run_state;
+  cg::DevBuf<uint32_t> val_snap;       // probe mode: state restored between choices
+  cg::DevBuf<uint64_t> bm_snap;
 
   // host-team traffic
   cg::DevBuf<uint32_t> d_share_vals, d_cand_id, d_cand_val;
@@ -48,6 +53,7 @@ struct cg_engine {
   // measured throughputs (edges/s) for the host/GPU split
   double rate_gpu = 2.0e10, rate_cpu = 2.0e8;
 
+  std::string trace_path, mlp_path;  // owned copies of the option strings
   FILE* trace = nullptr;
   bool result_valid = false;
   uint32_t last_root = 0;

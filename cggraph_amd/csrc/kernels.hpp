@@ -14,14 +14,36 @@ struct DevStats {
   uint32_t n_chunk;            // edge chunks of vertices with degree >= bin_chunk
   uint32_t n_active;           // frontier vertices (popcount of the bitmap)
   unsigned long long m_active; // sum of out-degrees over the frontier
-  uint32_t hist[8];            // frontier vertices by floor(log4(degree)), last bin open
-  uint32_t n_new;              // vertices reached for the first time by the expansion
+  uint32_t hist[8];            // frontier vertices by floor(log2(degree + 1) / 2), last bin open
+  uint32_t n_new;              // vertices first reached by the previous expansion
   uint32_t n_zero_deg;         // frontier vertices with out-degree 0
-  unsigned long long m_new;    // sum of out-degrees of the newly reached vertices (pull)
+  unsigned long long visited;  // vertices reached so far (BFS/SSSP), V for CC
   uint32_t n_cand_applied;     // host candidates that improved a value
-  uint32_t pad[3];
+  uint32_t ticket;             // k_stats workgroups finished (last one reduces)
+  int32_t choice;              // learned-scheduler argmax, -1 when it did not run
+  uint32_t iter;               // iteration index (1-based) seen by k_stats
+  float probs[16];             // learned-scheduler softmax
 };
 static_assert(sizeof(DevStats) % 8 == 0, "DevStats layout");
+
+// Counters written by many wavefronts are striped over kStripes cache lines
+// so that their atomics do not serialise on one address.
+constexpr int kStripes = 16;
+constexpr int kStripeStride = 32;  // uint32 per stripe = 128 bytes
+
+// Per-run device state the stats kernel carries from iteration to iteration.
+struct RunState {
+  unsigned long long visited;
+  uint32_t iter;
+  uint32_t pad;
+};
+
+// Per-workgroup partial of k_stats, reduced by the last workgroup.
+struct StatsPart {
+  uint32_t n_active, n_zero;
+  unsigned long long m_active;
+  uint32_t hist[8];
+};
 
 struct DevGraph {
   const uint64_t* off;
@@ -45,9 +67,17 @@ int launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
 // Sets bits [0, n) of a bitmap of `words` words (the rest cleared).
 int launch_bitmap_prefix(uint64_t* bm, uint64_t words, uint64_t n, hipStream_t s);
 int launch_set_root(uint32_t* val, uint64_t* bm, uint32_t root, hipStream_t s);
+int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s);
 // Frontier statistics of `in` (n_active, m_active, hist, n_zero_deg); zeroes
-// `zero_me` (the bitmap the coming expansion writes).
+// `zero_me` (the bitmap the coming expansion writes).  Workgroup partials go
+// to `parts` (one StatsPart per workgroup, stats_parts_needed(words)); the
+// last workgroup reduces them in a fixed order, folds the striped new-vertex
+// counters of the previous expansion (`stripes`, zeroed again) into
+// rs->visited, and -- when `mlp` is not NULL -- runs the learned scheduler on
+// the iteration's features and writes choice/probs into st.
+uint64_t stats_parts_needed(uint64_t words);
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
+                 uint64_t E, StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  DevStats* st, hipStream_t s);
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
@@ -62,18 +92,20 @@ uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bin
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
-                       hipStream_t s);
+                       uint32_t* stripes, hipStream_t s);
 // Bottom-up BFS step: unreached vertices look for a parent in `in`.
 int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
-                    DevStats* st, hipStream_t s);
+                    uint32_t* stripes, hipStream_t s);
 // Applies host-produced (vertex, value) candidates with atomic min and marks
 // improved vertices in `out`.
 int launch_apply_candidates(const uint32_t* ids, const uint32_t* vals, uint32_t n, uint32_t* val,
-                            uint64_t* out, DevStats* st, hipStream_t s);
+                            uint64_t* out, uint32_t* stripes, hipStream_t s);
 // Gathers (id, value) of the frontier vertices listed in `ids`.
 int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, uint32_t* out,
                          hipStream_t s);
-// Sum of out-degrees of vertices whose value != CG_UNREACHED (TEPS numerator).
+// Sum of out-degrees of vertices whose value != CG_UNREACHED (TEPS numerator),
+// accumulated into kStripes striped slots (acc[k * kStripeStride / 2]).
+constexpr int kAccSlots = kStripes * kStripeStride / 2;
 int launch_reached_edges(const DevGraph& g, const uint32_t* val, unsigned long long* acc,
                          hipStream_t s);
 // One push step straight from a bitmap (cg_push_step_device): compaction and

@@ -38,15 +38,17 @@ inline int encode_choice(const Choice& ch) { return (ch.pull & 1) | ((ch.tier & 
 //   0 log2(1+frontier vertices)/log2(1+V)   1 log2(1+frontier edges)/log2(1+E)
 //   2 unvisited fraction                    3..10 log-degree histogram / frontier
 //   11 min(iteration, 64)/64
-inline void make_features(const DevStats& s, uint32_t V, uint64_t E, uint64_t visited, uint32_t iter,
-                          float* f) {
-  const double lv = std::log2(1.0 + (double)V), le = std::log2(1.0 + (double)E);
-  f[0] = (float)(std::log2(1.0 + s.n_active) / (lv > 0 ? lv : 1));
-  f[1] = (float)(std::log2(1.0 + (double)s.m_active) / (le > 0 ? le : 1));
-  f[2] = V ? (float)(1.0 - (double)visited / V) : 0.f;
+// The same function runs on the host (traces) and in the stats kernel's
+// epilogue (inference).
+__host__ __device__ inline void make_features(const DevStats& s, uint32_t V, uint64_t E,
+                                              uint64_t visited, uint32_t iter, float* f) {
+  const double lv = log2(1.0 + (double)V), le = log2(1.0 + (double)E);
+  f[0] = (float)(log2(1.0 + (double)s.n_active) / (lv > 0 ? lv : 1.0));
+  f[1] = (float)(log2(1.0 + (double)s.m_active) / (le > 0 ? le : 1.0));
+  f[2] = V ? (float)(1.0 - (double)visited / (double)V) : 0.f;
   const double na = s.n_active ? (double)s.n_active : 1.0;
-  for (int k = 0; k < 8; ++k) f[3 + k] = (float)(s.hist[k] / na);
-  f[11] = (float)((iter < 64 ? iter : 64) / 64.0);
+  for (int k = 0; k < 8; ++k) f[3 + k] = (float)((double)s.hist[k] / na);
+  f[11] = (float)((double)(iter < 64 ? iter : 64) / 64.0);
 }
 
 }  // namespace cg

@@ -1,23 +1,29 @@
 // Frontier kernels of the traversal engine, written for gfx950 (wave64).
 //
-// One iteration of a push step is two launches:
-//   k_compact  one lane per 64-bit frontier word.  Active vertices are split by
-//              out-degree into three work queues -- small (edge-balanced inside
-//              a wavefront), medium (one wavefront per vertex) and chunked (a
-//              vertex of degree >= bin_chunk becomes ceil(deg / chunk_edges)
-//              chunks, one wavefront each).  Queue slots are allocated with one
-//              wave-level exclusive scan and one atomic per queue per wavefront.
-//              The kernel also accumulates the frontier statistics the
-//              schedulers read, and zeroes the bitmap the expansion will write.
-//   k_expand   one wavefront per work item; every edge relaxes
-//              value[dst] with a strict-less pre-check and atomic min; an
-//              improvement sets the destination's bit in the next frontier.
+// One push iteration is three launches over the frontier bitmap:
+//   k_stats    one lane per 64-bit frontier word: popcount, out-degree sum and
+//              log-degree histogram, reduced per workgroup into a partials
+//              array; the last workgroup (atomic ticket) reduces the partials
+//              in a fixed order and -- in learned mode -- runs the scheduler
+//              network on the iteration's features (inference fused into the
+//              statistics kernel).  It also zeroes the next bitmap.
+//   k_compact  splits active vertices by out-degree into three work queues:
+//              small (edges spread evenly over a wavefront's lanes), medium
+//              (one wavefront per vertex) and chunked (a vertex of degree >=
+//              bin_chunk becomes ceil(deg / chunk_edges) wavefront items).
+//              Queue slots come from wave-level scans and one atomic per queue
+//              per workgroup.
+//   k_expand   one wavefront per work item; every edge relaxes value[dst] with
+//              a strict-less pre-check and atomic min; an improvement sets the
+//              destination's bit in the next frontier.
 // The relaxation is the reference's edge operation (paths relative to
 // CGgraph/my_project/CGgraph/): device_algorithm.cuh:51-54 (BFS: value+1),
 // :253-257 (SSSP: value+w), host_algorithm.hpp:424-429 (CC: value).  The
 // degree tiers replace WarpDonate::schedule (device_warpDonate.cuh:43-388),
 // whose CTA / warp / half-warp tiers assume 32-lane warps.
 #include "kernels.hpp"
+#include "mlp_dev.hpp"
+#include "sched.hpp"
 
 namespace cg {
 
@@ -27,6 +33,7 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / kWave; }
 
 // Exclusive prefix sum over the 64 lanes; every lane must be active.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t* total) {
@@ -57,6 +64,10 @@ __device__ __forceinline__ uint32_t hist_bin(uint64_t deg) {
   uint32_t l = 63 - __clzll(deg + 1);  // floor(log2(deg + 1))
   l >>= 1;
   return l > 7 ? 7 : l;
+}
+
+__device__ __forceinline__ void stripe_add(uint32_t* stripes, uint32_t v) {
+  if (v) atomicAdd(stripes + (blockIdx.x % kStripes) * kStripeStride, v);
 }
 
 enum { ALG_BFS = CG_BFS, ALG_SSSP = CG_SSSP, ALG_CC = CG_CC };
@@ -118,77 +129,162 @@ __global__ void k_set_root(uint32_t* __restrict__ val, uint64_t* __restrict__ bm
   }
 }
 
-// Frontier statistics: popcount, out-degree sum, log-degree histogram.  Also
-// zeroes the bitmap the coming expansion writes.  One lane per frontier word.
+__global__ void k_init_run_state(RunState* rs, unsigned long long visited) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    rs->visited = visited;
+    rs->iter = 0;
+    rs->pad = 0;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ in,
                                                   uint64_t* __restrict__ zero_me, uint64_t words,
-                                                  DevGraph g, DevStats* __restrict__ st) {
+                                                  DevGraph g, uint64_t E, StatsPart* __restrict__ parts,
+                                                  uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
+                                                  const float* __restrict__ mlp, DevStats* __restrict__ st) {
+  __shared__ StatsPart wpart[kWavesPerBlock];
+  __shared__ bool is_last;
   const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   uint64_t word = 0;
   if (wi < words) {
     word = in[wi];
     if (zero_me) zero_me[wi] = 0;
   }
-  if (__ballot(word != 0) == 0) return;  // wave-uniform exit for empty stretches
-  uint32_t c_zero = 0;
-  uint64_t m = 0;
-  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint64_t t = word; t; t &= t - 1) {
-    const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
-    const uint64_t deg = g.off[v + 1] - g.off[v];
-    m += deg;
-    h[hist_bin(deg)]++;
-    c_zero += deg == 0;
+  StatsPart p{};
+  if (__ballot(word != 0)) {  // wave-uniform
+    uint32_t c_zero = 0;
+    uint64_t m = 0;
+    uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t t = word; t; t &= t - 1) {
+      const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
+      const uint64_t deg = g.off[v + 1] - g.off[v];
+      m += deg;
+      h[hist_bin(deg)]++;
+      c_zero += deg == 0;
+    }
+    p.n_active = wave_sum_u32((uint32_t)__popcll(word));
+    p.m_active = wave_sum_u64(m);
+    p.n_zero = wave_sum_u32(c_zero);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h[k]);
   }
-  const uint32_t act = wave_sum_u32((uint32_t)__popcll(word));
-  const uint64_t msum = wave_sum_u64(m);
-  const uint32_t zsum = wave_sum_u32(c_zero);
-  uint32_t hs[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) hs[k] = wave_sum_u32(h[k]);
-  if (lane_id() == 0) {
-    atomicAdd(&st->n_active, act);
-    atomicAdd(&st->m_active, (unsigned long long)msum);
-    if (zsum) atomicAdd(&st->n_zero_deg, zsum);
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (hs[k]) atomicAdd(&st->hist[k], hs[k]);
+  if (lane_id() == 0) wpart[wave_id()] = p;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    StatsPart b = wpart[0];
+    for (int w = 1; w < kWavesPerBlock; ++w) {
+      b.n_active += wpart[w].n_active;
+      b.n_zero += wpart[w].n_zero;
+      b.m_active += wpart[w].m_active;
+      for (int k = 0; k < 8; ++k) b.hist[k] += wpart[w].hist[k];
+    }
+    parts[blockIdx.x] = b;
+    __threadfence();
+    const uint32_t ticket = atomicAdd(&st->ticket, 1u);
+    is_last = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;  // block-uniform
+  __threadfence();
+  // ---- last workgroup: fixed-order reduction of the partials ----
+  __shared__ unsigned long long s_m[kBlock];
+  __shared__ uint32_t s_c[kBlock][10];
+  unsigned long long m = 0;
+  uint32_t c[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+    const StatsPart q = parts[b];
+    m += q.m_active;
+    c[0] += q.n_active;
+    c[1] += q.n_zero;
+    for (int k = 0; k < 8; ++k) c[2 + k] += q.hist[k];
+  }
+  s_m[threadIdx.x] = m;
+  for (int k = 0; k < 10; ++k) s_c[threadIdx.x][k] = c[k];
+  __syncthreads();
+  for (int w = kBlock / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_m[threadIdx.x] += s_m[threadIdx.x + w];
+      for (int k = 0; k < 10; ++k) s_c[threadIdx.x][k] += s_c[threadIdx.x + w][k];
+    }
+    __syncthreads();
+  }
+  __shared__ float feat[CG_NUM_FEATURES];
+  __shared__ float mlp_sh[2 * kMlpH + kMlpOut];
+  __shared__ float probs[kMlpOut];
+  __shared__ int best;
+  if (threadIdx.x == 0) {
+    uint32_t nn = 0;
+    for (int k = 0; k < kStripes; ++k) {
+      nn += stripes[k * kStripeStride];
+      stripes[k * kStripeStride] = 0;
+    }
+    rs->visited += nn;
+    rs->iter += 1;
+    st->n_active = s_c[0][0];
+    st->n_zero_deg = s_c[0][1];
+    st->m_active = s_m[0];
+    for (int k = 0; k < 8; ++k) st->hist[k] = s_c[0][2 + k];
+    st->n_new = nn;
+    st->visited = rs->visited;
+    st->iter = rs->iter;
+    st->choice = -1;
+    DevStats tmp = *st;
+    make_features(tmp, g.V, E, rs->visited, rs->iter, feat);
+  }
+  __syncthreads();
+  if (mlp) {
+    mlp_forward_block(mlp, feat, mlp_sh, probs, &best);
+    if (threadIdx.x < kMlpOut) st->probs[threadIdx.x] = probs[threadIdx.x];
+    if (threadIdx.x == 0) st->choice = best;
   }
 }
 
-// Splits the frontier into the three work queues.  Slots come from one
-// wave-level exclusive scan and one atomic per queue per wavefront.
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
                                                     DevGraph g, Bins bins,
                                                     uint32_t* __restrict__ q_small,
                                                     uint32_t* __restrict__ q_med,
                                                     uint64_t* __restrict__ q_chunk,
                                                     DevStats* __restrict__ st) {
+  __shared__ uint32_t s_tot[kWavesPerBlock][3];
+  __shared__ uint32_t s_base[kWavesPerBlock][3];
   const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   const uint64_t word = wi < words ? in[wi] : 0;
-  if (__ballot(word != 0) == 0) return;
+  const bool any = __ballot(word != 0) != 0;  // wave-uniform
   uint32_t c_small = 0, c_med = 0, c_chunk = 0;
-  for (uint64_t t = word; t; t &= t - 1) {
-    const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
-    const uint64_t deg = g.off[v + 1] - g.off[v];
-    if (deg == 0) continue;
-    if (deg < bins.bin_wave) c_small++;
-    else if (deg < bins.bin_chunk) c_med++;
-    else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+  uint32_t o_small = 0, o_med = 0, o_chunk = 0;
+  uint32_t t_small = 0, t_med = 0, t_chunk = 0;
+  if (any) {
+    for (uint64_t t = word; t; t &= t - 1) {
+      const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
+      const uint64_t deg = g.off[v + 1] - g.off[v];
+      if (deg == 0) continue;
+      if (deg < bins.bin_wave) c_small++;
+      else if (deg < bins.bin_chunk) c_med++;
+      else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+    }
+    o_small = wave_excl_scan(c_small, &t_small);
+    o_med = wave_excl_scan(c_med, &t_med);
+    o_chunk = wave_excl_scan(c_chunk, &t_chunk);
   }
-  uint32_t t_small, t_med, t_chunk;
-  const uint32_t o_small = wave_excl_scan(c_small, &t_small);
-  const uint32_t o_med = wave_excl_scan(c_med, &t_med);
-  const uint32_t o_chunk = wave_excl_scan(c_chunk, &t_chunk);
-  uint32_t b_small = 0, b_med = 0, b_chunk = 0;
   if (lane_id() == 0) {
-    if (t_small) b_small = atomicAdd(&st->n_small, t_small);
-    if (t_med) b_med = atomicAdd(&st->n_med, t_med);
-    if (t_chunk) b_chunk = atomicAdd(&st->n_chunk, t_chunk);
+    s_tot[wave_id()][0] = t_small;
+    s_tot[wave_id()][1] = t_med;
+    s_tot[wave_id()][2] = t_chunk;
   }
-  b_small = __shfl(b_small, 0, kWave) + o_small;
-  b_med = __shfl(b_med, 0, kWave) + o_med;
-  b_chunk = __shfl(b_chunk, 0, kWave) + o_chunk;
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) { s_base[w][q] = tot; tot += s_tot[w][q]; }
+    uint32_t base = 0;
+    if (tot) base = atomicAdd(q == 0 ? &st->n_small : (q == 1 ? &st->n_med : &st->n_chunk), tot);
+    for (int w = 0; w < kWavesPerBlock; ++w) s_base[w][q] += base;
+  }
+  __syncthreads();
+  if (!any) return;
+  uint32_t b_small = s_base[wave_id()][0] + o_small;
+  uint32_t b_med = s_base[wave_id()][1] + o_med;
+  uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
   for (uint64_t t = word; t; t &= t - 1) {
     const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
     const uint64_t deg = g.off[v + 1] - g.off[v];
@@ -209,13 +305,14 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
                                                    const uint32_t* __restrict__ q_med,
                                                    const uint64_t* __restrict__ q_chunk,
                                                    uint32_t small_limit, uint32_t chunk_edges,
-                                                   DevStats* __restrict__ st) {
+                                                   const DevStats* __restrict__ st,
+                                                   uint32_t* __restrict__ stripes) {
   // The grid is sized from an upper bound; the queue lengths come from the
   // compaction and surplus wavefronts leave at once.
   const uint32_t n_chunk = st->n_chunk, n_med = st->n_med;
   const uint32_t n_small = st->n_small < small_limit ? st->n_small : small_limit;
   const uint32_t lane = lane_id();
-  const uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+  const uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id();
   uint32_t n_new = 0;
   if (item < n_chunk) {
     // a slice of a high-degree vertex: lanes stride its edges
@@ -265,20 +362,20 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
   }
   if constexpr (ALG != ALG_CC) {
     const uint32_t tot = wave_sum_u32(n_new);
-    if (lane == 0 && tot) atomicAdd(&st->n_new, tot);
+    if (lane == 0) stripe_add(stripes, tot);
   }
 }
 
-// Bottom-up BFS (Beamer et al., SC'12): every unreached vertex scans its
-// in-edges for a parent in the current frontier.  One lane per vertex, so a
-// wavefront owns one 64-bit word of the next frontier and writes it whole.
+// Bottom-up BFS (Beamer, Asanovic, Patterson, SC'12): every unreached vertex
+// scans its in-edges for a parent in the current frontier.  One lane per
+// vertex, so a wavefront owns one 64-bit word of the next frontier and writes
+// it whole (no atomics).
 __global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __restrict__ val,
                                                      const uint64_t* __restrict__ in,
                                                      uint64_t* __restrict__ out,
-                                                     DevStats* __restrict__ st) {
+                                                     uint32_t* __restrict__ stripes) {
   const uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   bool found = false;
-  uint64_t mdeg = 0;
   if (v < g.V && val[v] == kUnreached) {
     const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
     for (uint64_t k = s; k < e; ++k) {
@@ -286,40 +383,35 @@ __global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __res
       if ((in[u >> 6] >> (u & 63)) & 1ull) {
         val[v] = val[u] + 1;
         found = true;
-        mdeg = g.off[v + 1] - g.off[v];
         break;
       }
     }
   }
   const uint64_t mask = __ballot(found);
   const uint64_t words = ((uint64_t)g.V + 63) / 64;
-  if (lane_id() == 0 && (v >> 6) < words) out[v >> 6] = mask;
-  if (mask) {
-    const uint64_t ms = wave_sum_u64(mdeg);
-    if (lane_id() == 0) {
-      atomicAdd(&st->n_new, (uint32_t)__popcll(mask));
-      atomicAdd(&st->m_new, (unsigned long long)ms);
-    }
+  if (lane_id() == 0) {
+    if ((v >> 6) < words) out[v >> 6] = mask;
+    stripe_add(stripes, (uint32_t)__popcll(mask));
   }
 }
 
 __global__ void k_apply_candidates(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ vals,
                                    uint32_t n, uint32_t* __restrict__ val, uint64_t* __restrict__ out,
-                                   DevStats* __restrict__ st) {
+                                   uint32_t* __restrict__ stripes) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t applied = 0;
+  uint32_t fresh = 0;
   if (i < n) {
     const uint32_t d = ids[i], c = vals[i];
     if (c < val[d]) {
       const uint32_t old = atomicMin(val + d, c);
       if (c < old) {
         atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
-        applied = 1;
+        fresh = old == kUnreached;
       }
     }
   }
-  const uint32_t tot = wave_sum_u32(applied);
-  if (lane_id() == 0 && tot) atomicAdd(&st->n_cand_applied, tot);
+  const uint32_t tot = wave_sum_u32(fresh);
+  if (lane_id() == 0) stripe_add(stripes, tot);
 }
 
 __global__ void k_gather_values(const uint32_t* __restrict__ ids, uint32_t n,
@@ -330,12 +422,19 @@ __global__ void k_gather_values(const uint32_t* __restrict__ ids, uint32_t n,
 
 __global__ __launch_bounds__(kBlock) void k_reached_edges(DevGraph g, const uint32_t* __restrict__ val,
                                                           unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long s[kWavesPerBlock];
   uint64_t sum = 0;
   for (uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x; v < g.V;
        v += (uint64_t)gridDim.x * kBlock)
     if (val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
   sum = wave_sum_u64(sum);
-  if (lane_id() == 0 && sum) atomicAdd(acc, (unsigned long long)sum);
+  if (lane_id() == 0) s[wave_id()] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += s[w];
+    if (t) atomicAdd(acc + (blockIdx.x % kStripes) * (kStripeStride / 2), t);
+  }
 }
 
 // Allocation-free push step straight from a bitmap: one lane per vertex, so a
@@ -425,11 +524,20 @@ int launch_set_root(uint32_t* val, uint64_t* bm, uint32_t root, hipStream_t s) {
   return CG_OK;
 }
 
-int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
-                 DevStats* st, hipStream_t s) {
-  if (!words) return CG_OK;
-  const uint64_t blocks = (words + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, st);
+int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_run_state, dim3(1), dim3(64), 0, s, rs, (unsigned long long)visited);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+uint64_t stats_parts_needed(uint64_t words) { return (words + kBlock - 1) / kBlock + 1; }
+
+int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
+                 StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp, DevStats* st,
+                 hipStream_t s) {
+  const uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
+  hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, E, parts,
+                     stripes, rs, mlp, st);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
@@ -453,7 +561,7 @@ uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bin
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
-                       hipStream_t s) {
+                       uint32_t* stripes, hipStream_t s) {
   if (!item_bound) return CG_OK;
   const uint64_t blocks = (item_bound + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks >= (1ull << 31)) return set_error(CG_ERR_INTERNAL, "expansion grid too large");
@@ -461,15 +569,15 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
   switch (alg) {
     case CG_BFS:
       hipLaunchKernelGGL(k_expand<ALG_BFS>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st);
+                         small_limit, bins.chunk_edges, st, stripes);
       break;
     case CG_SSSP:
       hipLaunchKernelGGL(k_expand<ALG_SSSP>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st);
+                         small_limit, bins.chunk_edges, st, stripes);
       break;
     case CG_CC:
       hipLaunchKernelGGL(k_expand<ALG_CC>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st);
+                         small_limit, bins.chunk_edges, st, stripes);
       break;
     default:
       return set_error(CG_ERR_UNSUPPORTED, "push expansion: algorithm %d", alg);
@@ -479,20 +587,20 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
 }
 
 int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
-                    DevStats* st, hipStream_t s) {
+                    uint32_t* stripes, hipStream_t s) {
   if (!g.in_off || !g.in_src) return set_error(CG_ERR_INTERNAL, "pull step without in-edges");
   const uint64_t threads = ((uint64_t)g.V + 63) / 64 * 64;
   const uint64_t blocks = (threads + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_pull_bfs, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, st);
+  hipLaunchKernelGGL(k_pull_bfs, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, stripes);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
 
 int launch_apply_candidates(const uint32_t* ids, const uint32_t* vals, uint32_t n, uint32_t* val,
-                            uint64_t* out, DevStats* st, hipStream_t s) {
+                            uint64_t* out, uint32_t* stripes, hipStream_t s) {
   if (!n) return CG_OK;
   hipLaunchKernelGGL(k_apply_candidates, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ids,
-                     vals, n, val, out, st);
+                     vals, n, val, out, stripes);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

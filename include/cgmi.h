@@ -70,7 +70,10 @@ enum {
   CG_SCHED_HEURISTIC = 0, /* the reference's LIGHT/HEAVY rule (CGgraph_engine.cuh:312-348) */
   CG_SCHED_GPU = 1,       /* every iteration on the GPU, push with direction switching for BFS */
   CG_SCHED_LEARNED = 2,   /* MLP over frontier statistics, heuristic as fallback */
-  CG_SCHED_FORCED = 3     /* cg_options.forced_choice every iteration (tests, trace collection) */
+  CG_SCHED_FORCED = 3,    /* cg_options.forced_choice every iteration (tests) */
+  CG_SCHED_PROBE = 4      /* trace collection: every iteration times all 16 choices from the
+                             same state (restored in between), writes them to trace_path,
+                             then continues with the CG_SCHED_GPU rule */
 };
 
 #define CG_NUM_CHOICES 16
