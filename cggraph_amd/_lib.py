@@ -45,6 +45,10 @@ class Options(ctypes.Structure):
         ("trace_path", ctypes.c_char_p),
         ("mlp_path", ctypes.c_char_p),
         ("fault_flags", ctypes.c_uint32),
+        ("batch_block", ctypes.c_uint32),
+        ("batch_grid", ctypes.c_uint32),
+        ("batch_levels", ctypes.c_uint32),
+        ("sssp_delta", ctypes.c_uint32),
     ]
 
 
@@ -111,6 +115,7 @@ SIGNATURES = {
     "cg_mlp_loss_grad": (_int, [_vp, _vp, _vp, _int, _vp, _vp]),
     "cg_check_result_file": (_int, [_cp, _vp, _u32, _vp, ctypes.POINTER(_u64)]),
     "cg_save_bin_u32": (_int, [_cp, _vp, _u64]),
+    "cg_abi_sizes": (_int, [ctypes.POINTER(ctypes.c_size_t)] * 3),
     "cg_last_error": (_cp, []),
     "cg_version": (_cp, []),
 }

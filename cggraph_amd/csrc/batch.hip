@@ -1,8 +1,435 @@
-// Batched BFS/SSSP queries in one persistent kernel (filled in below).
+// Batched BFS / SSSP queries over graphs of uneven size.
+//
+// All queries advance level by level together.  A level is two launches:
+//   k_batch_plan    one workgroup: exclusive scan of the per-query frontier
+//                   sizes into a global item space, clear of the next
+//                   level's counters, per-query delta-stepping threshold
+//                   update, done flag when nothing is left;
+//   k_batch_expand  a fixed grid of workgroups whose wavefronts stride over
+//                   groups of 64 items; an item is one frontier vertex of one
+//                   query, and the 64 items' edges are spread evenly over the
+//                   lanes (owner found by binary search over the wave's degree
+//                   prefix) whatever query or graph each item belongs to.  An
+//                   improved vertex joins its query's next frontier once,
+//                   guarded by a per-query level stamp.  For SSSP a vertex
+//                   whose distance is not below the query's threshold is
+//                   carried to the next frontier unexpanded (delta-stepping,
+//                   Meyer and Sanders 2003); the threshold moves past the
+//                   smallest waiting distance when nothing below it is left.
+// The launch arguments never change from level to level (the level number,
+// parity, thresholds and item count live in device memory), so
+// `batch_levels` levels are captured once into a HIP graph and the graph is
+// replayed until the done flag is set -- one host synchronisation per replay
+// instead of one per level per query.  mode 1 of cg_batch_run runs the same
+// queries one per engine launch for comparison.
+//
+// The relaxation is the reference's (my_project/CGgraph/device_algorithm.cuh:
+// 51-54 BFS, 253-257 SSSP), so each query ends at its unique fixed point,
+// bit-identical to a single-query run.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <vector>
+
 #include "engine.hpp"
+
+namespace cg {
+
+namespace {
+
+struct QueryDesc {
+  const uint64_t* off;
+  const uint32_t* dst;
+  const uint32_t* w;
+  uint32_t* val;
+  uint32_t* stamp;
+  uint32_t* front[2];
+  uint32_t V;
+  int alg;
+  uint32_t root;
+  uint32_t delta;  // 0: no deferral
+};
+
+struct BatchState {
+  uint32_t next_level;   // level the next plan prepares
+  uint32_t cur_level;    // level the running expansion processes
+  uint32_t done;
+  uint32_t total;        // items of the current level
+  uint32_t levels_run;   // levels with work
+  uint32_t pad[3];
+};
+
+constexpr int kPlanBlock = 1024;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+__global__ void k_batch_init(const QueryDesc* __restrict__ q, int nq, const uint64_t* __restrict__ vpre,
+                             uint64_t total) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nq - 1;  // largest query with vpre[q] <= i
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (vpre[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t v = (uint32_t)(i - vpre[lo]);
+    q[lo].val[v] = v == q[lo].root ? 0u : kUnreached;
+    q[lo].stamp[v] = 0;
+    if (v == q[lo].root) q[lo].front[0][0] = v;
+  }
+}
+
+// cnt[2][nq] frontier sizes, qs = thr[nq] | minv[2][nq] (smallest value
+// appended to each frontier).
+__global__ void k_batch_reset(const QueryDesc* __restrict__ qd, uint32_t* __restrict__ cnt,
+                              uint32_t* __restrict__ qs, int nq, BatchState* __restrict__ bs) {
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) {
+    cnt[i] = 1;  // level 0: the root
+    cnt[nq + i] = 0;
+    qs[i] = qd[i].delta ? qd[i].delta : kNone;
+    qs[nq + i] = 0;
+    qs[2 * nq + i] = kNone;
+  }
+  if (threadIdx.x == 0) {
+    bs->next_level = 0;
+    bs->cur_level = 0;
+    bs->done = 0;
+    bs->total = 0;
+    bs->levels_run = 0;
+  }
+}
+
+__global__ __launch_bounds__(kPlanBlock) void k_batch_plan(const QueryDesc* __restrict__ qd,
+                                                           uint32_t* __restrict__ cnt, uint32_t* __restrict__ qs,
+                                                           int nq, uint32_t* __restrict__ item_off,
+                                                           BatchState* __restrict__ bs) {
+  __shared__ uint32_t wsum[kPlanBlock / 64];
+  __shared__ uint32_t carry;
+  const uint32_t L = bs->next_level;
+  const uint32_t* cur = cnt + (L & 1) * nq;
+  uint32_t* nxt = cnt + ((L + 1) & 1) * nq;
+  uint32_t* thr = qs;
+  const uint32_t* minv_cur = qs + nq + (L & 1) * nq;
+  uint32_t* minv_nxt = qs + nq + ((L + 1) & 1) * nq;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nq; base += kPlanBlock) {
+    const int i = base + threadIdx.x;
+    const uint32_t x = i < nq ? cur[i] : 0u;
+    // block exclusive scan: wave scan + wave totals
+    uint32_t v = x;
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(v, d, 64);
+      if (lane >= (uint32_t)d) v += t;
+    }
+    if (lane == 63) wsum[threadIdx.x / 64] = v;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (int w = 0; w < (int)(threadIdx.x / 64); ++w) wpre += wsum[w];
+    if (i < nq) item_off[i] = carry + wpre + v - x;
+    __syncthreads();
+    if (threadIdx.x == kPlanBlock - 1) carry += wpre + v;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < nq; i += kPlanBlock) {
+    nxt[i] = 0;
+    minv_nxt[i] = kNone;
+    const uint32_t d = qd[i].delta;
+    if (d && cur[i] && minv_cur[i] >= thr[i]) {
+      // nothing of this frontier lies below the threshold: move it past the
+      // smallest waiting distance
+      const uint64_t t = (uint64_t)minv_cur[i] + d;
+      thr[i] = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+    }
+  }
+  if (threadIdx.x == 0) {
+    item_off[nq] = carry;
+    bs->total = carry;
+    bs->cur_level = L;
+    bs->next_level = L + 1;
+    if (carry == 0) bs->done = 1;
+    else bs->levels_run += 1;
+  }
+}
+
+// Appends the flagged lanes' vertices to their queries' next frontiers with
+// one atomic per distinct query in the wavefront, and folds their values into
+// the queries' frontier minima.
+__device__ __forceinline__ void wave_append(bool push, int q, uint32_t v, uint32_t value,
+                                            const QueryDesc* __restrict__ qd, uint32_t* __restrict__ nxt_cnt,
+                                            uint32_t* __restrict__ minv_nxt, int np, uint32_t lane) {
+  uint64_t pending = __ballot(push);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lq = __shfl(q, leader, 64);
+    const bool mine = push && q == lq;
+    const uint64_t same = __ballot(mine);
+    uint32_t mn = mine ? value : kNone;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const uint32_t o = __shfl_xor(mn, d, 64);
+      mn = o < mn ? o : mn;
+    }
+    uint32_t pos = 0;
+    if ((int)lane == leader) {
+      pos = atomicAdd(nxt_cnt + lq, (uint32_t)__popcll(same));
+      if (qd[lq].delta) atomicMin(minv_nxt + lq, mn);
+    }
+    pos = __shfl(pos, leader, 64);
+    if (mine) qd[lq].front[np][pos + (uint32_t)__popcll(same & ((1ull << lane) - 1))] = v;
+    pending &= ~same;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restrict__ qd, int nq,
+                                                       const uint32_t* __restrict__ item_off,
+                                                       uint32_t* __restrict__ cnt, const uint32_t* __restrict__ qs,
+                                                       uint32_t* __restrict__ qs_mut, BatchState* __restrict__ bs) {
+  const uint32_t total = bs->total;
+  if (total == 0) return;
+  const uint32_t L = bs->cur_level;
+  const int cp = L & 1, np = cp ^ 1;
+  uint32_t* nxt_cnt = cnt + np * nq;
+  const uint32_t* thr = qs;
+  uint32_t* minv_nxt = qs_mut + nq + np * nq;
+  const uint32_t lane = __lane_id();
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  // wavefronts stride over 64-item groups (no shared cursor to contend on)
+  for (uint64_t grp = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; grp * 64 < total; grp += waves) {
+    const uint32_t i = (uint32_t)(grp * 64) + lane;
+    int q = 0;
+    uint32_t deg = 0, sv = 0, v = 0;
+    uint64_t s = 0;
+    bool carry = false;
+    if (i < total) {
+      int lo = 0, hi = nq - 1;  // largest q with item_off[q] <= i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (item_off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+      }
+      q = lo;
+      v = qd[q].front[cp][i - item_off[q]];
+      sv = qd[q].val[v];
+      if (sv >= thr[q]) {
+        // waits for a later bucket: keep it in the frontier, expand nothing
+        carry = atomicMax(qd[q].stamp + v, L + 1) < L + 1;
+      } else {
+        s = qd[q].off[v];
+        deg = (uint32_t)(qd[q].off[v + 1] - s);
+      }
+    }
+    wave_append(carry, q, v, sv, qd, nxt_cnt, minv_nxt, np, lane);
+    // exclusive scan of degrees over the wave
+    uint32_t excl = deg;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(excl, d, 64);
+      if (lane >= (uint32_t)d) excl += t;
+    }
+    const uint32_t tot = __shfl(excl, 63, 64);
+    excl -= deg;
+    for (uint32_t j0 = 0; j0 < tot; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      uint32_t lo = 0;
+#pragma unroll
+      for (uint32_t step = 32; step >= 1; step >>= 1) {
+        const uint32_t pm = __shfl(excl, (int)(lo + step), 64);
+        if (pm <= j) lo += step;
+      }
+      const int oq = __shfl(q, (int)lo, 64);
+      const uint64_t os = __shfl(s, (int)lo, 64);
+      const uint32_t osv = __shfl(sv, (int)lo, 64);
+      const uint32_t oex = __shfl(excl, (int)lo, 64);
+      bool push = false;
+      uint32_t d = 0, cand = 0;
+      if (j < tot) {
+        const QueryDesc& Q = qd[oq];
+        const uint64_t e = os + (j - oex);
+        d = Q.dst[e];
+        cand = Q.alg == CG_BFS ? osv + 1 : osv + Q.w[e];
+        if (cand < Q.val[d]) {
+          const uint32_t old = atomicMin(Q.val + d, cand);
+          if (cand < old) push = atomicMax(Q.stamp + d, L + 1) < L + 1;
+        }
+      }
+      wave_append(push, oq, d, cand, qd, nxt_cnt, minv_nxt, np, lane);
+    }
+  }
+}
+
+__global__ void k_batch_reached(const QueryDesc* __restrict__ qd, int q, unsigned long long* __restrict__ acc) {
+  const QueryDesc& Q = qd[q];
+  unsigned long long sum = 0;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < Q.V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    if (Q.val[v] != kUnreached) sum += Q.off[v + 1] - Q.off[v];
+  for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+  if (__lane_id() == 0 && sum) atomicAdd(acc + (q % 64) * 16, sum);
+}
+
+#define BATCH_LAUNCH_OK()                                                                 \
+  do {                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                    \
+    if (e_ != hipSuccess)                                                                 \
+      return set_error(CG_ERR_HIP, "batch kernel launch: %s (%s:%d)", hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                               \
+  } while (0)
+
+struct GraphExec {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~GraphExec() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots, int nq, uint32_t* const* outs,
+                const cg_options& opt, cg_run_stats* rs) {
+  CG_TRY(require_device());
+  const uint32_t block = opt.batch_block ? opt.batch_block : 256;
+  const uint32_t grid = opt.batch_grid ? opt.batch_grid : 2048;
+  const uint32_t levels = opt.batch_levels ? opt.batch_levels : 16;
+  if (block % 64 || block > 1024) return set_error(CG_ERR_INVALID, "batch_block must be a multiple of 64, <= 1024");
+  for (int i = 0; i < nq; ++i) {
+    if (!graphs[i]) return set_error(CG_ERR_INVALID, "query %d has no graph", i);
+    if (algs[i] != CG_BFS && algs[i] != CG_SSSP)
+      return set_error(CG_ERR_UNSUPPORTED, "query %d: batched queries are BFS or SSSP", i);
+    if (algs[i] == CG_SSSP && !graphs[i]->has_w) return set_error(CG_ERR_INVALID, "query %d: SSSP needs weights", i);
+    if (roots[i] >= graphs[i]->V) return set_error(CG_ERR_INVALID, "query %d: root %u >= V", i, roots[i]);
+    CG_TRY(graphs[i]->ensure_device());
+  }
+  hipStream_t s = nullptr;
+  CG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  std::unique_ptr<std::remove_pointer<hipStream_t>::type, decltype(&hipStreamDestroy)> sguard(s, hipStreamDestroy);
+  // per-query device arrays: values, stamps and two frontier queues
+  std::vector<uint64_t> vpre(nq + 1, 0);
+  for (int i = 0; i < nq; ++i) vpre[i + 1] = vpre[i] + graphs[i]->V;
+  const uint64_t totalV = vpre[nq];
+  DevBuf<uint32_t> pool;
+  CG_TRY(pool.ensure(4 * totalV));
+  std::vector<QueryDesc> hq(nq);
+  for (int i = 0; i < nq; ++i) {
+    cg_graph* g = graphs[i];
+    uint32_t* b = pool.p + 4 * vpre[i];
+    const uint32_t delta = algs[i] == CG_SSSP ? opt.sssp_delta : 0;
+    hq[i] = QueryDesc{g->d_off.p, g->d_dst.p, g->has_w ? g->d_w.p : nullptr, b, b + g->V,
+                      {b + 2 * (uint64_t)g->V, b + 3 * (uint64_t)g->V}, g->V, algs[i], roots[i], delta};
+  }
+  DevBuf<QueryDesc> dq;
+  DevBuf<uint64_t> dvpre;
+  DevBuf<uint32_t> cnt, item_off, qs;
+  DevBuf<BatchState> bs;
+  CG_TRY(dq.ensure(nq));
+  CG_TRY(dvpre.ensure(nq + 1));
+  CG_TRY(cnt.ensure(2 * (uint64_t)nq));
+  CG_TRY(qs.ensure(3 * (uint64_t)nq));
+  CG_TRY(item_off.ensure(nq + 1));
+  CG_TRY(bs.ensure(1));
+  CG_HIP(hipMemcpyAsync(dq.p, hq.data(), nq * sizeof(QueryDesc), hipMemcpyHostToDevice, s));
+  CG_HIP(hipMemcpyAsync(dvpre.p, vpre.data(), (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  CG_HIP(hipStreamSynchronize(s));
+
+  // capture `levels` levels once
+  GraphExec ge;
+  CG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (uint32_t l = 0; l < levels; ++l) {
+    hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanBlock), 0, s, dq.p, cnt.p, qs.p, nq, item_off.p, bs.p);
+    hipLaunchKernelGGL(k_batch_expand, dim3(grid), dim3(block), 0, s, dq.p, nq, item_off.p, cnt.p, qs.p, qs.p,
+                       bs.p);
+  }
+  hipError_t ce = hipStreamEndCapture(s, &ge.graph);
+  if (ce != hipSuccess) return set_error(CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+  CG_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+
+  const auto t0 = std::chrono::steady_clock::now();
+  const unsigned ig = (unsigned)std::min<uint64_t>((totalV + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_batch_init, dim3(ig ? ig : 1), dim3(256), 0, s, dq.p, nq, dvpre.p, totalV);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1024), 0, s, dq.p, cnt.p, qs.p, nq, bs.p);
+  BATCH_LAUNCH_OK();
+  BatchState hb{};
+  uint32_t replays = 0;
+  for (;;) {
+    CG_HIP(hipGraphLaunch(ge.exec, s));
+    ++replays;
+    CG_HIP(hipMemcpyAsync(&hb, bs.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+    CG_HIP(hipStreamSynchronize(s));
+    if (hb.done) break;
+    if (replays > (1u << 22)) return set_error(CG_ERR_INTERNAL, "batch did not converge");
+  }
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+
+  // TEPS numerator per query and the outputs, outside the timed region
+  DevBuf<unsigned long long> acc;
+  CG_TRY(acc.ensure(64 * 16));
+  CG_HIP(hipMemsetAsync(acc.p, 0, 64 * 16 * sizeof(unsigned long long), s));
+  for (int i = 0; i < nq; ++i) {
+    const unsigned bg = (unsigned)std::min<uint64_t>(((uint64_t)hq[i].V + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_batch_reached, dim3(bg ? bg : 1), dim3(256), 0, s, dq.p, i, acc.p);
+  }
+  BATCH_LAUNCH_OK();
+  std::vector<unsigned long long> hacc(64 * 16);
+  CG_HIP(hipMemcpyAsync(hacc.data(), acc.p, hacc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  for (int i = 0; i < nq; ++i)
+    if (outs && outs[i])
+      CG_HIP(hipMemcpyAsync(outs[i], hq[i].val, (size_t)hq[i].V * 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  if (rs) {
+    std::memset(rs, 0, sizeof(*rs));
+    rs->ms = ms;
+    rs->iterations = hb.levels_run;
+    rs->expand_launches = replays;
+    unsigned long long m = 0;
+    for (auto x : hacc) m += x;
+    rs->edges_traversed = m;
+  }
+  return CG_OK;
+}
+
+int run_one_per_launch(cg_graph* const* graphs, const int* algs, const uint32_t* roots, int nq,
+                       uint32_t* const* outs, const cg_options& opt, cg_run_stats* rs) {
+  cg_options o = opt;
+  o.sched_mode = CG_SCHED_GPU;
+  o.trace_path = nullptr;
+  double ms = 0;
+  uint64_t m = 0;
+  uint32_t iters = 0;
+  for (int i = 0; i < nq; ++i) {
+    cg_engine* e = nullptr;
+    CG_TRY(cg_engine_create(graphs[i], algs[i], CG_COORDINATION, 1, &o, &e));
+    std::unique_ptr<cg_engine, decltype(&cg_engine_destroy)> eg(e, cg_engine_destroy);
+    cg_run_stats st{};
+    CG_TRY(cg_engine_run(e, roots[i], nullptr, &st));
+    ms += st.ms;
+    m += st.edges_traversed;
+    iters += st.iterations;
+    if (outs && outs[i]) CG_TRY(cg_engine_result(e, outs[i]));
+  }
+  if (rs) {
+    std::memset(rs, 0, sizeof(*rs));
+    rs->ms = ms;
+    rs->edges_traversed = m;
+    rs->iterations = iters;
+    rs->expand_launches = (uint32_t)nq;
+  }
+  return CG_OK;
+}
+
+}  // namespace
+
+}  // namespace cg
 
 extern "C" int cg_batch_run(cg_graph* const* graphs, const int* algorithms, const uint32_t* roots, int nq,
                             uint32_t* const* outs, int mode, const cg_options* opt, cg_run_stats* stats) {
-  (void)graphs; (void)algorithms; (void)roots; (void)nq; (void)outs; (void)mode; (void)opt; (void)stats;
-  return cg::set_error(CG_ERR_UNSUPPORTED, "batched queries are not built yet");
+  cg::clear_error();
+  if (!graphs || !algorithms || !roots || nq <= 0) return cg::set_error(CG_ERR_INVALID, "bad batch arguments");
+  cg_options o;
+  if (opt) o = *opt;
+  else cg_options_default(&o);
+  if (mode == 0) return cg::run_batched(graphs, algorithms, roots, nq, outs, o, stats);
+  if (mode == 1) return cg::run_one_per_launch(graphs, algorithms, roots, nq, outs, o, stats);
+  return cg::set_error(CG_ERR_INVALID, "batch mode %d", mode);
 }

@@ -186,6 +186,10 @@ int cg_options_default(cg_options* o) {
   o->trace_path = nullptr;
   o->mlp_path = nullptr;
   o->fault_flags = 0;
+  o->batch_block = 256;
+  o->batch_grid = 2048;
+  o->batch_levels = 16;
+  o->sssp_delta = 0;
   return CG_OK;
 }
 
@@ -545,6 +549,13 @@ int cg_save_bin_u32(const char* path, const uint32_t* values, uint64_t n) {
   const size_t wr = n ? std::fwrite(values, 4, n, f) : 0;
   std::fclose(f);
   if (wr != n) return set_error(CG_ERR_IO, "short write to %s", path);
+  return CG_OK;
+}
+
+int cg_abi_sizes(size_t* options, size_t* run_stats, size_t* partition_info) {
+  if (options) *options = sizeof(cg_options);
+  if (run_stats) *run_stats = sizeof(cg_run_stats);
+  if (partition_info) *partition_info = sizeof(cg_partition_info);
   return CG_OK;
 }
 

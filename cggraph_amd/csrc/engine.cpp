@@ -158,12 +158,14 @@ bool probs_usable(const DevStats& s) {
 
 struct Step {
   Choice ch;
-  int host_share;  // 0 GPU only, 1 host team only, 2 co-executed
+  int host_share;      // 0 GPU only, 1 host team only, 2 co-executed
+  uint32_t threshold;  // delta-stepping: GPU pushes expand only values below it
 };
 
 // Executes one iteration's expansion from frontier `in` into `out`.
 int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats& sst, DevStats* dst,
               uint64_t* in, uint64_t* out, bool* timing_pending, cg_run_stats* local) {
+  const uint32_t* tval = step.threshold != 0xFFFFFFFFu ? e->val.p : nullptr;
   hipStream_t s = e->stream;
   const uint64_t words = ((uint64_t)dg.V + 63) / 64;
   const int alg = e->alg;
@@ -183,7 +185,8 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
     // LIGHT: the host team relaxes the whole frontier
     ++local->light_iterations;
     const Bins all_small{0xFFFFFFFFu, 0xFFFFFFFFu, e->opt.chunk_edges};
-    CG_TRY(launch_compact(in, words, dg, all_small, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, s));
+    CG_TRY(launch_compact(in, words, dg, all_small, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, nullptr,
+                          0xFFFFFFFFu, nullptr, s));
     const uint32_t n = sst.n_active - sst.n_zero_deg;
     CG_TRY(queue_share_copy(e, 0, n));
     CG_HIP(hipEventSynchronize(e->ev_copy));
@@ -196,7 +199,8 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
     bins.bin_wave = kTierSets[step.ch.tier].bin_wave;
     bins.bin_chunk = kTierSets[step.ch.tier].bin_chunk;
   }
-  CG_TRY(launch_compact(in, words, dg, bins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, s));
+  CG_TRY(launch_compact(in, words, dg, bins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, tval,
+                        step.threshold, out, s));
   uint32_t small_limit = 0xFFFFFFFFu, n_host = 0;
   if (step.host_share == 2) {
     // Split the small tier by measured throughput (the analytical split of
@@ -235,7 +239,7 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
 // CG_SCHED_PROBE: time every choice from the current state, restoring the
 // values, the frontier, the new-vertex counters and the host shadow after each.
 int probe_all(cg_engine* e, const DevGraph& dg, const DevStats& sst, DevStats* dst, uint64_t* in,
-              uint64_t* out, double* us) {
+              uint64_t* out, uint32_t threshold, double* us) {
   hipStream_t s = e->stream;
   const uint32_t V = dg.V;
   const uint64_t words = ((uint64_t)V + 63) / 64;
@@ -246,7 +250,7 @@ int probe_all(cg_engine* e, const DevGraph& dg, const DevStats& sst, DevStats* d
   std::vector<uint32_t> shadow_snap = e->team.shadow;
   CG_HIP(hipStreamSynchronize(s));
   for (int c = 0; c < CG_NUM_CHOICES; ++c) {
-    Step st{decode_choice(c), 0};
+    Step st{decode_choice(c), 0, threshold};
     if (e->alg != CG_BFS) st.ch.pull = 0;
     st.host_share = (st.ch.host && !st.ch.pull) ? 1 : 0;
     cg_run_stats scratch{};
@@ -310,6 +314,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   uint64_t m_explored = 0;
   bool timing_pending = false;
   IterRecord rec{};
+  // delta-stepping threshold: frontier vertices at or above it wait
+  const uint32_t delta = alg == CG_SSSP ? e->opt.sssp_delta : 0;
+  uint32_t threshold = delta ? delta : 0xFFFFFFFFu;
+  const uint32_t* tval = delta ? e->val.p : nullptr;
   auto tit = clk::now();
   for (;;) {
     ++it;
@@ -317,7 +325,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     DevStats* dst = e->st.p + slot;
     CG_HIP(hipMemsetAsync(dst, 0, sizeof(DevStats), s));
     CG_TRY(launch_stats(e->bm[cur].p, e->bm[cur ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p,
-                        e->run_state.p, mlp_dev, dst, s));
+                        e->run_state.p, mlp_dev, tval, threshold, dst, s));
     CG_HIP(hipMemcpyAsync(e->hst.p + slot, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     CG_HIP(hipStreamSynchronize(s));
     if (e->opt.fault_flags & CG_FAULT_HIP_ERROR)
@@ -335,10 +343,15 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     }
     tit = clk::now();
     if (sst.n_active == 0) break;
+    if (delta && sst.n_near == 0) {
+      // nothing below the threshold: move it past the smallest waiting value
+      const uint64_t t = (uint64_t)sst.min_val + delta;
+      threshold = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+    }
 
     // ---- schedule ----
     const uint64_t m_unexplored = g.E > m_explored ? g.E - m_explored : 0;
-    Step step{Choice{0, 0, 0}, 0};
+    Step step{Choice{0, 0, 0}, 0, threshold};
     int took_learned = 0;
     if (mode == CG_SCHED_HEURISTIC) {
       step.host_share = heuristic_heavy(e, sst) ? 2 : 1;
@@ -375,7 +388,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     uint64_t* out = e->bm[cur ^ 1].p;
     if (probe) {
       double us[CG_NUM_CHOICES];
-      CG_TRY(probe_all(e, dg, sst, dst, in, out, us));
+      CG_TRY(probe_all(e, dg, sst, dst, in, out, threshold, us));
       write_probe(e, rec, root, us);
     }
     CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));

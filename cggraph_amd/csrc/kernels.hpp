@@ -23,6 +23,8 @@ struct DevStats {
   int32_t choice;              // learned-scheduler argmax, -1 when it did not run
   uint32_t iter;               // iteration index (1-based) seen by k_stats
   float probs[16];             // learned-scheduler softmax
+  uint32_t min_val;            // smallest value on the frontier (threshold runs only)
+  uint32_t n_near;             // frontier vertices below the threshold
 };
 static_assert(sizeof(DevStats) % 8 == 0, "DevStats layout");
 
@@ -43,6 +45,7 @@ struct StatsPart {
   uint32_t n_active, n_zero;
   unsigned long long m_active;
   uint32_t hist[8];
+  uint32_t min_val, n_near;
 };
 
 struct DevGraph {
@@ -75,14 +78,18 @@ int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s);
 // counters of the previous expansion (`stripes`, zeroed again) into
 // rs->visited, and -- when `mlp` is not NULL -- runs the learned scheduler on
 // the iteration's features and writes choice/probs into st.
+// With `val` not NULL it also reports the frontier's smallest value and how
+// many frontier vertices lie below `threshold` (delta-stepping runs).
 uint64_t stats_parts_needed(uint64_t words);
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
                  uint64_t E, StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
-                 DevStats* st, hipStream_t s);
+                 const uint32_t* val, uint32_t threshold, DevStats* st, hipStream_t s);
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
+// Frontier vertices whose value is >= threshold are not queued but carried
+// into `carry` (the next frontier) unchanged.
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                   hipStream_t s);
+                   const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s);
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);

@@ -141,7 +141,9 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
                                                   uint64_t* __restrict__ zero_me, uint64_t words,
                                                   DevGraph g, uint64_t E, StatsPart* __restrict__ parts,
                                                   uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
-                                                  const float* __restrict__ mlp, DevStats* __restrict__ st) {
+                                                  const float* __restrict__ mlp,
+                                                  const uint32_t* __restrict__ val, uint32_t threshold,
+                                                  DevStats* __restrict__ st) {
   __shared__ StatsPart wpart[kWavesPerBlock];
   __shared__ bool is_last;
   const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
@@ -151,8 +153,9 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
     if (zero_me) zero_me[wi] = 0;
   }
   StatsPart p{};
+  p.min_val = 0xFFFFFFFFu;
   if (__ballot(word != 0)) {  // wave-uniform
-    uint32_t c_zero = 0;
+    uint32_t c_zero = 0, c_near = 0, mn = 0xFFFFFFFFu;
     uint64_t m = 0;
     uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint64_t t = word; t; t &= t - 1) {
@@ -161,12 +164,26 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
       m += deg;
       h[hist_bin(deg)]++;
       c_zero += deg == 0;
+      if (val) {
+        const uint32_t x = val[v];
+        mn = x < mn ? x : mn;
+        c_near += x < threshold;
+      }
     }
     p.n_active = wave_sum_u32((uint32_t)__popcll(word));
     p.m_active = wave_sum_u64(m);
     p.n_zero = wave_sum_u32(c_zero);
 #pragma unroll
     for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h[k]);
+    if (val) {
+      p.n_near = wave_sum_u32(c_near);
+#pragma unroll
+      for (int d = kWave / 2; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(mn, d, kWave);
+        mn = o < mn ? o : mn;
+      }
+      p.min_val = mn;
+    }
   }
   if (lane_id() == 0) wpart[wave_id()] = p;
   __syncthreads();
@@ -177,6 +194,8 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
       b.n_zero += wpart[w].n_zero;
       b.m_active += wpart[w].m_active;
       for (int k = 0; k < 8; ++k) b.hist[k] += wpart[w].hist[k];
+      b.n_near += wpart[w].n_near;
+      b.min_val = wpart[w].min_val < b.min_val ? wpart[w].min_val : b.min_val;
     }
     parts[blockIdx.x] = b;
     __threadfence();
@@ -188,23 +207,30 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
   __threadfence();
   // ---- last workgroup: fixed-order reduction of the partials ----
   __shared__ unsigned long long s_m[kBlock];
-  __shared__ uint32_t s_c[kBlock][10];
+  __shared__ uint32_t s_c[kBlock][11];
+  __shared__ uint32_t s_min[kBlock];
   unsigned long long m = 0;
-  uint32_t c[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t c[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t mn = 0xFFFFFFFFu;
   for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
     const StatsPart q = parts[b];
     m += q.m_active;
     c[0] += q.n_active;
     c[1] += q.n_zero;
     for (int k = 0; k < 8; ++k) c[2 + k] += q.hist[k];
+    c[10] += q.n_near;
+    mn = q.min_val < mn ? q.min_val : mn;
   }
   s_m[threadIdx.x] = m;
-  for (int k = 0; k < 10; ++k) s_c[threadIdx.x][k] = c[k];
+  for (int k = 0; k < 11; ++k) s_c[threadIdx.x][k] = c[k];
+  s_min[threadIdx.x] = mn;
   __syncthreads();
   for (int w = kBlock / 2; w >= 1; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s_m[threadIdx.x] += s_m[threadIdx.x + w];
-      for (int k = 0; k < 10; ++k) s_c[threadIdx.x][k] += s_c[threadIdx.x + w][k];
+      for (int k = 0; k < 11; ++k) s_c[threadIdx.x][k] += s_c[threadIdx.x + w][k];
+      const uint32_t o = s_min[threadIdx.x + w];
+      if (o < s_min[threadIdx.x]) s_min[threadIdx.x] = o;
     }
     __syncthreads();
   }
@@ -225,6 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
     st->m_active = s_m[0];
     for (int k = 0; k < 8; ++k) st->hist[k] = s_c[0][2 + k];
     st->n_new = nn;
+    st->n_near = s_c[0][10];
+    st->min_val = s_min[0];
     st->visited = rs->visited;
     st->iter = rs->iter;
     st->choice = -1;
@@ -244,11 +272,26 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
                                                     uint32_t* __restrict__ q_small,
                                                     uint32_t* __restrict__ q_med,
                                                     uint64_t* __restrict__ q_chunk,
-                                                    DevStats* __restrict__ st) {
+                                                    DevStats* __restrict__ st,
+                                                    const uint32_t* __restrict__ val, uint32_t threshold,
+                                                    uint64_t* __restrict__ carry) {
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
-  const uint64_t word = wi < words ? in[wi] : 0;
+  uint64_t word = wi < words ? in[wi] : 0;
+  if (val && word) {
+    // delta-stepping: vertices at or above the threshold wait in the next
+    // frontier; this lane owns word wi of both bitmaps during the compaction
+    uint64_t far = 0;
+    for (uint64_t t = word; t; t &= t - 1) {
+      const int b = __builtin_ctzll(t);
+      if (val[wi * 64 + b] >= threshold) far |= 1ull << b;
+    }
+    if (far) {
+      carry[wi] = far;
+      word &= ~far;
+    }
+  }
   const bool any = __ballot(word != 0) != 0;  // wave-uniform
   uint32_t c_small = 0, c_med = 0, c_chunk = 0;
   uint32_t o_small = 0, o_med = 0, o_chunk = 0;
@@ -533,22 +576,22 @@ int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s) {
 uint64_t stats_parts_needed(uint64_t words) { return (words + kBlock - 1) / kBlock + 1; }
 
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
-                 StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp, DevStats* st,
-                 hipStream_t s) {
+                 StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
+                 const uint32_t* val, uint32_t threshold, DevStats* st, hipStream_t s) {
   const uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
   hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, E, parts,
-                     stripes, rs, mlp, st);
+                     stripes, rs, mlp, val, threshold, st);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
 
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                   hipStream_t s) {
+                   const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s) {
   if (!words) return CG_OK;
   const uint64_t blocks = (words + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, words, g, bins,
-                     q_small, q_med, q_chunk, st);
+                     q_small, q_med, q_chunk, st, val, threshold, carry);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

@@ -98,6 +98,12 @@ typedef struct cg_options {
   const char* trace_path;         /* per-iteration JSONL trace, NULL = off */
   const char* mlp_path;           /* learned-scheduler weights, NULL = built-in untrained */
   uint32_t fault_flags;           /* CG_FAULT_* (tests only) */
+  uint32_t batch_block;           /* cg_batch_run: threads per workgroup (256 / 512 / 1024) */
+  uint32_t batch_grid;            /* cg_batch_run: workgroups of the expansion kernel */
+  uint32_t batch_levels;          /* cg_batch_run: levels captured per HIP-graph replay */
+  uint32_t sssp_delta;            /* SSSP bucket width (delta-stepping): frontier vertices whose
+                                     distance is >= the current threshold wait; 0 = relax the
+                                     whole frontier every iteration */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */
@@ -216,6 +222,8 @@ int cg_save_bin_u32(const char* path, const uint32_t* values, uint64_t n);
 
 const char* cg_last_error(void);
 const char* cg_version(void);
+/* sizeof of the ABI structs as this library was compiled (binding checks). */
+int cg_abi_sizes(size_t* options, size_t* run_stats, size_t* partition_info);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,10 @@ def test_options_defaults_match_reference_constants():
     assert o.heavy_active_vertices == 16384  # CGgraph_engine.cuh:21
     assert o.heavy_active_edges == 25_000_000  # CGgraph_engine.cuh:22
     assert o.pr_iterations == 20 and abs(o.pr_damping - 0.85) < 1e-7
-    assert ctypes.sizeof(_lib.Options) == 80
+    sizes = [ctypes.c_size_t() for _ in range(3)]
+    _lib.lib().cg_abi_sizes(*[ctypes.byref(s) for s in sizes])
+    assert [s.value for s in sizes] == [ctypes.sizeof(_lib.Options), ctypes.sizeof(_lib.RunStats),
+                                        ctypes.sizeof(_lib.PartitionInfo)]
 
 
 def test_graph_validation_and_export():

@@ -1,0 +1,73 @@
+"""Batched queries (one HIP-graph-captured level loop for many queries over
+uneven graphs): every query's output is bit-exact against the oracle and
+identical to the one-query-per-launch path."""
+import numpy as np
+import pytest
+
+import cggraph_amd as cg
+from cggraph_amd import batch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_matches_oracle(oracle):
+    ogs = [oracle.rmat_graph(s, 16, seed=s, weights=(1, 1000)) for s in (9, 11, 13)]
+    ogs += [oracle.grid_graph(n, 0.1, seed=n) for n in (20, 45)]
+    gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
+    rng = np.random.default_rng(0)
+    qg, qo, qa, qr = [], [], [], []
+    for i in range(40):
+        k = i % len(gs)
+        qg.append(gs[k])
+        qo.append(ogs[k])
+        qa.append(i % 2)
+        qr.append(int(rng.integers(0, ogs[k].V)))
+    outs, st = batch.run_batch(qg, qa, qr, mode=batch.BATCHED, batch_levels=4, batch_grid=64)
+    for o, a, r, got in zip(qo, qa, qr, outs):
+        ref = oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)
+        assert np.array_equal(got, ref), (a, r)
+    assert st["edges_traversed"] == sum(oracle.reached_edges(o, x) for o, x in zip(qo, outs))
+    outs1, st1 = batch.run_batch(qg, qa, qr, mode=batch.ONE_PER_LAUNCH)
+    for a, b in zip(outs, outs1):
+        assert np.array_equal(a, b)
+    assert st1["edges_traversed"] == st["edges_traversed"]
+
+
+@pytest.mark.parametrize("block,grid,levels,delta", [(64, 1, 1, 0), (1024, 512, 32, 0), (256, 64, 8, 1),
+                                                     (256, 2048, 16, 700), (512, 128, 4, 10**7)])
+def test_batch_launch_shapes(oracle, block, grid, levels, delta):
+    o = oracle.grid_graph(30, 0.1, seed=5)
+    g = cg.Graph.from_csr(o.offsets, o.dest, o.weight)
+    roots = [0, 17, 899, 450]
+    outs, _ = batch.run_batch([g] * 4, [0, 1, 0, 1], roots, batch_block=block, batch_grid=grid,
+                              batch_levels=levels, sssp_delta=delta)
+    for a, r, got in zip([0, 1, 0, 1], roots, outs):
+        assert np.array_equal(got, oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r))
+
+
+def test_batch_rejects_bad_queries(oracle):
+    o = oracle.rmat_graph(8, 4, seed=1, weights=None)
+    g = cg.Graph.from_csr(o.offsets, o.dest)
+    with pytest.raises(cg.CgError):
+        batch.run_batch([g], [1], [0])  # SSSP without weights
+    with pytest.raises(cg.CgError):
+        batch.run_batch([g], [0], [o.V])  # root out of range
+    with pytest.raises(cg.CgError):
+        batch.run_batch([g], [2], [0])  # CC is not a batched query
+
+
+def test_batch_delta_stepping_mixed(oracle):
+    ogs = [oracle.rmat_graph(12, 16, seed=3, weights=(1, 1000)), oracle.grid_graph(64, 0.1, seed=2)]
+    gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
+    rng = np.random.default_rng(4)
+    qg, qo, qa, qr = [], [], [], []
+    for i in range(16):
+        k = i % 2
+        qg.append(gs[k])
+        qo.append(ogs[k])
+        qa.append(1 if i >= 4 else 0)
+        qr.append(int(rng.integers(0, ogs[k].V)))
+    for delta in (250, 4000):
+        outs, st = batch.run_batch(qg, qa, qr, sssp_delta=delta, batch_levels=8)
+        for o, a, r, got in zip(qo, qa, qr, outs):
+            assert np.array_equal(got, oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)), (delta, a, r)

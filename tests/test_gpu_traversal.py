@@ -109,6 +109,20 @@ def test_rmat_small(oracle, scale, ef):
     assert np.array_equal(c, oracle.cc_undirected(sym))
 
 
+@pytest.mark.parametrize("delta", [1, 37, 1000, 10**8])
+def test_sssp_delta_stepping(oracle, delta):
+    """Deferring frontier vertices above a moving threshold changes the order
+    of relaxations only; the distances stay the unique fixed point."""
+    cases = [random_graph(oracle, 21, 30000, 200000, hub=30000), oracle.grid_graph(120, 0.1, seed=8),
+             oracle.rmat_graph(15, 16, seed=4)]
+    for og in cases:
+        g = gpu_graph(og)
+        for root in (oracle.max_outdeg_vertex(og), og.V // 3):
+            for mode in (_lib.CG_SCHED_GPU, _lib.CG_SCHED_HEURISTIC):
+                got, st = run(g, SSSP, root, sched_mode=mode, sssp_delta=delta)
+                assert np.array_equal(got, oracle.sssp(og, root)), (delta, root, mode)
+
+
 def test_gpu_generator_matches_oracle(oracle):
     for scale, ef, sym in ((12, 16, False), (13, 8, True)):
         og = oracle.rmat_graph(scale, ef, seed=5, symmetrize=sym)
