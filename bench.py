@@ -73,7 +73,11 @@ def main():
     log(f"R-MAT s{args.scale} ef{args.edge_factor}: V={info['V']} E={info['E']} root={root} "
         f"built in {time.perf_counter() - t:.2f}s")
 
-    eng = cg.Engine(g, 1, sched_mode=1)
+    from cggraph_amd import autotune
+    sig = autotune.engine_signature("rmat", 1, info["V"], info["E"])
+    tuned = autotune.best_options(sig)
+    log(f"tuned options for {sig}: {tuned or 'none (defaults)'}")
+    eng = cg.Engine(g, 1, sched_mode=1, **tuned)
     dt, st = timed_runs(eng, root, args.steps, args.warmup)
     gpu_vals = eng.result()
     eng.close()
@@ -82,7 +86,9 @@ def main():
     log(f"SSSP: {dt * 1e3:.3f} ms/query, iterations {st['iterations']}, reached edges {m_q}, "
         f"{gteps:.2f} GTEPS")
 
-    pst = profile_run(cg, g, 1, root, sched_mode=1)
+    learned = run_learned(cg, g, root, tuned, gpu_vals)
+
+    pst = profile_run(cg, g, 1, root, sched_mode=1, **tuned)
     kms = pst["expand_kernel_ms"] / max(pst["expand_launches"], 1)
     alg_bytes = pst["expand_alg_bytes"] / max(pst["expand_launches"], 1)
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
@@ -108,9 +114,9 @@ def main():
                          f"{ms:.0f} ms", "bit_exact_vs_gpu": parity}
         log(f"CPU baseline: {cpu}")
 
-    extra = {}
+    extra = {"learned_sssp_s24": learned}
     if not args.no_extra:
-        extra = run_extra(cg, np, info)
+        extra.update(run_extra(cg, np, info))
     g.close()
 
     line = {
@@ -122,14 +128,59 @@ def main():
                                f"(a/b/c/d .57/.19/.19/.05, seed 1, self-loops and duplicates removed, "
                                f"weights U[1,100]), root = max out-degree vertex",
                    "V": info["V"], "E": info["E"], "root": root, "schedule": "gpu",
-                   "iterations": st["iterations"], "reached_edges": m_q},
+                   "tuned_options": tuned, "iterations": st["iterations"], "reached_edges": m_q},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(line), flush=True)
 
 
+def run_learned(cg, g, root, tuned, ref_vals):
+    """SSSP with the learned per-iteration scheduler (weights trained on the
+    GPU from the engine's own probe traces, cggraph_amd/data/sched_mlp.bin)."""
+    import numpy as np
+    wpath = os.path.join(ROOT, "cggraph_amd", "data", "sched_mlp.bin")
+    if not os.path.exists(wpath):
+        return None
+    eng = cg.Engine(g, 1, sched_mode=2, mlp_path=wpath, **tuned)
+    dt, st = timed_runs(eng, root, 10, 3)
+    same = bool(np.array_equal(eng.result(), ref_vals))
+    eng.close()
+    res = {"ms": round(dt * 1e3, 4), "gteps": round(st["edges_traversed"] / dt / 1e9, 3),
+           "iterations": st["iterations"], "learned_fallbacks": st["learned_fallbacks"],
+           "light_iterations": st["light_iterations"], "bit_exact_vs_plain": same}
+    log(f"learned SSSP s24: {res}")
+    return res
+
+
+def run_batched(cg, np):
+    """256 BFS + 256 SSSP queries over R-MAT s14-22 and lattices 256^2-2048^2:
+    one HIP-graph-captured batch vs one query per engine launch."""
+    from cggraph_amd import autotune, batch
+    graphs, qg, qa, qr = batch.mixed_workload()
+    opts = autotune.best_options("batch/mixed") or {"sssp_delta": 10000}
+    batch.run_batch(qg[:32], qa[:32], qr[:32], copy_out=False, **opts)  # warm-up
+    outs_b, st_b = batch.run_batch(qg, qa, qr, copy_out=True, **opts)
+    t = time.perf_counter()
+    outs_1, st_1 = batch.run_batch(qg, qa, qr, mode=batch.ONE_PER_LAUNCH, copy_out=True,
+                                   sssp_delta=opts.get("sssp_delta", 0))
+    wall_1 = time.perf_counter() - t
+    same = all(np.array_equal(a, b) for a, b in zip(outs_b, outs_1))
+    del outs_b, outs_1
+    res = {"queries": len(qr), "graphs": len(graphs), "options": opts,
+           "batched_ms": round(st_b["ms"], 2), "batched_gteps": round(st_b["edges_traversed"] / st_b["ms"] / 1e6, 3),
+           "batched_levels": st_b["iterations"], "graph_replays": st_b["expand_launches"],
+           "one_per_launch_ms": round(st_1["ms"], 2),
+           "one_per_launch_gteps": round(st_1["edges_traversed"] / st_1["ms"] / 1e6, 3),
+           "one_per_launch_wall_s": round(wall_1, 2), "speedup": round(st_1["ms"] / st_b["ms"], 2),
+           "reached_edges": st_b["edges_traversed"], "outputs_identical": same}
+    log(f"batched: {res}")
+    for gr in graphs:
+        gr.close()
+    return res
+
+
 def run_extra(cg, np, info24):
-    extra = {}
+    extra = {"batched_mixed": run_batched(cg, np)}
     # BFS R-MAT scale 18 (the reference's own CPU-runnable configuration)
     g = cg.Graph.rmat(18, 16, seed=1, weights=None)
     inf = g.info()

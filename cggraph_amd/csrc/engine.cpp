@@ -249,16 +249,25 @@ int probe_all(cg_engine* e, const DevGraph& dg, const DevStats& sst, DevStats* d
   CG_HIP(hipMemcpyAsync(e->bm_snap.p, in, words * 8, hipMemcpyDeviceToDevice, s));
   std::vector<uint32_t> shadow_snap = e->team.shadow;
   CG_HIP(hipStreamSynchronize(s));
+  // Choices that execute identically share one timing: a bottom-up step
+  // ignores the tiers and the side, a host step ignores the tiers.
+  double by_key[6] = {-1, -1, -1, -1, -1, -1};
   for (int c = 0; c < CG_NUM_CHOICES; ++c) {
     Step st{decode_choice(c), 0, threshold};
     if (e->alg != CG_BFS) st.ch.pull = 0;
     st.host_share = (st.ch.host && !st.ch.pull) ? 1 : 0;
+    const int key = st.ch.pull ? 4 : (st.host_share ? 5 : st.ch.tier);
+    if (by_key[key] >= 0) {
+      us[c] = by_key[key];
+      continue;
+    }
     cg_run_stats scratch{};
     bool tp = false;
     const auto t0 = clk::now();
     CG_TRY(exec_step(e, dg, st, sst, dst, in, out, &tp, &scratch));
     CG_HIP(hipStreamSynchronize(s));
     us[c] = std::chrono::duration<double, std::micro>(clk::now() - t0).count();
+    by_key[key] = us[c];
     CG_HIP(hipMemcpyAsync(e->val.p, e->val_snap.p, (size_t)V * 4, hipMemcpyDeviceToDevice, s));
     CG_HIP(hipMemcpyAsync(in, e->bm_snap.p, words * 8, hipMemcpyDeviceToDevice, s));
     CG_HIP(hipMemsetAsync(out, 0, words * 8, s));
