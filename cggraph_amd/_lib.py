@@ -49,6 +49,7 @@ class Options(ctypes.Structure):
         ("batch_grid", ctypes.c_uint32),
         ("batch_levels", ctypes.c_uint32),
         ("sssp_delta", ctypes.c_uint32),
+        ("pr_segment_log2", ctypes.c_uint32),
     ]
 
 

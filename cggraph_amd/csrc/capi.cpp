@@ -190,6 +190,7 @@ int cg_options_default(cg_options* o) {
   o->batch_grid = 2048;
   o->batch_levels = 16;
   o->sssp_delta = 0;
+  o->pr_segment_log2 = 0;
   return CG_OK;
 }
 
@@ -413,7 +414,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->contrib.ensure(V));
     CG_TRY(e->contrib2.ensure(V));
     DevGraph dg{g->d_off.p, g->d_dst.p, nullptr, g->d_in_off.p, g->d_in_src.p, g->V};
-    CG_TRY(pr_plan_create(dg, &e->plan, e->stream));
+    CG_TRY(pr_plan_create(dg, g->E, e->opt.pr_segment_log2, &e->plan, e->stream));
   } else {
     CG_TRY(e->val.ensure(V));
     CG_TRY(e->bm[0].ensure(words));

@@ -125,8 +125,9 @@ int wcc_hooking(const DevGraph& g, uint32_t* parent, uint32_t sample_rounds, hip
                 uint64_t* edges_relaxed);
 
 // ---- PageRank (pagerank.hip) ----
-struct PrPlan;  // per-graph vertex tiers for the pull kernel
-int pr_plan_create(const DevGraph& g, PrPlan** plan, hipStream_t s);
+struct PrPlan;  // per-graph layout for the pull kernels (tiers or source windows)
+// seg_log2: source-window size 2^k (0 automatic, >= 31 plain pull).
+int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t seg_log2, PrPlan** plan, hipStream_t s);
 void pr_plan_destroy(PrPlan* plan);
 int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* rank, float* tmp_rank,
            float* contrib, float* contrib2, hipStream_t s, int profile, double* kernel_ms,

@@ -4,31 +4,65 @@
 // The reference's PR is not functional (it accumulates into the source on
 // uint32, my_project/CGgraph/device_algorithm.cuh:505-545); only its update
 // rule is recoverable: PR(v) = (1 - d)/N + d * sum_{u -> v} PR(u)/outdeg(u),
-// d = 0.85 (:543), dangling mass dropped.  One iteration here is one sweep per
-// in-degree tier:
-//   low    (in-degree < 32)    one lane per vertex, sequential sum in CSC order
-//   medium (32 .. 8191)        one wavefront per vertex: lane j sums edges
-//                              j, j+64, ... in order, then a fixed xor-butterfly
-//   high   (>= 8192)           one workgroup per vertex: 256 strided partial
-//                              sums, then a fixed LDS tree
-// Tier membership depends only on the graph, so the result is bit-identical
-// from run to run.  Each sweep also writes contrib(v) = PR(v)/outdeg(v) for the
-// next iteration, so the division is fused into the producing kernel.
+// d = 0.85 (:543), dangling mass dropped.  contrib(u) = PR(u)/outdeg(u) is
+// written by the kernel that produces PR(u), so each iteration reads it once.
+//
+// Two layouts:
+//  plain       (small graphs) one sweep over the CSC per iteration, tiered by
+//              in-degree: one lane / one wavefront / one workgroup per vertex.
+//  segmented   (large graphs) cache blocking of the source side.  The sources
+//              are cut into windows of 2^k vertices whose contrib slice fits in
+//              an XCD's L2; the edges are reordered by (window, destination,
+//              source) once per graph, so window s is a list of runs "edges of
+//              destination d from window s".  One iteration walks the windows
+//              in order; a run's partial sum (sequential in source order, or a
+//              fixed wavefront tree for long runs) is added to sum[d], and a
+//              final sweep turns sum into PR and contrib.  The gathers of a
+//              window hit L2 instead of the 134 MB contrib array in the
+//              Infinity Cache / HBM; the price is one read-modify-write of
+//              sum[d] per (window, d) pair.
+// Both layouts fix every summation order from the graph alone, so results are
+// bit-identical from run to run.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
 #include <algorithm>
+#include <vector>
 
 #include "kernels.hpp"
 
 namespace cg {
 
 struct PrPlan {
+  // plain pull tiers
   uint32_t n_med = 0, n_high = 0;
   DevBuf<uint32_t> med, high;
+  // segmented pull
+  bool segmented = false;
+  uint32_t shift = 0, nseg = 0;
+  uint64_t R = 0;
+  DevBuf<uint32_t> src, run_dst, run_off, long_runs;
+  std::vector<uint32_t> seg_run, seg_long;
+  DevBuf<float> sum;
 };
 
 namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kMed = 32, kHigh = 8192;
+constexpr uint32_t kLongRun = 64;
+
+unsigned grid_of(uint64_t n) {
+  uint64_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  return (unsigned)(b < 65536 ? b : 65536);
+}
+
+uint32_t bits_for(uint64_t V) {
+  uint32_t b = 1;
+  while ((1ull << b) < V) ++b;
+  return b;
+}
 
 __global__ void k_classify(const uint64_t* __restrict__ in_off, uint32_t V, uint32_t* __restrict__ med,
                            uint32_t* __restrict__ high, uint32_t* __restrict__ cnt) {
@@ -104,31 +138,263 @@ __global__ void k_pr_init(DevGraph g, float r0, float* __restrict__ rank, float*
   }
 }
 
-}  // namespace
+// ---- segmented layout construction ----
 
-int pr_plan_create(const DevGraph& g, PrPlan** plan, hipStream_t s) {
-  PrPlan* p = new PrPlan();
+__global__ void k_row_heads(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ head) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    if (off[v + 1] > off[v]) head[off[v]] = (uint32_t)v;
+}
+
+// key = window(u) << 2b | v << b | u for edge u -> v
+__global__ void k_seg_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
+                           uint32_t b, uint32_t shift, uint64_t* __restrict__ keys) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t u = src[e];
+    keys[e] = ((u >> shift) << (2 * b)) | ((uint64_t)dst[e] << b) | u;
+  }
+}
+
+__global__ void k_seg_flags(const uint64_t* __restrict__ keys, uint64_t E, uint32_t b,
+                            uint32_t* __restrict__ src_out, uint32_t* __restrict__ flags) {
+  const uint64_t mask = (1ull << b) - 1;
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[e];
+    src_out[e] = (uint32_t)(k & mask);
+    flags[e] = (e == 0 || (keys[e - 1] >> b) != (k >> b)) ? 1u : 0u;
+  }
+}
+
+__global__ void k_seg_runs(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
+                           const uint32_t* __restrict__ pos, uint64_t E, uint32_t b,
+                           uint32_t* __restrict__ run_dst, uint32_t* __restrict__ run_off) {
+  const uint64_t mask = (1ull << b) - 1;
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    if (flags[e]) {
+      run_dst[pos[e]] = (uint32_t)((keys[e] >> b) & mask);
+      run_off[pos[e]] = (uint32_t)e;
+    }
+  }
+}
+
+// seg_run[s] = index of the first run of window s (runs are window-major).
+__global__ void k_seg_bounds(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ pos, uint64_t E,
+                             uint64_t R, uint32_t b, uint32_t nseg, uint32_t* __restrict__ seg_run) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nseg) return;
+  const uint64_t target = (uint64_t)s << (2 * b);
+  uint64_t lo = 0, hi = E;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  seg_run[s] = lo < E ? pos[lo] : (uint32_t)R;
+}
+
+__global__ void k_long_flags(const uint32_t* __restrict__ run_off, uint64_t R, uint32_t* __restrict__ flags) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x)
+    flags[r] = (run_off[r + 1] - run_off[r]) >= kLongRun ? 1u : 0u;
+}
+
+__global__ void k_long_scatter(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint64_t R,
+                               uint32_t* __restrict__ long_runs) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x)
+    if (flags[r]) long_runs[pos[r]] = (uint32_t)r;
+}
+
+// ---- segmented iteration ----
+
+// One lane per run of the window; runs of kLongRun edges or more are left to
+// k_seg_long.
+__global__ __launch_bounds__(kBlock) void k_seg_short(const uint32_t* __restrict__ src,
+                                                      const uint32_t* __restrict__ run_dst,
+                                                      const uint32_t* __restrict__ run_off, uint32_t r0, uint32_t r1,
+                                                      const float* __restrict__ contrib, float* __restrict__ sum) {
+  const uint32_t r = r0 + blockIdx.x * kBlock + threadIdx.x;
+  if (r >= r1) return;
+  const uint32_t s = __builtin_nontemporal_load(run_off + r), e = __builtin_nontemporal_load(run_off + r + 1);
+  if (e - s >= kLongRun) return;
+  float p = 0.0f;
+  for (uint32_t k = s; k < e; ++k) p += contrib[__builtin_nontemporal_load(src + k)];
+  const uint32_t d = __builtin_nontemporal_load(run_dst + r);
+  sum[d] += p;
+}
+
+// One wavefront per long run: lane j sums edges j, j+64, ... in order, then a
+// fixed xor butterfly.
+__global__ __launch_bounds__(kBlock) void k_seg_long(const uint32_t* __restrict__ src,
+                                                     const uint32_t* __restrict__ run_dst,
+                                                     const uint32_t* __restrict__ run_off,
+                                                     const uint32_t* __restrict__ long_runs, uint32_t l0, uint32_t l1,
+                                                     const float* __restrict__ contrib, float* __restrict__ sum) {
+  const uint32_t i = l0 + blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  if (i >= l1) return;  // wave-uniform
+  const uint32_t lane = __lane_id();
+  const uint32_t r = long_runs[i];
+  const uint32_t s = run_off[r], e = run_off[r + 1];
+  float p = 0.0f;
+  for (uint32_t k = s + lane; k < e; k += 64) p += contrib[__builtin_nontemporal_load(src + k)];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) p += __shfl_xor(p, d, 64);
+  if (lane == 0) sum[run_dst[r]] += p;
+}
+
+__global__ void k_seg_finish(DevGraph g, float base, float damping, float* __restrict__ sum, float* __restrict__ rank,
+                             float* __restrict__ contrib) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < g.V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    finish((uint32_t)v, sum[v], base, damping, g.off, rank, contrib);
+    sum[v] = 0.0f;
+  }
+}
+
+#define PR_LAUNCH_OK()                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess)                                                               \
+      return set_error(CG_ERR_HIP, "PageRank kernel launch: %s (%s:%d)", hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                             \
+  } while (0)
+
+int build_plain(const DevGraph& g, PrPlan* p, hipStream_t s) {
   DevBuf<uint32_t> cnt;
-  int st = cnt.ensure(2);
-  if (st == CG_OK) st = p->med.ensure(g.V ? g.V : 1);
-  if (st == CG_OK) st = p->high.ensure(g.V ? g.V : 1);
-  if (st != CG_OK) { delete p; return st; }
-  hipError_t he = hipMemsetAsync(cnt.p, 0, 2 * sizeof(uint32_t), s);
-  if (he == hipSuccess && g.V) {
-    unsigned blocks = (unsigned)std::min<uint64_t>(((uint64_t)g.V + kBlock - 1) / kBlock, 65536);
-    hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(kBlock), 0, s, g.in_off, g.V, p->med.p, p->high.p,
+  CG_TRY(cnt.ensure(2));
+  CG_TRY(p->med.ensure(g.V ? g.V : 1));
+  CG_TRY(p->high.ensure(g.V ? g.V : 1));
+  CG_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(uint32_t), s));
+  if (g.V) {
+    hipLaunchKernelGGL(k_classify, dim3(grid_of(g.V)), dim3(kBlock), 0, s, g.in_off, g.V, p->med.p, p->high.p,
                        cnt.p);
-    he = hipGetLastError();
+    PR_LAUNCH_OK();
   }
   uint32_t h[2] = {0, 0};
-  if (he == hipSuccess) he = hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s);
-  if (he == hipSuccess) he = hipStreamSynchronize(s);
-  if (he != hipSuccess) {
-    delete p;
-    return set_error(CG_ERR_HIP, "PageRank plan: %s", hipGetErrorString(he));
-  }
+  CG_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
   p->n_med = h[0];
   p->n_high = h[1];
+  return CG_OK;
+}
+
+int build_segmented(const DevGraph& g, uint64_t E, uint32_t shift, PrPlan* p, hipStream_t s) {
+  const uint32_t V = g.V;
+  const uint32_t b = bits_for(V);
+  const uint32_t nseg = (uint32_t)(((uint64_t)V + (1ull << shift) - 1) >> shift);
+  const uint32_t segbits = bits_for(nseg) + 1;
+  p->shift = shift;
+  p->nseg = nseg;
+  DevBuf<uint64_t> keys, alt;
+  CG_TRY(keys.ensure(E));
+  CG_TRY(alt.ensure(E));
+  {
+    DevBuf<uint32_t> head, srcv;
+    CG_TRY(head.ensure(E));
+    CG_TRY(srcv.ensure(E));
+    CG_HIP(hipMemsetAsync(head.p, 0, E * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_row_heads, dim3(grid_of(V)), dim3(kBlock), 0, s, g.off, V, head.p);
+    PR_LAUNCH_OK();
+    size_t sb = 0;
+    CG_HIP(rocprim::inclusive_scan(nullptr, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
+    DevBuf<uint8_t> st;
+    CG_TRY(st.ensure(sb));
+    CG_HIP(rocprim::inclusive_scan(st.p, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
+    hipLaunchKernelGGL(k_seg_keys, dim3(grid_of(E)), dim3(kBlock), 0, s, srcv.p, g.dst, E, b, shift, keys.p);
+    PR_LAUNCH_OK();
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  rocprim::double_buffer<uint64_t> db(keys.p, alt.p);
+  {
+    size_t tb = 0;
+    CG_HIP(rocprim::radix_sort_keys(nullptr, tb, db, E, 0, 2 * b + segbits, s));
+    DevBuf<uint8_t> tmp;
+    CG_TRY(tmp.ensure(tb));
+    CG_HIP(rocprim::radix_sort_keys(tmp.p, tb, db, E, 0, 2 * b + segbits, s));
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  uint64_t* sorted = db.current();
+  DevBuf<uint32_t> flags, pos;
+  CG_TRY(p->src.ensure(E));
+  CG_TRY(flags.ensure(E));
+  CG_TRY(pos.ensure(E));
+  hipLaunchKernelGGL(k_seg_flags, dim3(grid_of(E)), dim3(kBlock), 0, s, sorted, E, b, p->src.p, flags.p);
+  PR_LAUNCH_OK();
+  size_t sb = 0;
+  CG_HIP(rocprim::exclusive_scan(nullptr, sb, flags.p, pos.p, 0u, E, rocprim::plus<uint32_t>(), s));
+  DevBuf<uint8_t> st;
+  CG_TRY(st.ensure(sb));
+  CG_HIP(rocprim::exclusive_scan(st.p, sb, flags.p, pos.p, 0u, E, rocprim::plus<uint32_t>(), s));
+  uint32_t last_pos = 0, last_flag = 0;
+  CG_HIP(hipMemcpyAsync(&last_pos, pos.p + E - 1, 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipMemcpyAsync(&last_flag, flags.p + E - 1, 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  const uint64_t R = (uint64_t)last_pos + last_flag;
+  p->R = R;
+  CG_TRY(p->run_dst.ensure(R));
+  CG_TRY(p->run_off.ensure(R + 1));
+  hipLaunchKernelGGL(k_seg_runs, dim3(grid_of(E)), dim3(kBlock), 0, s, sorted, flags.p, pos.p, E, b, p->run_dst.p,
+                     p->run_off.p);
+  PR_LAUNCH_OK();
+  const uint32_t Eu = (uint32_t)E;
+  CG_HIP(hipMemcpyAsync(p->run_off.p + R, &Eu, 4, hipMemcpyHostToDevice, s));
+  DevBuf<uint32_t> segb;
+  CG_TRY(segb.ensure(nseg + 1));
+  hipLaunchKernelGGL(k_seg_bounds, dim3((nseg + 1 + 255) / 256), dim3(256), 0, s, sorted, pos.p, E, R, b, nseg,
+                     segb.p);
+  PR_LAUNCH_OK();
+  p->seg_run.resize(nseg + 1);
+  CG_HIP(hipMemcpyAsync(p->seg_run.data(), segb.p, (nseg + 1) * 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  // long runs, in run order
+  hipLaunchKernelGGL(k_long_flags, dim3(grid_of(R)), dim3(kBlock), 0, s, p->run_off.p, R, flags.p);
+  PR_LAUNCH_OK();
+  size_t sb2 = 0;
+  CG_HIP(rocprim::exclusive_scan(nullptr, sb2, flags.p, pos.p, 0u, R, rocprim::plus<uint32_t>(), s));
+  CG_TRY(st.ensure(sb2));
+  CG_HIP(rocprim::exclusive_scan(st.p, sb2, flags.p, pos.p, 0u, R, rocprim::plus<uint32_t>(), s));
+  CG_HIP(hipMemcpyAsync(&last_pos, pos.p + R - 1, 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipMemcpyAsync(&last_flag, flags.p + R - 1, 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  const uint64_t NL = (uint64_t)last_pos + last_flag;
+  CG_TRY(p->long_runs.ensure(NL ? NL : 1));
+  hipLaunchKernelGGL(k_long_scatter, dim3(grid_of(R)), dim3(kBlock), 0, s, flags.p, pos.p, R, p->long_runs.p);
+  PR_LAUNCH_OK();
+  // window boundaries in the long-run list: first long run with id >= seg_run[w]
+  std::vector<uint32_t> hl(NL);
+  if (NL) CG_HIP(hipMemcpyAsync(hl.data(), p->long_runs.p, NL * 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  p->seg_long.resize(nseg + 1);
+  for (uint32_t w = 0; w <= nseg; ++w)
+    p->seg_long[w] = (uint32_t)(std::lower_bound(hl.begin(), hl.end(), p->seg_run[w]) - hl.begin());
+  CG_TRY(p->sum.ensure(V));
+  CG_HIP(hipMemsetAsync(p->sum.p, 0, (size_t)V * sizeof(float), s));
+  CG_HIP(hipStreamSynchronize(s));
+  p->segmented = true;
+  return CG_OK;
+}
+
+}  // namespace
+
+int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t seg_log2, PrPlan** plan, hipStream_t s) {
+  PrPlan* p = new PrPlan();
+  const uint32_t b = bits_for(g.V);
+  uint32_t shift = seg_log2 ? seg_log2 : 20;
+  const bool auto_seg = seg_log2 == 0;
+  bool seg = shift < 31 && (uint64_t)g.V > (1ull << shift) && E < 0xFFFFFFFFull && E > 0;
+  if (auto_seg) seg = seg && g.V > (1u << 21);
+  if (seg) {
+    const uint32_t nseg = (uint32_t)(((uint64_t)g.V + (1ull << shift) - 1) >> shift);
+    if (2 * b + bits_for(nseg) + 1 > 64) seg = false;
+  }
+  const int st = seg ? build_segmented(g, E, shift, p, s) : build_plain(g, p, s);
+  if (st != CG_OK) {
+    delete p;
+    return st;
+  }
   *plan = p;
   return CG_OK;
 }
@@ -156,13 +422,28 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
   double total = 0.0;
   for (int it = 0; it < iters; ++it) {
     if (profile) CG_HIP(hipEventRecord(a, s));
-    hipLaunchKernelGGL(k_pr_low, dim3(vblocks), dim3(kBlock), 0, s, g, c_cur, base, damping, r_nxt, c_nxt);
-    if (plan->n_med)
-      hipLaunchKernelGGL(k_pr_med, dim3((plan->n_med + 3) / 4), dim3(kBlock), 0, s, g, plan->med.p,
-                         plan->n_med, c_cur, base, damping, r_nxt, c_nxt);
-    if (plan->n_high)
-      hipLaunchKernelGGL(k_pr_high, dim3(plan->n_high), dim3(kBlock), 0, s, g, plan->high.p, c_cur, base,
-                         damping, r_nxt, c_nxt);
+    if (plan->segmented) {
+      for (uint32_t w = 0; w < plan->nseg; ++w) {
+        const uint32_t r0 = plan->seg_run[w], r1 = plan->seg_run[w + 1];
+        if (r1 > r0)
+          hipLaunchKernelGGL(k_seg_short, dim3((r1 - r0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, plan->src.p,
+                             plan->run_dst.p, plan->run_off.p, r0, r1, c_cur, plan->sum.p);
+        const uint32_t l0 = plan->seg_long[w], l1 = plan->seg_long[w + 1];
+        if (l1 > l0)
+          hipLaunchKernelGGL(k_seg_long, dim3((l1 - l0 + 3) / 4), dim3(kBlock), 0, s, plan->src.p, plan->run_dst.p,
+                             plan->run_off.p, plan->long_runs.p, l0, l1, c_cur, plan->sum.p);
+      }
+      hipLaunchKernelGGL(k_seg_finish, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, g, base,
+                         damping, plan->sum.p, r_nxt, c_nxt);
+    } else {
+      hipLaunchKernelGGL(k_pr_low, dim3(vblocks), dim3(kBlock), 0, s, g, c_cur, base, damping, r_nxt, c_nxt);
+      if (plan->n_med)
+        hipLaunchKernelGGL(k_pr_med, dim3((plan->n_med + 3) / 4), dim3(kBlock), 0, s, g, plan->med.p, plan->n_med,
+                           c_cur, base, damping, r_nxt, c_nxt);
+      if (plan->n_high)
+        hipLaunchKernelGGL(k_pr_high, dim3(plan->n_high), dim3(kBlock), 0, s, g, plan->high.p, c_cur, base,
+                           damping, r_nxt, c_nxt);
+    }
     if (profile) {
       CG_HIP(hipEventRecord(b, s));
       CG_HIP(hipEventSynchronize(b));

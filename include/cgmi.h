@@ -104,6 +104,8 @@ typedef struct cg_options {
   uint32_t sssp_delta;            /* SSSP bucket width (delta-stepping): frontier vertices whose
                                      distance is >= the current threshold wait; 0 = relax the
                                      whole frontier every iteration */
+  uint32_t pr_segment_log2;       /* PageRank source-window size 2^k vertices (cache blocking);
+                                     0 = automatic, 31 = one window (plain pull) */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */
