@@ -425,6 +425,9 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->q_chunk.ensure(V + g->E / min_chunk + 1));
     CG_TRY(e->st.ensure(2));
     CG_TRY(e->hst.ensure(3));
+    CG_TRY(e->hst_map.ensure(2, hipHostMallocMapped | hipHostMallocCoherent));
+    e->hst_map_dev = e->hst_map.device_ptr();
+    if (!e->hst_map_dev) return set_error(CG_ERR_HIP, "mapped statistics buffer has no device address");
     CG_TRY(e->acc.ensure(kAccSlots));
     CG_TRY(e->parts.ensure(stats_parts_needed(words)));
     CG_TRY(e->stripes.ensure(kStripes * kStripeStride));

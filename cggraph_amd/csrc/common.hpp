@@ -87,11 +87,13 @@ struct PinnedBuf {
     p = nullptr;
     n = 0;
   }
-  int ensure(size_t count) {
+  // flags: hipHostMallocDefault, or hipHostMallocMapped | hipHostMallocCoherent
+  // for memory a kernel writes directly (then see device_ptr()).
+  int ensure(size_t count, unsigned flags = hipHostMallocDefault) {
     if (count <= n && p) return CG_OK;
     reset();
     size_t bytes = (count ? count : 1) * sizeof(T);
-    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), bytes, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), bytes, flags);
     if (e != hipSuccess) {
       p = nullptr;
       (void)hipGetLastError();
@@ -99,6 +101,15 @@ struct PinnedBuf {
     }
     n = count;
     return CG_OK;
+  }
+  // Device address of a mapped allocation.
+  T* device_ptr() const {
+    void* d = nullptr;
+    if (!p || hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    return static_cast<T*>(d);
   }
 };
 

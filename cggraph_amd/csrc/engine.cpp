@@ -156,6 +156,29 @@ bool probs_usable(const DevStats& s) {
   return std::fabs(sum - 1.f) < 1e-3f && mx >= 0.35f;
 }
 
+// Waits for k_stats of iteration `it` to publish into mapped host memory:
+// a short spin on the `iter` word (written last), then a stream
+// synchronisation if the kernel takes longer.  Copies the stats to hst_last.
+int wait_stats(cg_engine* e, int slot, uint32_t it) {
+  DevStats* h = e->hst_map.p + slot;
+  const auto t0 = clk::now();
+  for (uint32_t spin = 1;; ++spin) {
+    if (__atomic_load_n(&h->iter, __ATOMIC_ACQUIRE) == it) {
+      std::memcpy(&e->hst_last, h, sizeof(DevStats));
+      e->hst_last.iter = it;
+      return CG_OK;
+    }
+    __builtin_ia32_pause();
+    if ((spin & 1023) == 0 && ms_since(t0) > 50.0) break;  // long iteration: block instead
+  }
+  CG_HIP(hipStreamSynchronize(e->stream));
+  if (__atomic_load_n(&h->iter, __ATOMIC_ACQUIRE) != it)
+    return set_error(CG_ERR_INTERNAL, "statistics of iteration %u were not published", it);
+  std::memcpy(&e->hst_last, h, sizeof(DevStats));
+  e->hst_last.iter = it;
+  return CG_OK;
+}
+
 struct Step {
   Choice ch;
   int host_share;      // 0 GPU only, 1 host team only, 2 co-executed
@@ -170,7 +193,8 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
   const uint64_t words = ((uint64_t)dg.V + 63) / 64;
   const int alg = e->alg;
   // queue lengths restart at zero for this compaction
-  CG_HIP(hipMemsetAsync(dst, 0, 3 * sizeof(uint32_t), s));
+  // k_stats leaves them zero; probes run several compactions per iteration
+  if (e->opt.sched_mode == CG_SCHED_PROBE) CG_HIP(hipMemsetAsync(dst, 0, 3 * sizeof(uint32_t), s));
   if (step.ch.pull) {
     ++local->pull_iterations;
     ++local->heavy_iterations;
@@ -306,6 +330,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   CG_HIP(hipMemsetAsync(e->bm[0].p, 0, words * sizeof(uint64_t), s));
   CG_HIP(hipMemsetAsync(e->bm[1].p, 0, words * sizeof(uint64_t), s));
   CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripes * kStripeStride * sizeof(uint32_t), s));
+  CG_HIP(hipMemsetAsync(e->st.p, 0, 2 * sizeof(DevStats), s));
+  for (int k = 0; k < 2; ++k) __atomic_store_n(&e->hst_map.p[k].iter, 0xFFFFFFFFu, __ATOMIC_RELEASE);
   if (alg == CG_CC) {
     CG_TRY(launch_iota_u32(e->val.p, V, s));
     CG_TRY(launch_bitmap_prefix(e->bm[0].p, words, (uint64_t)V - g.zero_tail, s));
@@ -332,17 +358,17 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     ++it;
     const int slot = it & 1;
     DevStats* dst = e->st.p + slot;
-    CG_HIP(hipMemsetAsync(dst, 0, sizeof(DevStats), s));
     CG_TRY(launch_stats(e->bm[cur].p, e->bm[cur ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p,
-                        e->run_state.p, mlp_dev, tval, threshold, dst, s));
-    CG_HIP(hipMemcpyAsync(e->hst.p + slot, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
-    CG_HIP(hipStreamSynchronize(s));
+                        e->run_state.p, mlp_dev, tval, threshold, dst, e->st.p + (slot ^ 1),
+                        e->hst_map_dev + slot, s));
+    CG_TRY(wait_stats(e, slot, it));
     if (e->opt.fault_flags & CG_FAULT_HIP_ERROR)
       return set_error(CG_ERR_HIP, "injected HIP failure at iteration %u", it);
-    const DevStats sst = e->hst.p[slot];
+    const DevStats sst = e->hst_last;
     if (it > 1) {
       if (timing_pending) {
         float ms = 0.f;
+        CG_HIP(hipEventSynchronize(e->ev1));
         CG_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
         local.expand_kernel_ms += ms;
         timing_pending = false;

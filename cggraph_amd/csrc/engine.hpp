@@ -25,6 +25,9 @@ struct cg_engine {
   cg::DevBuf<uint64_t> q_chunk;
   cg::DevBuf<cg::DevStats> st;       // two slots, alternating per iteration
   cg::PinnedBuf<cg::DevStats> hst;
+  cg::PinnedBuf<cg::DevStats> hst_map;  // mapped: k_stats publishes here
+  cg::DevStats* hst_map_dev = nullptr;  // its device address
+  cg::DevStats hst_last{};
   cg::DevBuf<unsigned long long> acc;  // kAccSlots striped accumulators
   cg::DevBuf<cg::StatsPart> parts;     // k_stats workgroup partials
   cg::DevBuf<uint32_t> stripes;        // striped new-vertex counters

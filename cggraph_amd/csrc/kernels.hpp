@@ -81,9 +81,13 @@ int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s);
 // With `val` not NULL it also reports the frontier's smallest value and how
 // many frontier vertices lie below `threshold` (delta-stepping runs).
 uint64_t stats_parts_needed(uint64_t words);
+// The last workgroup also zeroes st's queue counters and other->ticket, and
+// publishes the stats to `host_out` (mapped host memory, may be NULL) with
+// `iter` written last.
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
                  uint64_t E, StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
-                 const uint32_t* val, uint32_t threshold, DevStats* st, hipStream_t s);
+                 const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* other,
+                 DevStats* host_out, hipStream_t s);
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
 // Frontier vertices whose value is >= threshold are not queued but carried
 // into `carry` (the next frontier) unchanged.

@@ -143,7 +143,8 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
                                                   uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
                                                   const float* __restrict__ mlp,
                                                   const uint32_t* __restrict__ val, uint32_t threshold,
-                                                  DevStats* __restrict__ st) {
+                                                  DevStats* __restrict__ st, DevStats* __restrict__ other,
+                                                  DevStats* __restrict__ host_out) {
   __shared__ StatsPart wpart[kWavesPerBlock];
   __shared__ bool is_last;
   const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
@@ -264,6 +265,24 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
     mlp_forward_block(mlp, feat, mlp_sh, probs, &best);
     if (threadIdx.x < kMlpOut) st->probs[threadIdx.x] = probs[threadIdx.x];
     if (threadIdx.x == 0) st->choice = best;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // this iteration's compaction counts from zero; the other slot's ticket
+    // is ready for the next iteration's statistics
+    st->n_small = 0;
+    st->n_med = 0;
+    st->n_chunk = 0;
+    if (other) other->ticket = 0;
+    if (host_out) {
+      // publish to mapped host memory; `iter` goes last so the host can poll it
+      const uint32_t itv = st->iter;
+      DevStats tmp = *st;
+      tmp.iter = 0xFFFFFFFFu;
+      *host_out = tmp;
+      __threadfence_system();
+      __hip_atomic_store(&host_out->iter, itv, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
@@ -577,10 +596,11 @@ uint64_t stats_parts_needed(uint64_t words) { return (words + kBlock - 1) / kBlo
 
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
                  StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
-                 const uint32_t* val, uint32_t threshold, DevStats* st, hipStream_t s) {
+                 const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* other, DevStats* host_out,
+                 hipStream_t s) {
   const uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
   hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, E, parts,
-                     stripes, rs, mlp, val, threshold, st);
+                     stripes, rs, mlp, val, threshold, st, other, host_out);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

@@ -283,6 +283,19 @@ void or_pagerank64(uint32_t V, const uint64_t* off, const uint32_t* dst, int ite
   free(nxt);
 }
 
+void or_pagerank_step(uint32_t V, const uint64_t* off, const uint32_t* dst, const float* in,
+                      double damping, double* out) {
+  const double base = (1.0 - damping) / V;
+  for (uint32_t v = 0; v < V; ++v) out[v] = 0.0;
+  for (uint32_t u = 0; u < V; ++u) {
+    uint64_t deg = off[u + 1] - off[u];
+    if (!deg) continue;
+    double c = (double)in[u] / (double)deg;
+    for (uint64_t e = off[u]; e < off[u + 1]; ++e) out[dst[e]] += c;
+  }
+  for (uint32_t v = 0; v < V; ++v) out[v] = base + damping * out[v];
+}
+
 int64_t or_check_sssp_cert(uint32_t V, const uint64_t* off, const uint32_t* dst,
                            const uint32_t* w, uint32_t root, const uint32_t* dist) {
   if (root >= V) return 1;

@@ -78,6 +78,10 @@ void or_cc_undirected(uint32_t V, const uint64_t* off, const uint32_t* dst, uint
 void or_cc_directed(uint32_t V, const uint64_t* off, const uint32_t* dst, uint32_t* out);
 void or_pagerank64(uint32_t V, const uint64_t* off, const uint32_t* dst, int iters,
                    double damping, double* out);
+/* One PageRank step in float64 from given (float32) ranks: the size-independent
+ * check of a full-size run (PR_k must equal step(PR_{k-1}) up to FP32 rounding). */
+void or_pagerank_step(uint32_t V, const uint64_t* off, const uint32_t* dst, const float* in,
+                      double damping, double* out);
 
 /* ---- checks ---- */
 /* Shortest-path certificate (w == NULL means unit weights, i.e. BFS):

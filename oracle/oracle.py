@@ -42,6 +42,8 @@ def lib():
             "or_cc_undirected": (None, [u32, _u64p, _u32p, _u32p]),
             "or_cc_directed": (None, [u32, _u64p, _u32p, _u32p]),
             "or_pagerank64": (None, [u32, _u64p, _u32p, i32, dbl, _f64p]),
+            "or_pagerank_step": (None, [u32, _u64p, _u32p, np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS"),
+                                        dbl, _f64p]),
             "or_check_sssp_cert": (i64, [u32, _u64p, _u32p, _vp, u32, _u32p]),
             "or_reached_edges": (u64, [u32, _u64p, _u32p]),
             "or_max_outdeg_vertex": (u32, [u32, _u64p]),
@@ -155,6 +157,12 @@ def cc_directed(g):
 def pagerank64(g, iters=20, damping=0.85):
     out = np.empty(g.V, np.float64)
     lib().or_pagerank64(g.V, g.offsets, g.dest, iters, damping, out)
+    return out
+
+
+def pagerank_step(g, ranks32, damping=0.85):
+    out = np.empty(g.V, np.float64)
+    lib().or_pagerank_step(g.V, g.offsets, g.dest, np.ascontiguousarray(ranks32, np.float32), damping, out)
     return out
 
 

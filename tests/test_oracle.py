@@ -210,3 +210,15 @@ def test_multicore_random(oracle, seed):
         assert np.array_equal(v, oracle.sssp(g, 0))
         v, _, _ = oracle.multicore_run(2, g, 0, threads)
         assert np.array_equal(v, oracle.cc_directed(g))
+
+
+def test_pagerank_step_consistent(oracle):
+    """One float64 step from float32 ranks reproduces the next iteration of the
+    float64 power iteration to within float32 input rounding."""
+    rng = np.random.default_rng(8)
+    g = random_graph(oracle, rng, 500, 4000)
+    p19 = oracle.pagerank64(g, 19, 0.85)
+    p20 = oracle.pagerank64(g, 20, 0.85)
+    step = oracle.pagerank_step(g, p19.astype(np.float32), 0.85)
+    assert np.abs(step - p20).sum() < 1e-6
+    assert np.abs(step - p20).sum() > 0  # the float32 rounding is visible, so inputs were used
