@@ -46,8 +46,39 @@ def profile_run(cg, g, alg, root, **opt):
     return st
 
 
+def profile_workload(args):
+    """For rocprofv3 counter runs: one 1 GiB device-to-device copy (the
+    calibration kernel of known size: 1 GiB read + 1 GiB written) and one
+    SSSP query with the benchmark's settings."""
+    import ctypes
+    import cggraph_amd as cg
+    from cggraph_amd import autotune
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemcpy(b, a, n, 3)
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    g = cg.Graph.rmat(args.scale, args.edge_factor, seed=1, weights=(1, 100))
+    info = g.info()
+    tuned = autotune.best_options(autotune.engine_signature("rmat", 1, info["V"], info["E"]))
+    e = cg.Engine(g, 1, sched_mode=1, **tuned)
+    _, st = e.run(info["max_outdeg_vertex"], copy_out=False)
+    e.close()
+    print(json.dumps({"profile_workload": "1 GiB D2D copy + one SSSP query", "expand_launches": st["expand_launches"],
+                      "expand_alg_bytes": st["expand_alg_bytes"], "iterations": st["iterations"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--profile-run", action="store_true",
+                    help="run the calibration copy and one query only (for rocprofv3 counter runs)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
@@ -57,6 +88,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    if args.profile_run:
+        profile_workload(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
               "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
@@ -92,9 +126,16 @@ def main():
     kms = pst["expand_kernel_ms"] / max(pst["expand_launches"], 1)
     alg_bytes = pst["expand_alg_bytes"] / max(pst["expand_launches"], 1)
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # (scaled by the same counters on a 1 GiB copy), kept in profiles/
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_expand_sssp_rmat24.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": "k_expand<SSSP>", "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_ms": round(kms, 5),
+                "traffic": traffic, "avg_launch_ms": round(kms, 5),
                 "alg_bytes_per_launch": round(alg_bytes, 1), "launches": pst["expand_launches"]}
 
     cpu = None

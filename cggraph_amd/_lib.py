@@ -50,6 +50,7 @@ class Options(ctypes.Structure):
         ("batch_levels", ctypes.c_uint32),
         ("sssp_delta", ctypes.c_uint32),
         ("pr_segment_log2", ctypes.c_uint32),
+        ("atomic_return", ctypes.c_uint32),
     ]
 
 

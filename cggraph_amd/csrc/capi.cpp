@@ -191,6 +191,7 @@ int cg_options_default(cg_options* o) {
   o->batch_levels = 16;
   o->sssp_delta = 0;
   o->pr_segment_log2 = 0;
+  o->atomic_return = 1;
   return CG_OK;
 }
 

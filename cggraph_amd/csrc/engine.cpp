@@ -244,7 +244,7 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
   if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
   const auto tg = clk::now();
   CG_TRY(launch_expand_push(alg, dg, e->val.p, out, e->q_small.p, e->q_med.p, e->q_chunk.p, bound,
-                            small_limit, bins, dst, e->stripes.p, s));
+                            small_limit, bins, dst, e->stripes.p, e->opt.atomic_return != 0, s));
   if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); *timing_pending = true; }
   ++local->expand_launches;
   local->expand_alg_bytes += 20.0 * (sst.n_active - sst.n_zero_deg) +

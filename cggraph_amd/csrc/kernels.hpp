@@ -103,7 +103,7 @@ uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bin
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
-                       uint32_t* stripes, hipStream_t s);
+                       uint32_t* stripes, bool exact, hipStream_t s);
 // Bottom-up BFS step: unreached vertices look for a parent in `in`.
 int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
                     uint32_t* stripes, hipStream_t s);

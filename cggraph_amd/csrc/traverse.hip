@@ -73,8 +73,13 @@ __device__ __forceinline__ void stripe_add(uint32_t* stripes, uint32_t v) {
 enum { ALG_BFS = CG_BFS, ALG_SSSP = CG_SSSP, ALG_CC = CG_CC };
 
 // Edge operation shared by every push kernel.  Returns 1 when this update
-// reached the vertex for the first time.
-template <int ALG>
+// reached the vertex for the first time.  EXACT: activate only when the
+// atomic min returned a larger value.  Otherwise the atomics return nothing
+// (the wavefront does not wait for them) and the destination is activated on
+// the strict-less pre-check: a pre-check that passes means the value is
+// lowered in this iteration by this or another update (or the read was
+// stale), so the active set is a superset and the fixed point is the same.
+template <int ALG, bool EXACT>
 __device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_t* __restrict__ dst,
                                           const uint32_t* __restrict__ w, uint32_t* __restrict__ val,
                                           uint64_t* __restrict__ out) {
@@ -83,11 +88,18 @@ __device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_
   if constexpr (ALG == ALG_BFS) cand = sv + 1;
   else if constexpr (ALG == ALG_SSSP) cand = sv + w[e];
   else cand = sv;
-  if (cand < val[d]) {
-    const uint32_t old = atomicMin(val + d, cand);
-    if (cand < old) {
-      atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
-      return old == kUnreached ? 1u : 0u;
+  const uint32_t seen = val[d];
+  if (cand < seen) {
+    if constexpr (EXACT) {
+      const uint32_t old = atomicMin(val + d, cand);
+      if (cand < old) {
+        atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
+        return old == kUnreached ? 1u : 0u;
+      }
+    } else {
+      (void)atomicMin(val + d, cand);
+      (void)atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
+      return seen == kUnreached ? 1u : 0u;
     }
   }
   return 0u;
@@ -360,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   }
 }
 
-template <int ALG>
+template <int ALG, bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restrict__ val,
                                                    uint64_t* __restrict__ out,
                                                    const uint32_t* __restrict__ q_small,
@@ -384,12 +396,12 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
     const uint64_t end = g.off[v + 1];
     const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
     const uint32_t sv = val[v];
-    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG>(sv, k, g.dst, g.w, val, out);
+    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG, EXACT>(sv, k, g.dst, g.w, val, out);
   } else if (item < (uint64_t)n_chunk + n_med) {
     const uint32_t v = q_med[item - n_chunk];
     const uint64_t s = g.off[v], e = g.off[v + 1];
     const uint32_t sv = val[v];
-    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG>(sv, k, g.dst, g.w, val, out);
+    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG, EXACT>(sv, k, g.dst, g.w, val, out);
   } else {
     // 64 low-degree vertices per wavefront, their edges spread evenly over
     // the lanes: lane j of round r takes edge r*64+j of the concatenated
@@ -419,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
       const uint64_t os = __shfl(s, (int)lo, kWave);
       const uint32_t osv = __shfl(sv, (int)lo, kWave);
       const uint32_t oex = __shfl(excl, (int)lo, kWave);
-      if (j < total) n_new += relax<ALG>(osv, os + (j - oex), g.dst, g.w, val, out);
+      if (j < total) n_new += relax<ALG, EXACT>(osv, os + (j - oex), g.dst, g.w, val, out);
     }
   }
   if constexpr (ALG != ALG_CC) {
@@ -531,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_push_bitmap(DevGraph g, uint32_t* __
     const uint64_t os = __shfl(s, (int)lo, kWave);
     const uint32_t osv = __shfl(sv, (int)lo, kWave);
     const uint32_t oex = __shfl(excl, (int)lo, kWave);
-    if (j < total) (void)relax<ALG>(osv, os + (j - oex), g.dst, g.w, val, out);
+    if (j < total) (void)relax<ALG, true>(osv, os + (j - oex), g.dst, g.w, val, out);
   }
 }
 
@@ -624,27 +636,25 @@ uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bin
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
-                       uint32_t* stripes, hipStream_t s) {
+                       uint32_t* stripes, bool exact, hipStream_t s) {
   if (!item_bound) return CG_OK;
   const uint64_t blocks = (item_bound + kWavesPerBlock - 1) / kWavesPerBlock;
   if (blocks >= (1ull << 31)) return set_error(CG_ERR_INTERNAL, "expansion grid too large");
   dim3 grid((unsigned)blocks), block(kBlock);
-  switch (alg) {
-    case CG_BFS:
-      hipLaunchKernelGGL(k_expand<ALG_BFS>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st, stripes);
-      break;
-    case CG_SSSP:
-      hipLaunchKernelGGL(k_expand<ALG_SSSP>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st, stripes);
-      break;
-    case CG_CC:
-      hipLaunchKernelGGL(k_expand<ALG_CC>, grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st, stripes);
-      break;
+#define CG_EXPAND(A, X)                                                                          \
+  hipLaunchKernelGGL((k_expand<A, X>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk, small_limit, \
+                     bins.chunk_edges, st, stripes)
+  switch (alg * 2 + (exact ? 1 : 0)) {
+    case CG_BFS * 2: CG_EXPAND(ALG_BFS, false); break;
+    case CG_BFS * 2 + 1: CG_EXPAND(ALG_BFS, true); break;
+    case CG_SSSP * 2: CG_EXPAND(ALG_SSSP, false); break;
+    case CG_SSSP * 2 + 1: CG_EXPAND(ALG_SSSP, true); break;
+    case CG_CC * 2: CG_EXPAND(ALG_CC, false); break;
+    case CG_CC * 2 + 1: CG_EXPAND(ALG_CC, true); break;
     default:
       return set_error(CG_ERR_UNSUPPORTED, "push expansion: algorithm %d", alg);
   }
+#undef CG_EXPAND
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

@@ -106,6 +106,10 @@ typedef struct cg_options {
                                      whole frontier every iteration */
   uint32_t pr_segment_log2;       /* PageRank source-window size 2^k vertices (cache blocking);
                                      0 = automatic, 31 = one window (plain pull) */
+  uint32_t atomic_return;         /* 1: activate a destination only when its atomic min returned a
+                                     larger value (exact first-visit counts); 0: fire-and-forget
+                                     atomics, activate on the strict-less pre-check (a superset,
+                                     same fixed point) */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

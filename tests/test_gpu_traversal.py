@@ -123,6 +123,22 @@ def test_sssp_delta_stepping(oracle, delta):
                 assert np.array_equal(got, oracle.sssp(og, root)), (delta, root, mode)
 
 
+def test_fire_and_forget_atomics(oracle):
+    """Activation on the pre-check (atomics without return) over-activates at
+    most; the values are still the fixed point."""
+    cases = [random_graph(oracle, 31, 40000, 300000, hub=40000), oracle.rmat_graph(16, 16, seed=7),
+             oracle.grid_graph(90, 0.1, seed=1)]
+    for og in cases:
+        g = gpu_graph(og)
+        root = oracle.max_outdeg_vertex(og)
+        for tier in (0, 6):
+            opts = dict(atomic_return=0, sched_mode=_lib.CG_SCHED_FORCED, forced_choice=tier)
+            assert np.array_equal(run(g, BFS, root, **opts)[0], oracle.bfs(og, root))
+            assert np.array_equal(run(g, SSSP, root, **opts)[0], oracle.sssp(og, root))
+            assert np.array_equal(run(g, CC, **opts)[0], oracle.cc_directed(og))
+        assert np.array_equal(run(g, SSSP, root, atomic_return=0, sssp_delta=50)[0], oracle.sssp(og, root))
+
+
 def test_gpu_generator_matches_oracle(oracle):
     for scale, ef, sym in ((12, 16, False), (13, 8, True)):
         og = oracle.rmat_graph(scale, ef, seed=5, symmetrize=sym)
