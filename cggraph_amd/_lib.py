@@ -95,6 +95,8 @@ SIGNATURES = {
     "cg_options_default": (_int, [ctypes.POINTER(Options)]),
     "cg_graph_from_csr": (_int, [_u32, _u64, _vp, _int, _vp, _vp, _u32, _pp]),
     "cg_graph_load_files": (_int, [_cp, _int, _cp, _cp, _u32, _u64, _pp]),
+    "cg_graph_from_edges": (_int, [_u32, _u64, _vp, _vp, _vp, _int, _pp]),
+    "cg_graph_load_edge_file": (_int, [_cp, _u32, _int, _int, _pp]),
     "cg_graph_generate_rmat": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _u64, _u32, _u32, _int, _pp]),
     "cg_graph_generate_grid": (_int, [_u32, _dbl, _u64, _u32, _u32, _pp]),
     "cg_graph_info": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64), ctypes.POINTER(_int),

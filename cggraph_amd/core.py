@@ -40,6 +40,23 @@ class Graph:
         return cls(h)
 
     @classmethod
+    def from_edges(cls, V, src, dst, weight=None, symmetrize=False):
+        s = np.ascontiguousarray(src, dtype=np.uint32)
+        d = np.ascontiguousarray(dst, dtype=np.uint32)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().cg_graph_from_edges(int(V), len(s), _arr_ptr(s), _arr_ptr(d), _arr_ptr(w), int(symmetrize),
+                                        ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_edge_file(cls, path, V, has_weights=True, symmetrize=False):
+        h = ctypes.c_void_p()
+        check(lib().cg_graph_load_edge_file(path.encode(), int(V), int(has_weights), int(symmetrize),
+                                            ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
     def rmat(cls, scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1, weights=(1, 100), symmetrize=False):
         lo, hi = weights if weights is not None else (0, 0)
         h = ctypes.c_void_p()

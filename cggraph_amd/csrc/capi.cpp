@@ -253,6 +253,44 @@ int cg_graph_load_files(const char* offset_file, int offsets_u64, const char* ds
   return CG_OK;
 }
 
+int cg_graph_from_edges(uint32_t V, uint64_t m, const uint32_t* src, const uint32_t* dst, const uint32_t* w,
+                        int symmetrize, cg_graph** out) {
+  clear_error();
+  if (!out || (m && (!src || !dst))) return set_error(CG_ERR_INVALID, "NULL argument");
+  std::unique_ptr<cg_graph> g(new (std::nothrow) cg_graph());
+  if (!g) return set_error(CG_ERR_OOM, "graph object");
+  CG_TRY(build_from_edges(src, dst, w, m, V, symmetrize, g.get()));
+  *out = g.release();
+  return CG_OK;
+}
+
+int cg_graph_load_edge_file(const char* path, uint32_t V, int has_weights, int symmetrize, cg_graph** out) {
+  clear_error();
+  if (!path || !out) return set_error(CG_ERR_INVALID, "NULL argument");
+  struct stat stt;
+  if (::stat(path, &stt) != 0) return set_error(CG_ERR_IO, "cannot open %s", path);
+  const uint64_t rec = has_weights ? 12 : 8;
+  if ((uint64_t)stt.st_size % rec)
+    return set_error(CG_ERR_IO, "%s holds %lld bytes, not a whole number of %llu-byte records", path,
+                     (long long)stt.st_size, (unsigned long long)rec);
+  const uint64_t m = (uint64_t)stt.st_size / rec;
+  std::vector<uint32_t> raw;
+  try {
+    raw.resize(m * (rec / 4));
+  } catch (const std::bad_alloc&) {
+    return set_error(CG_ERR_OOM, "edge file buffer");
+  }
+  CG_TRY(read_file(path, raw.data(), (uint64_t)stt.st_size));
+  std::vector<uint32_t> s(m), d(m), w(has_weights ? m : 0);
+  const uint64_t k = rec / 4;
+  for (uint64_t i = 0; i < m; ++i) {
+    s[i] = raw[i * k];
+    d[i] = raw[i * k + 1];
+    if (has_weights) w[i] = raw[i * k + 2];
+  }
+  return cg_graph_from_edges(V, m, s.data(), d.data(), has_weights ? w.data() : nullptr, symmetrize, out);
+}
+
 int cg_graph_generate_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
                            uint32_t wlo, uint32_t whi, int symmetrize, cg_graph** out) {
   clear_error();

@@ -312,6 +312,141 @@ int gen_grid(uint32_t n, double drop, uint64_t seed, uint32_t wlo, uint32_t whi,
   return device_max_outdeg(g);
 }
 
+namespace {
+
+__global__ void k_edge_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                            const uint32_t* __restrict__ w, uint64_t m, uint32_t bits, int sym,
+                            uint64_t* __restrict__ keys, uint32_t* __restrict__ wk) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = src[i], d = dst[i];
+    const uint32_t wi = w ? w[i] : 0u;
+    if (sym) {
+      keys[2 * i] = (s << bits) | d;
+      keys[2 * i + 1] = (d << bits) | s;
+      if (wk) { wk[2 * i] = wi; wk[2 * i + 1] = wi; }
+    } else {
+      keys[i] = (s << bits) | d;
+      if (wk) wk[i] = wi;
+    }
+  }
+}
+
+// For every first occurrence of a key: the smallest weight among its copies
+// (shortest paths only ever use the lightest of parallel edges).
+__global__ void k_run_min(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ wk,
+                          const uint32_t* __restrict__ flags, const uint64_t* __restrict__ pos, uint64_t n,
+                          uint32_t* __restrict__ w_out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (!flags[i]) continue;
+    uint32_t m = wk[i];
+    for (uint64_t j = i + 1; j < n && keys[j] == keys[i]; ++j) m = wk[j] < m ? wk[j] : m;
+    w_out[pos[i]] = m;
+  }
+}
+
+}  // namespace
+
+int build_from_edges(const uint32_t* h_src, const uint32_t* h_dst, const uint32_t* h_w, uint64_t m, uint32_t V,
+                     int symmetrize, cg_graph* g) {
+  if (V == 0) return set_error(CG_ERR_INVALID, "V must be positive");
+  CG_TRY(require_device());
+  for (uint64_t i = 0; i < m; ++i)
+    if (h_src[i] >= V || h_dst[i] >= V)
+      return set_error(CG_ERR_INVALID, "edge %llu (%u -> %u) has an endpoint >= V", (unsigned long long)i, h_src[i],
+                       h_dst[i]);
+  if (h_w)
+    for (uint64_t i = 0; i < m; ++i)
+      if (h_w[i] > 0xFFFFFFFFu - kUnreached)
+        return set_error(CG_ERR_INVALID, "weight %u of edge %llu would let value + weight wrap", h_w[i],
+                         (unsigned long long)i);
+  hipStream_t s = nullptr;
+  const uint32_t bits = bits_for(V);
+  const uint64_t n = symmetrize ? 2 * m : m;
+  DevBuf<uint32_t> ds, dd, dw;
+  CG_TRY(ds.ensure(m));
+  CG_TRY(dd.ensure(m));
+  if (m) {
+    CG_HIP(hipMemcpy(ds.p, h_src, m * 4, hipMemcpyHostToDevice));
+    CG_HIP(hipMemcpy(dd.p, h_dst, m * 4, hipMemcpyHostToDevice));
+  }
+  if (h_w) {
+    CG_TRY(dw.ensure(m));
+    if (m) CG_HIP(hipMemcpy(dw.p, h_w, m * 4, hipMemcpyHostToDevice));
+  }
+  DevBuf<uint64_t> keys, kalt;
+  DevBuf<uint32_t> wk, walt;
+  CG_TRY(keys.ensure(n));
+  CG_TRY(kalt.ensure(n));
+  if (h_w) {
+    CG_TRY(wk.ensure(n));
+    CG_TRY(walt.ensure(n));
+  }
+  hipLaunchKernelGGL(k_edge_keys, dim3(grid_of(m)), dim3(kBlock), 0, s, ds.p, dd.p, h_w ? dw.p : nullptr, m, bits,
+                     symmetrize, keys.p, h_w ? wk.p : nullptr);
+  LAUNCH_OK();
+  ds.reset();
+  dd.reset();
+  dw.reset();
+  rocprim::double_buffer<uint64_t> kb(keys.p, kalt.p);
+  size_t tb = 0;
+  DevBuf<uint8_t> tmp;
+  if (h_w) {
+    rocprim::double_buffer<uint32_t> vb(wk.p, walt.p);
+    CG_HIP(rocprim::radix_sort_pairs(nullptr, tb, kb, vb, n, 0, 2 * bits, s));
+    CG_TRY(tmp.ensure(tb));
+    CG_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kb, vb, n, 0, 2 * bits, s));
+    if (vb.current() != wk.p) std::swap(wk, walt);
+  } else {
+    CG_HIP(rocprim::radix_sort_keys(nullptr, tb, kb, n, 0, 2 * bits, s));
+    CG_TRY(tmp.ensure(tb));
+    CG_HIP(rocprim::radix_sort_keys(tmp.p, tb, kb, n, 0, 2 * bits, s));
+  }
+  if (kb.current() != keys.p) std::swap(keys, kalt);
+  // from here: keys.p sorted, wk.p weights in the same order, kalt.p free
+  DevBuf<uint32_t> flags;
+  DevBuf<uint64_t> pos;
+  CG_TRY(flags.ensure(n));
+  CG_TRY(pos.ensure(n));
+  hipLaunchKernelGGL(k_mark, dim3(grid_of(n)), dim3(kBlock), 0, s, keys.p, n, bits, flags.p);
+  LAUNCH_OK();
+  size_t sb = 0;
+  CG_HIP(rocprim::exclusive_scan(nullptr, sb, flags.p, pos.p, (uint64_t)0, n, rocprim::plus<uint64_t>(), s));
+  DevBuf<uint8_t> st;
+  CG_TRY(st.ensure(sb));
+  CG_HIP(rocprim::exclusive_scan(st.p, sb, flags.p, pos.p, (uint64_t)0, n, rocprim::plus<uint64_t>(), s));
+  uint64_t last_pos = 0;
+  uint32_t last_flag = 0;
+  if (n) {
+    CG_HIP(hipMemcpyAsync(&last_pos, pos.p + n - 1, 8, hipMemcpyDeviceToHost, s));
+    CG_HIP(hipMemcpyAsync(&last_flag, flags.p + n - 1, 4, hipMemcpyDeviceToHost, s));
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  const uint64_t E = n ? last_pos + last_flag : 0;
+  g->V = V;
+  g->E = E;
+  CG_TRY(g->d_off.ensure((uint64_t)V + 1));
+  CG_TRY(g->d_dst.ensure(E));
+  hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(kBlock), 0, s, keys.p, flags.p, pos.p, n, bits, kalt.p,
+                     g->d_dst.p);
+  LAUNCH_OK();
+  g->has_w = h_w != nullptr;
+  if (g->has_w) {
+    CG_TRY(g->d_w.ensure(E));
+    hipLaunchKernelGGL(k_run_min, dim3(grid_of(n)), dim3(kBlock), 0, s, keys.p, wk.p, flags.p, pos.p, n, g->d_w.p);
+    LAUNCH_OK();
+  }
+  hipLaunchKernelGGL(k_offsets, dim3(grid_of((uint64_t)V + 1)), dim3(kBlock), 0, s, kalt.p, E, V, bits, g->d_off.p);
+  LAUNCH_OK();
+  CG_HIP(hipStreamSynchronize(s));
+  g->dev_valid = true;
+  g->host_valid = false;
+  g->symmetric = symmetrize != 0;
+  g->zero_tail = 0;
+  return device_max_outdeg(g);
+}
+
 int build_transpose(cg_graph* g) {
   if (g->dev_t_valid) return CG_OK;
   CG_TRY(g->ensure_device());

@@ -142,6 +142,8 @@ int gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
              uint32_t wlo, uint32_t whi, int symmetrize, cg_graph* g);
 int gen_grid(uint32_t n, double drop, uint64_t seed, uint32_t wlo, uint32_t whi, cg_graph* g);
 int build_transpose(cg_graph* g);
+int build_from_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* w, uint64_t m, uint32_t V,
+                     int symmetrize, cg_graph* g);
 int device_max_outdeg(cg_graph* g);
 
 }  // namespace cg

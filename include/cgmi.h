@@ -156,6 +156,16 @@ int cg_graph_from_csr(uint32_t V, uint64_t E, const void* offsets, int offsets_u
  * csrWeightFile); w_file may be NULL.  Sizes are checked as load_binFile does. */
 int cg_graph_load_files(const char* offset_file, int offsets_u64, const char* dst_file,
                         const char* w_file, uint32_t V, uint64_t E, cg_graph** g);
+/* Edge list -> CSR on the GPU: self-loops dropped, parallel edges merged
+ * keeping the smallest weight, rows ascending; symmetrize != 0 adds every
+ * reverse edge.  w may be NULL.  (GraphBinReader::group_load_directedGraph +
+ * sort_nbr + build_csr, Basic/Graph/graphBinReader.hpp:89-326.) */
+int cg_graph_from_edges(uint32_t V, uint64_t m, const uint32_t* src, const uint32_t* dst, const uint32_t* w,
+                        int symmetrize, cg_graph** g);
+/* The reference's binary edge file: packed little-endian records
+ * {u32 src, u32 dst, u32 weight} (12 bytes, graphBinReader.hpp:35-39) when
+ * has_weights != 0, {u32 src, u32 dst} otherwise. */
+int cg_graph_load_edge_file(const char* path, uint32_t V, int has_weights, int symmetrize, cg_graph** g);
 /* R-MAT on the GPU: 2^scale vertices, edge_factor * 2^scale generated edges,
  * self-loops and duplicates removed, neighbours ascending; weights uniform in
  * [wlo, whi] (whi == 0: no weights); symmetrize != 0 adds every reverse edge. */

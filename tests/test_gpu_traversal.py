@@ -248,3 +248,38 @@ def test_error_paths(oracle):
     with pytest.raises(cg.CgError):
         e.result()  # a failed run leaves no result behind
     e.close()
+
+
+def test_edge_list_ingest(oracle, tmp_path):
+    """The reference's packed {src, dst, w} edge records -> CSR on the GPU:
+    self-loops dropped, parallel edges merged with their smallest weight."""
+    rng = np.random.default_rng(12)
+    V, m = 3000, 40000
+    src = rng.integers(0, V, m).astype(np.uint32)
+    dst = rng.integers(0, V, m).astype(np.uint32)
+    src[:500] = dst[:500]  # self-loops
+    src[500:1500], dst[500:1500] = src[1500:2500], dst[1500:2500]  # parallel edges
+    w = rng.integers(1, 1000, m).astype(np.uint32)
+    rec = np.stack([src, dst, w], axis=1).astype("<u4")
+    path = str(tmp_path / "edges.bin")
+    rec.tofile(path)
+    for sym in (False, True):
+        g = cg.Graph.load_edge_file(path, V, has_weights=True, symmetrize=sym)
+        off, d2, w2 = g.export()
+        ref = oracle.csr_build(V, src, dst, symmetrize=sym)
+        assert np.array_equal(off, ref.offsets) and np.array_equal(d2, ref.dest)
+        # expected weight: minimum over all copies of (u, v) (and (v, u) when symmetrized)
+        ss, dd, ww = (np.concatenate([src, dst]), np.concatenate([dst, src]), np.concatenate([w, w])) if sym \
+            else (src, dst, w)
+        best = {}
+        for a, b, c in zip(ss.tolist(), dd.tolist(), ww.tolist()):
+            if a != b:
+                best[(a, b)] = min(best.get((a, b), 1 << 31), c)
+        rows = np.repeat(np.arange(V), np.diff(off).astype(np.int64))
+        assert [best[(a, b)] for a, b in zip(rows.tolist(), d2.tolist())] == w2.tolist()
+        ref.weight = w2
+        root = int(src[2000])
+        s, _ = run(g, SSSP, root)
+        assert np.array_equal(s, oracle.sssp(ref, root))
+    g = cg.Graph.from_edges(V, src, dst)  # no weights
+    assert np.array_equal(g.export()[1], oracle.csr_build(V, src, dst).dest)
