@@ -87,9 +87,27 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="run the batched mixed workload once (for rocprofv3 kernel traces)")
+    ap.add_argument("--profile-pr", action="store_true",
+                    help="run PageRank on R-MAT scale 25 / ef44 once (for rocprofv3 kernel traces)")
     args = ap.parse_args()
     if args.profile_run:
         profile_workload(args)
+        return
+    if args.profile_batch or args.profile_pr:
+        import cggraph_amd as cg
+        from cggraph_amd import autotune, batch
+        if args.profile_batch:
+            graphs, qg, qa, qr = batch.mixed_workload()
+            opts = autotune.best_options("batch/mixed") or {"sssp_delta": 10000}
+            _, st = batch.run_batch(qg, qa, qr, copy_out=False, **opts)
+            print(json.dumps({"profile_batch": st["ms"], "levels": st["iterations"]}), flush=True)
+        if args.profile_pr:
+            g = cg.Graph.rmat(25, 44, seed=1, weights=None)
+            e = cg.Engine(g, 3)
+            _, st = e.run(0, copy_out=False)
+            print(json.dumps({"profile_pr_ms": st["ms"]}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
