@@ -114,6 +114,7 @@ SIGNATURES = {
     "cg_graph_device_csr": (_int, [_vp, _pp, _pp, _pp]),
     "cg_batch_run": (_int, [_vp, _vp, _vp, _int, _vp, _int, ctypes.POINTER(Options),
                             ctypes.POINTER(RunStats)]),
+    "cg_engine_run_batch": (_int, [_vp, _vp, _int, _vp, ctypes.POINTER(RunStats)]),
     "cg_mlp_train": (_int, [_cp, _cp, _int, _flt, _u64, ctypes.POINTER(_flt)]),
     "cg_mlp_forward": (_int, [_vp, _vp, _int, _vp]),
     "cg_mlp_loss_grad": (_int, [_vp, _vp, _vp, _int, _vp, _vp]),

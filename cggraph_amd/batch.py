@@ -33,6 +33,21 @@ def run_batch(graphs, algorithms, roots, mode=BATCHED, copy_out=True, **opt):
     return arrays, st.as_dict()
 
 
+def run_engine_batch(engines, roots, copy_out=True):
+    """cg_engine_run_batch: query i is engines[i]'s algorithm on its graph."""
+    nq = len(roots)
+    E = (ctypes.c_void_p * nq)(*[e._h.value for e in engines])
+    R = (ctypes.c_uint32 * nq)(*[int(r) for r in roots])
+    arrays, outs = None, None
+    if copy_out:
+        arrays = [np.empty(e.V, np.uint32) for e in engines]
+        outs = (ctypes.c_void_p * nq)(*[a.ctypes.data for a in arrays])
+    st = _lib.RunStats()
+    check(lib().cg_engine_run_batch(ctypes.cast(E, ctypes.c_void_p), ctypes.cast(R, ctypes.c_void_p), nq,
+                                    None if outs is None else ctypes.cast(outs, ctypes.c_void_p), ctypes.byref(st)))
+    return arrays, st.as_dict()
+
+
 def mixed_workload(n_bfs=256, n_sfs=256, rmat_scales=range(14, 23), grid_sides=(256, 512, 1024, 2048),
                    seed=1, weights=(1, 1000)):
     """Builds the graphs on the GPU and returns (graphs, per-query graph list,

@@ -433,3 +433,23 @@ extern "C" int cg_batch_run(cg_graph* const* graphs, const int* algorithms, cons
   if (mode == 1) return cg::run_one_per_launch(graphs, algorithms, roots, nq, outs, o, stats);
   return cg::set_error(CG_ERR_INVALID, "batch mode %d", mode);
 }
+
+// Engine-handle form: query i runs engine i's algorithm on engine i's graph;
+// the batch takes engines[0]'s options (block, grid, levels, SSSP delta).
+extern "C" int cg_engine_run_batch(cg_engine* const* engines, const uint32_t* roots, int nq, void* const* outs,
+                                   cg_run_stats* stats) {
+  cg::clear_error();
+  if (!engines || !roots || nq <= 0) return cg::set_error(CG_ERR_INVALID, "bad batch arguments");
+  std::vector<cg_graph*> graphs(nq);
+  std::vector<int> algs(nq);
+  std::vector<uint32_t*> o(nq, nullptr);
+  for (int i = 0; i < nq; ++i) {
+    if (!engines[i]) return cg::set_error(CG_ERR_INVALID, "engine %d is NULL", i);
+    if (engines[i]->alg != CG_BFS && engines[i]->alg != CG_SSSP)
+      return cg::set_error(CG_ERR_UNSUPPORTED, "engine %d: batched queries are BFS or SSSP", i);
+    graphs[i] = engines[i]->g;
+    algs[i] = engines[i]->alg;
+    if (outs) o[i] = static_cast<uint32_t*>(outs[i]);
+  }
+  return cg::run_batched(graphs.data(), algs.data(), roots, nq, o.data(), engines[0]->opt, stats);
+}

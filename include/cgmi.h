@@ -217,6 +217,13 @@ int cg_graph_device_csr(const cg_graph* g, const uint64_t** offsets, const uint3
  * persistent kernel; 1: one query per launch (the comparison path). */
 int cg_batch_run(cg_graph* const* graphs, const int* algorithms, const uint32_t* roots, int nq,
                  uint32_t* const* outs, int mode, const cg_options* opt, cg_run_stats* stats);
+/* The same batched run over engine handles: query i is engine i's algorithm
+ * (BFS or SSSP) on engine i's graph from roots[i]; outs[i] is a host
+ * uint32[V_i] buffer or NULL.  The batch takes engines[0]'s options.  Each
+ * engine's own state is not touched, so engines may repeat in the list.
+ * (Many CG_co_execution calls, CGgraph_engine.cuh:265, issued as one.) */
+int cg_engine_run_batch(cg_engine* const* engines, const uint32_t* roots, int nq, void* const* outs,
+                        cg_run_stats* stats);
 
 /* ---- learned scheduler ---- */
 /* Trains the MLP (12 -> 64 -> 64 -> 16, ReLU, softmax) on the GPU from a

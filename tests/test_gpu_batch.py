@@ -71,3 +71,23 @@ def test_batch_delta_stepping_mixed(oracle):
         outs, st = batch.run_batch(qg, qa, qr, sssp_delta=delta, batch_levels=8)
         for o, a, r, got in zip(qo, qa, qr, outs):
             assert np.array_equal(got, oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)), (delta, a, r)
+
+
+def test_engine_handle_batch(oracle):
+    """cg_engine_run_batch: engines repeat in the list, each query takes its
+    engine's algorithm and graph; a PR engine is refused."""
+    ogs = [oracle.rmat_graph(10, 8, seed=7, weights=(1, 100)), oracle.grid_graph(24, 0.1, seed=3)]
+    gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
+    engs = [cg.Engine(gs[0], 0), cg.Engine(gs[0], 1), cg.Engine(gs[1], 1)]
+    owner = [0, 1, 2, 1, 0, 2, 2]
+    rng = np.random.default_rng(11)
+    roots = [int(rng.integers(0, ogs[0 if k < 2 else 1].V)) for k in owner]
+    outs, st = batch.run_engine_batch([engs[k] for k in owner], roots)
+    for k, r, got in zip(owner, roots, outs):
+        o = ogs[0 if k < 2 else 1]
+        assert np.array_equal(got, oracle.bfs(o, r) if k == 0 else oracle.sssp(o, r)), (k, r)
+    assert st["edges_traversed"] == sum(oracle.reached_edges(ogs[0 if k < 2 else 1], x)
+                                        for k, x in zip(owner, outs))
+    pr = cg.Engine(gs[0], 3)
+    with pytest.raises(cg.CgError):
+        batch.run_engine_batch([engs[0], pr], [0, 0])
