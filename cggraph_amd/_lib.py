@@ -51,6 +51,7 @@ class Options(ctypes.Structure):
         ("sssp_delta", ctypes.c_uint32),
         ("pr_segment_log2", ctypes.c_uint32),
         ("atomic_return", ctypes.c_uint32),
+        ("pr_layout", ctypes.c_uint32),
     ]
 
 

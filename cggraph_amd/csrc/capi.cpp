@@ -192,6 +192,7 @@ int cg_options_default(cg_options* o) {
   o->sssp_delta = 0;
   o->pr_segment_log2 = 0;
   o->atomic_return = 1;
+  o->pr_layout = 0;
   return CG_OK;
 }
 
@@ -453,7 +454,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->contrib.ensure(V));
     CG_TRY(e->contrib2.ensure(V));
     DevGraph dg{g->d_off.p, g->d_dst.p, nullptr, g->d_in_off.p, g->d_in_src.p, g->V};
-    CG_TRY(pr_plan_create(dg, g->E, e->opt.pr_segment_log2, &e->plan, e->stream));
+    CG_TRY(pr_plan_create(dg, g->E, e->opt.pr_layout, e->opt.pr_segment_log2, &e->plan, e->stream));
   } else {
     CG_TRY(e->val.ensure(V));
     CG_TRY(e->bm[0].ensure(words));

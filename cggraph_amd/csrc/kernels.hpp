@@ -129,9 +129,13 @@ int wcc_hooking(const DevGraph& g, uint32_t* parent, uint32_t sample_rounds, hip
                 uint64_t* edges_relaxed);
 
 // ---- PageRank (pagerank.hip) ----
-struct PrPlan;  // per-graph layout for the pull kernels (tiers or source windows)
-// seg_log2: source-window size 2^k (0 automatic, >= 31 plain pull).
-int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t seg_log2, PrPlan** plan, hipStream_t s);
+struct PrPlan;  // per-graph layout for the pull kernels (tiers, relabelled CSC or source windows)
+// layout: 0 automatic, 1 plain pull, 2 pull over a CSC renumbered by out-degree,
+// 3 source windows of 2^seg_log2 vertices.  Automatic: seg_log2 in [1, 30]
+// asks for windows, 31 for plain pull, 0 picks relabelled pull above 2^16
+// vertices.
+int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t layout, uint32_t seg_log2, PrPlan** plan,
+                   hipStream_t s);
 void pr_plan_destroy(PrPlan* plan);
 int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* rank, float* tmp_rank,
            float* contrib, float* contrib2, hipStream_t s, int profile, double* kernel_ms,

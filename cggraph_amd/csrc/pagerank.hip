@@ -7,22 +7,26 @@
 // d = 0.85 (:543), dangling mass dropped.  contrib(u) = PR(u)/outdeg(u) is
 // written by the kernel that produces PR(u), so each iteration reads it once.
 //
-// Two layouts:
-//  plain       (small graphs) one sweep over the CSC per iteration, tiered by
-//              in-degree: one lane / one wavefront / one workgroup per vertex.
-//  segmented   (large graphs) cache blocking of the source side.  The sources
-//              are cut into windows of 2^k vertices whose contrib slice fits in
-//              an XCD's L2; the edges are reordered by (window, destination,
-//              source) once per graph, so window s is a list of runs "edges of
-//              destination d from window s".  One iteration walks the windows
-//              in order; a run's partial sum (sequential in source order, or a
-//              fixed wavefront tree for long runs) is added to sum[d], and a
-//              final sweep turns sum into PR and contrib.  The gathers of a
-//              window hit L2 instead of the 134 MB contrib array in the
-//              Infinity Cache / HBM; the price is one read-modify-write of
-//              sum[d] per (window, d) pair.
-// Both layouts fix every summation order from the graph alone, so results are
-// bit-identical from run to run.
+// Three layouts of the in-edges (CSC), chosen once per graph:
+//  plain       one sweep over the transposed graph per iteration, rows tiered
+//              by in-degree: one lane / one wavefront / one workgroup per row.
+//  relabelled  the same sweep over a copy of the CSC whose vertex ids are
+//              renumbered by descending out-degree (ties by id).  On skewed
+//              graphs a small prefix of the ids then carries most edges'
+//              sources (R-MAT: the top 1/16 of vertices are the source of
+//              ~85 % of edges), so the contrib values those gathers read sit
+//              in a few MB of contiguous lines that stay in L2 / Infinity
+//              Cache instead of being spread over the whole array.  The ranks
+//              live in the new numbering while iterating and are scattered
+//              back to the caller's ids once at the end.
+//  segmented   cache blocking of the source side: sources are cut into
+//              windows of 2^k vertices; edges are ordered by (window,
+//              destination, source), so window s is a list of runs "edges of
+//              destination d from window s"; a run's partial sum (sequential,
+//              or a fixed wavefront tree for long runs) is added to sum[d]
+//              window after window, and a final sweep turns sum into PR.
+// Every layout fixes each row's summation order from the graph alone, so
+// results are bit-identical from run to run.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -33,12 +37,19 @@
 
 namespace cg {
 
+enum { kLayoutAuto = 0, kLayoutPlain = 1, kLayoutRelabel = 2, kLayoutSegmented = 3 };
+
 struct PrPlan {
-  // plain pull tiers
+  int layout = kLayoutPlain;
+  // plain and relabelled pull: tiers over the rows of the CSC in use
   uint32_t n_med = 0, n_high = 0;
   DevBuf<uint32_t> med, high;
+  DevBuf<uint32_t> deg;  // out-degree of each vertex (in the numbering in use)
+  // relabelled pull
+  DevBuf<uint32_t> perm;  // new id -> caller's id
+  DevBuf<uint64_t> r_off;
+  DevBuf<uint32_t> r_src;
   // segmented pull
-  bool segmented = false;
   uint32_t shift = 0, nseg = 0;
   uint64_t R = 0;
   DevBuf<uint32_t> src, run_dst, run_off, long_runs;
@@ -52,6 +63,14 @@ constexpr int kBlock = 256;
 constexpr uint32_t kMed = 32, kHigh = 8192;
 constexpr uint32_t kLongRun = 64;
 
+// The in-edges the pull kernels read: row v's sources are src[off[v]..off[v+1]).
+struct Csc {
+  const uint64_t* off;
+  const uint32_t* src;
+  const uint32_t* deg;  // out-degree, for contrib = rank / deg
+  uint32_t V;
+};
+
 unsigned grid_of(uint64_t n) {
   uint64_t b = (n + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -64,6 +83,8 @@ uint32_t bits_for(uint64_t V) {
   return b;
 }
 
+__device__ __forceinline__ uint32_t ld_stream(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+
 __global__ void k_classify(const uint64_t* __restrict__ in_off, uint32_t V, uint32_t* __restrict__ med,
                            uint32_t* __restrict__ high, uint32_t* __restrict__ cnt) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
@@ -74,68 +95,151 @@ __global__ void k_classify(const uint64_t* __restrict__ in_off, uint32_t V, uint
   }
 }
 
+__global__ void k_out_degree(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ deg) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    deg[v] = (uint32_t)(off[v + 1] - off[v]);
+}
+
 __device__ __forceinline__ void finish(uint32_t v, float sum, float base, float damping,
-                                       const uint64_t* __restrict__ off, float* __restrict__ rank,
+                                       const uint32_t* __restrict__ deg, float* __restrict__ rank,
                                        float* __restrict__ contrib) {
   const float r = base + damping * sum;
   rank[v] = r;
-  const uint64_t deg = off[v + 1] - off[v];
-  contrib[v] = deg ? r / (float)deg : 0.0f;
+  const uint32_t d = deg[v];
+  contrib[v] = d ? r / (float)d : 0.0f;
 }
 
-__global__ __launch_bounds__(kBlock) void k_pr_low(DevGraph g, const float* __restrict__ contrib_in,
-                                                   float base, float damping, float* __restrict__ rank,
+// Rows with fewer than kMed in-edges: one lane per row, sources in order.
+__global__ __launch_bounds__(kBlock) void k_pr_low(Csc g, const float* __restrict__ contrib_in, float base,
+                                                   float damping, float* __restrict__ rank,
                                                    float* __restrict__ contrib_out) {
   const uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   if (v >= g.V) return;
-  const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
+  const uint64_t s = g.off[v], e = g.off[v + 1];
   if (e - s >= kMed) return;
   float sum = 0.0f;
-  for (uint64_t k = s; k < e; ++k) sum += contrib_in[g.in_src[k]];
-  finish((uint32_t)v, sum, base, damping, g.off, rank, contrib_out);
+  for (uint64_t k = s; k < e; ++k) sum += contrib_in[ld_stream(g.src + k)];
+  finish((uint32_t)v, sum, base, damping, g.deg, rank, contrib_out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pr_med(DevGraph g, const uint32_t* __restrict__ list,
-                                                   uint32_t n, const float* __restrict__ contrib_in,
-                                                   float base, float damping, float* __restrict__ rank,
+// Lane-strided partial sum over [s, e): the lane adds sources s+lane,
+// s+lane+stride, ... in that order; four sources are fetched ahead so four
+// gathers are in flight per lane.
+template <int STRIDE>
+__device__ __forceinline__ float strided_sum(const uint32_t* __restrict__ src, const float* __restrict__ contrib,
+                                             uint64_t s, uint64_t e, uint32_t lane) {
+  float sum = 0.0f;
+  uint64_t k = s + lane;
+  for (; k + 3 * STRIDE < e; k += 4 * STRIDE) {
+    const uint32_t u0 = ld_stream(src + k), u1 = ld_stream(src + k + STRIDE);
+    const uint32_t u2 = ld_stream(src + k + 2 * STRIDE), u3 = ld_stream(src + k + 3 * STRIDE);
+    const float c0 = contrib[u0], c1 = contrib[u1], c2 = contrib[u2], c3 = contrib[u3];
+    sum += c0;
+    sum += c1;
+    sum += c2;
+    sum += c3;
+  }
+  for (; k < e; k += STRIDE) sum += contrib[ld_stream(src + k)];
+  return sum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pr_med(Csc g, const uint32_t* __restrict__ list, uint32_t n,
+                                                   const float* __restrict__ contrib_in, float base,
+                                                   float damping, float* __restrict__ rank,
                                                    float* __restrict__ contrib_out) {
   const uint64_t i = blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64;
   if (i >= n) return;  // wave-uniform
   const uint32_t lane = __lane_id();
   const uint32_t v = list[i];
-  const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
-  float sum = 0.0f;
-  for (uint64_t k = s + lane; k < e; k += 64) sum += contrib_in[g.in_src[k]];
+  float sum = strided_sum<64>(g.src, contrib_in, g.off[v], g.off[v + 1], lane);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
-  if (lane == 0) finish(v, sum, base, damping, g.off, rank, contrib_out);
+  if (lane == 0) finish(v, sum, base, damping, g.deg, rank, contrib_out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pr_high(DevGraph g, const uint32_t* __restrict__ list,
+__global__ __launch_bounds__(kBlock) void k_pr_high(Csc g, const uint32_t* __restrict__ list,
                                                     const float* __restrict__ contrib_in, float base,
                                                     float damping, float* __restrict__ rank,
                                                     float* __restrict__ contrib_out) {
   __shared__ float part[kBlock];
   const uint32_t v = list[blockIdx.x];
-  const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
-  float sum = 0.0f;
-  for (uint64_t k = s + threadIdx.x; k < e; k += kBlock) sum += contrib_in[g.in_src[k]];
-  part[threadIdx.x] = sum;
+  part[threadIdx.x] = strided_sum<kBlock>(g.src, contrib_in, g.off[v], g.off[v + 1], threadIdx.x);
   __syncthreads();
   for (int w = kBlock / 2; w >= 1; w >>= 1) {
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) finish(v, part[0], base, damping, g.off, rank, contrib_out);
+  if (threadIdx.x == 0) finish(v, part[0], base, damping, g.deg, rank, contrib_out);
 }
 
-__global__ void k_pr_init(DevGraph g, float r0, float* __restrict__ rank, float* __restrict__ contrib) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < g.V;
+__global__ void k_pr_init(uint32_t V, const uint32_t* __restrict__ deg, float r0, float* __restrict__ rank,
+                          float* __restrict__ contrib) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     rank[v] = r0;
-    const uint64_t deg = g.off[v + 1] - g.off[v];
-    contrib[v] = deg ? r0 / (float)deg : 0.0f;
+    const uint32_t d = deg[v];
+    contrib[v] = d ? r0 / (float)d : 0.0f;
   }
+}
+
+// ---- relabelled layout construction ----
+
+// Sort key for descending out-degree; radix sort is stable, so ties keep
+// ascending ids.
+__global__ void k_degree_keys(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ key,
+                              uint32_t* __restrict__ id) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    key[v] = ~(uint32_t)(off[v + 1] - off[v]);
+    id[v] = (uint32_t)v;
+  }
+}
+
+__global__ void k_invert(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ key, uint32_t V,
+                         uint32_t* __restrict__ new_of, uint32_t* __restrict__ deg_new) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    new_of[perm[i]] = (uint32_t)i;
+    deg_new[i] = ~key[i];
+  }
+}
+
+// in-degree of new row i (the caller's vertex perm[i]); entry V is 0 so the
+// exclusive scan over V + 1 entries ends with E.
+__global__ void k_in_degree_new(const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ perm,
+                                uint32_t V, uint64_t* __restrict__ cnt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= V;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < V) {
+      const uint32_t v = perm[i];
+      cnt[i] = in_off[v + 1] - in_off[v];
+    } else {
+      cnt[i] = 0;
+    }
+  }
+}
+
+// key = new(v) << b | new(u) for edge u -> v
+__global__ void k_relabel_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
+                               const uint32_t* __restrict__ new_of, uint32_t b, uint64_t* __restrict__ keys) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    keys[e] = ((uint64_t)new_of[dst[e]] << b) | new_of[src[e]];
+}
+
+__global__ void k_low_bits(const uint64_t* __restrict__ keys, uint64_t E, uint32_t b, uint32_t* __restrict__ out) {
+  const uint64_t mask = (1ull << b) - 1;
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    out[e] = (uint32_t)(keys[e] & mask);
+}
+
+__global__ void k_unpermute(const uint32_t* __restrict__ perm, const float* __restrict__ in, uint32_t V,
+                            float* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[perm[i]] = in[i];
 }
 
 // ---- segmented layout construction ----
@@ -245,11 +349,11 @@ __global__ __launch_bounds__(kBlock) void k_seg_long(const uint32_t* __restrict_
   if (lane == 0) sum[run_dst[r]] += p;
 }
 
-__global__ void k_seg_finish(DevGraph g, float base, float damping, float* __restrict__ sum, float* __restrict__ rank,
-                             float* __restrict__ contrib) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < g.V;
+__global__ void k_seg_finish(uint32_t V, const uint32_t* __restrict__ deg, float base, float damping,
+                             float* __restrict__ sum, float* __restrict__ rank, float* __restrict__ contrib) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
-    finish((uint32_t)v, sum[v], base, damping, g.off, rank, contrib);
+    finish((uint32_t)v, sum[v], base, damping, deg, rank, contrib);
     sum[v] = 0.0f;
   }
 }
@@ -262,15 +366,15 @@ __global__ void k_seg_finish(DevGraph g, float base, float damping, float* __res
                        __FILE__, __LINE__);                                             \
   } while (0)
 
-int build_plain(const DevGraph& g, PrPlan* p, hipStream_t s) {
+// med / high row lists of a CSC (any order: each row's sum is fixed by the row).
+int build_tiers(const uint64_t* in_off, uint32_t V, PrPlan* p, hipStream_t s) {
   DevBuf<uint32_t> cnt;
   CG_TRY(cnt.ensure(2));
-  CG_TRY(p->med.ensure(g.V ? g.V : 1));
-  CG_TRY(p->high.ensure(g.V ? g.V : 1));
+  CG_TRY(p->med.ensure(V ? V : 1));
+  CG_TRY(p->high.ensure(V ? V : 1));
   CG_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(uint32_t), s));
-  if (g.V) {
-    hipLaunchKernelGGL(k_classify, dim3(grid_of(g.V)), dim3(kBlock), 0, s, g.in_off, g.V, p->med.p, p->high.p,
-                       cnt.p);
+  if (V) {
+    hipLaunchKernelGGL(k_classify, dim3(grid_of(V)), dim3(kBlock), 0, s, in_off, V, p->med.p, p->high.p, cnt.p);
     PR_LAUNCH_OK();
   }
   uint32_t h[2] = {0, 0};
@@ -278,6 +382,103 @@ int build_plain(const DevGraph& g, PrPlan* p, hipStream_t s) {
   CG_HIP(hipStreamSynchronize(s));
   p->n_med = h[0];
   p->n_high = h[1];
+  return CG_OK;
+}
+
+int build_degrees(const DevGraph& g, PrPlan* p, hipStream_t s) {
+  CG_TRY(p->deg.ensure(g.V ? g.V : 1));
+  if (g.V) {
+    hipLaunchKernelGGL(k_out_degree, dim3(grid_of(g.V)), dim3(kBlock), 0, s, g.off, g.V, p->deg.p);
+    PR_LAUNCH_OK();
+  }
+  return CG_OK;
+}
+
+// Per-edge source ids of the forward CSR: heads of non-empty rows, then a
+// running maximum.
+int edge_sources(const DevGraph& g, uint64_t E, DevBuf<uint32_t>& srcv, hipStream_t s) {
+  DevBuf<uint32_t> head;
+  CG_TRY(head.ensure(E));
+  CG_TRY(srcv.ensure(E));
+  CG_HIP(hipMemsetAsync(head.p, 0, E * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_row_heads, dim3(grid_of(g.V)), dim3(kBlock), 0, s, g.off, g.V, head.p);
+  PR_LAUNCH_OK();
+  size_t sb = 0;
+  CG_HIP(rocprim::inclusive_scan(nullptr, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
+  DevBuf<uint8_t> st;
+  CG_TRY(st.ensure(sb));
+  CG_HIP(rocprim::inclusive_scan(st.p, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
+  CG_HIP(hipStreamSynchronize(s));
+  return CG_OK;
+}
+
+int build_relabelled(const DevGraph& g, uint64_t E, PrPlan* p, hipStream_t s) {
+  const uint32_t V = g.V;
+  const uint32_t b = bits_for(V);
+  DevBuf<uint32_t> new_of;
+  CG_TRY(p->perm.ensure(V));
+  CG_TRY(p->deg.ensure(V));
+  CG_TRY(new_of.ensure(V));
+  {
+    DevBuf<uint32_t> key, id, key_out;
+    CG_TRY(key.ensure(V));
+    CG_TRY(id.ensure(V));
+    CG_TRY(key_out.ensure(V));
+    hipLaunchKernelGGL(k_degree_keys, dim3(grid_of(V)), dim3(kBlock), 0, s, g.off, V, key.p, id.p);
+    PR_LAUNCH_OK();
+    size_t tb = 0;
+    CG_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.p, key_out.p, id.p, p->perm.p, (size_t)V, 0, 32, s));
+    DevBuf<uint8_t> tmp;
+    CG_TRY(tmp.ensure(tb ? tb : 1));
+    CG_HIP(rocprim::radix_sort_pairs(tmp.p, tb, key.p, key_out.p, id.p, p->perm.p, (size_t)V, 0, 32, s));
+    hipLaunchKernelGGL(k_invert, dim3(grid_of(V)), dim3(kBlock), 0, s, p->perm.p, key_out.p, V, new_of.p,
+                       p->deg.p);
+    PR_LAUNCH_OK();
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  // row offsets of the relabelled CSC
+  CG_TRY(p->r_off.ensure((uint64_t)V + 1));
+  {
+    DevBuf<uint64_t> cnt;
+    CG_TRY(cnt.ensure((uint64_t)V + 1));
+    hipLaunchKernelGGL(k_in_degree_new, dim3(grid_of((uint64_t)V + 1)), dim3(kBlock), 0, s, g.in_off, p->perm.p, V,
+                       cnt.p);
+    PR_LAUNCH_OK();
+    size_t sb = 0;
+    CG_HIP(rocprim::exclusive_scan(nullptr, sb, cnt.p, p->r_off.p, (uint64_t)0, (size_t)V + 1,
+                                   rocprim::plus<uint64_t>(), s));
+    DevBuf<uint8_t> st;
+    CG_TRY(st.ensure(sb ? sb : 1));
+    CG_HIP(rocprim::exclusive_scan(st.p, sb, cnt.p, p->r_off.p, (uint64_t)0, (size_t)V + 1,
+                                   rocprim::plus<uint64_t>(), s));
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  // sources of each relabelled row, ascending: sort (new(v), new(u)) keys
+  CG_TRY(p->r_src.ensure(E ? E : 1));
+  if (E) {
+    DevBuf<uint64_t> keys, alt;
+    CG_TRY(keys.ensure(E));
+    CG_TRY(alt.ensure(E));
+    {
+      DevBuf<uint32_t> srcv;
+      CG_TRY(edge_sources(g, E, srcv, s));
+      hipLaunchKernelGGL(k_relabel_keys, dim3(grid_of(E)), dim3(kBlock), 0, s, srcv.p, g.dst, E, new_of.p, b,
+                         keys.p);
+      PR_LAUNCH_OK();
+      CG_HIP(hipStreamSynchronize(s));
+    }
+    rocprim::double_buffer<uint64_t> db(keys.p, alt.p);
+    size_t tb = 0;
+    CG_HIP(rocprim::radix_sort_keys(nullptr, tb, db, (size_t)E, 0, 2 * b, s));
+    DevBuf<uint8_t> tmp;
+    CG_TRY(tmp.ensure(tb ? tb : 1));
+    CG_HIP(rocprim::radix_sort_keys(tmp.p, tb, db, (size_t)E, 0, 2 * b, s));
+    hipLaunchKernelGGL(k_low_bits, dim3(grid_of(E)), dim3(kBlock), 0, s, db.current(), E, b, p->r_src.p);
+    PR_LAUNCH_OK();
+    CG_HIP(hipStreamSynchronize(s));
+  }
+  CG_TRY(build_tiers(p->r_off.p, V, p, s));
+  p->layout = kLayoutRelabel;
   return CG_OK;
 }
 
@@ -292,17 +493,8 @@ int build_segmented(const DevGraph& g, uint64_t E, uint32_t shift, PrPlan* p, hi
   CG_TRY(keys.ensure(E));
   CG_TRY(alt.ensure(E));
   {
-    DevBuf<uint32_t> head, srcv;
-    CG_TRY(head.ensure(E));
-    CG_TRY(srcv.ensure(E));
-    CG_HIP(hipMemsetAsync(head.p, 0, E * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_row_heads, dim3(grid_of(V)), dim3(kBlock), 0, s, g.off, V, head.p);
-    PR_LAUNCH_OK();
-    size_t sb = 0;
-    CG_HIP(rocprim::inclusive_scan(nullptr, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
-    DevBuf<uint8_t> st;
-    CG_TRY(st.ensure(sb));
-    CG_HIP(rocprim::inclusive_scan(st.p, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
+    DevBuf<uint32_t> srcv;
+    CG_TRY(edge_sources(g, E, srcv, s));
     hipLaunchKernelGGL(k_seg_keys, dim3(grid_of(E)), dim3(kBlock), 0, s, srcv.p, g.dst, E, b, shift, keys.p);
     PR_LAUNCH_OK();
     CG_HIP(hipStreamSynchronize(s));
@@ -372,25 +564,44 @@ int build_segmented(const DevGraph& g, uint64_t E, uint32_t shift, PrPlan* p, hi
     p->seg_long[w] = (uint32_t)(std::lower_bound(hl.begin(), hl.end(), p->seg_run[w]) - hl.begin());
   CG_TRY(p->sum.ensure(V));
   CG_HIP(hipMemsetAsync(p->sum.p, 0, (size_t)V * sizeof(float), s));
+  CG_TRY(build_degrees(g, p, s));
   CG_HIP(hipStreamSynchronize(s));
-  p->segmented = true;
+  p->layout = kLayoutSegmented;
   return CG_OK;
 }
 
 }  // namespace
 
-int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t seg_log2, PrPlan** plan, hipStream_t s) {
-  PrPlan* p = new PrPlan();
+int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t layout, uint32_t seg_log2, PrPlan** plan,
+                   hipStream_t s) {
+  if (layout > kLayoutSegmented) return set_error(CG_ERR_INVALID, "pr_layout %u outside [0, 3]", layout);
   const uint32_t b = bits_for(g.V);
-  uint32_t shift = seg_log2 ? seg_log2 : 20;
-  const bool auto_seg = seg_log2 == 0;
-  bool seg = shift < 31 && (uint64_t)g.V > (1ull << shift) && E < 0xFFFFFFFFull && E > 0;
-  if (auto_seg) seg = seg && g.V > (1u << 21);
-  if (seg) {
-    const uint32_t nseg = (uint32_t)(((uint64_t)g.V + (1ull << shift) - 1) >> shift);
-    if (2 * b + bits_for(nseg) + 1 > 64) seg = false;
+  if (layout == kLayoutAuto) {
+    // an explicit window size asks for the segmented layout, 31 for plain pull;
+    // otherwise relabelled pull above 2^16 vertices
+    if (seg_log2 >= 1 && seg_log2 <= 30) layout = kLayoutSegmented;
+    else if (seg_log2 == 0 && g.V > (1u << 16) && E > 0 && 2 * b <= 64) layout = kLayoutRelabel;
+    else layout = kLayoutPlain;
   }
-  const int st = seg ? build_segmented(g, E, shift, p, s) : build_plain(g, p, s);
+  uint32_t shift = seg_log2 >= 1 && seg_log2 <= 30 ? seg_log2 : 20;
+  if (layout == kLayoutSegmented) {
+    const uint32_t nseg = (uint32_t)(((uint64_t)g.V + (1ull << shift) - 1) >> shift);
+    // segmented needs 32-bit run offsets and keys within 64 bits; plain otherwise
+    if (E == 0 || E >= 0xFFFFFFFFull || 2 * b + bits_for(nseg) + 1 > 64 || (uint64_t)g.V <= (1ull << shift))
+      layout = kLayoutPlain;
+  }
+  if (layout == kLayoutRelabel && (E == 0 || 2 * b > 64)) layout = kLayoutPlain;
+  PrPlan* p = new PrPlan();
+  int st = CG_OK;
+  if (layout == kLayoutSegmented) {
+    st = build_segmented(g, E, shift, p, s);
+  } else if (layout == kLayoutRelabel) {
+    st = build_relabelled(g, E, p, s);
+  } else {
+    st = build_degrees(g, p, s);
+    if (st == CG_OK) st = build_tiers(g.in_off, g.V, p, s);
+    p->layout = kLayoutPlain;
+  }
   if (st != CG_OK) {
     delete p;
     return st;
@@ -408,7 +619,10 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
   if (!V) return CG_OK;
   const float base = (1.0f - damping) / (float)V;
   const unsigned vblocks = (unsigned)(((uint64_t)V + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_pr_init, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, g,
+  const bool relabel = plan->layout == kLayoutRelabel;
+  const Csc csc = relabel ? Csc{plan->r_off.p, plan->r_src.p, plan->deg.p, V}
+                          : Csc{g.in_off, g.in_src, plan->deg.p, V};
+  hipLaunchKernelGGL(k_pr_init, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, V, plan->deg.p,
                      1.0f / (float)V, rank, contrib);
   float* r_cur = rank;
   float* r_nxt = tmp_rank;
@@ -422,7 +636,7 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
   double total = 0.0;
   for (int it = 0; it < iters; ++it) {
     if (profile) CG_HIP(hipEventRecord(a, s));
-    if (plan->segmented) {
+    if (plan->layout == kLayoutSegmented) {
       for (uint32_t w = 0; w < plan->nseg; ++w) {
         const uint32_t r0 = plan->seg_run[w], r1 = plan->seg_run[w + 1];
         if (r1 > r0)
@@ -433,15 +647,15 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
           hipLaunchKernelGGL(k_seg_long, dim3((l1 - l0 + 3) / 4), dim3(kBlock), 0, s, plan->src.p, plan->run_dst.p,
                              plan->run_off.p, plan->long_runs.p, l0, l1, c_cur, plan->sum.p);
       }
-      hipLaunchKernelGGL(k_seg_finish, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, g, base,
-                         damping, plan->sum.p, r_nxt, c_nxt);
+      hipLaunchKernelGGL(k_seg_finish, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, V,
+                         plan->deg.p, base, damping, plan->sum.p, r_nxt, c_nxt);
     } else {
-      hipLaunchKernelGGL(k_pr_low, dim3(vblocks), dim3(kBlock), 0, s, g, c_cur, base, damping, r_nxt, c_nxt);
+      hipLaunchKernelGGL(k_pr_low, dim3(vblocks), dim3(kBlock), 0, s, csc, c_cur, base, damping, r_nxt, c_nxt);
       if (plan->n_med)
-        hipLaunchKernelGGL(k_pr_med, dim3((plan->n_med + 3) / 4), dim3(kBlock), 0, s, g, plan->med.p, plan->n_med,
-                           c_cur, base, damping, r_nxt, c_nxt);
+        hipLaunchKernelGGL(k_pr_med, dim3((plan->n_med + 3) / 4), dim3(kBlock), 0, s, csc, plan->med.p,
+                           plan->n_med, c_cur, base, damping, r_nxt, c_nxt);
       if (plan->n_high)
-        hipLaunchKernelGGL(k_pr_high, dim3(plan->n_high), dim3(kBlock), 0, s, g, plan->high.p, c_cur, base,
+        hipLaunchKernelGGL(k_pr_high, dim3(plan->n_high), dim3(kBlock), 0, s, csc, plan->high.p, c_cur, base,
                            damping, r_nxt, c_nxt);
     }
     if (profile) {
@@ -453,6 +667,12 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
     }
     std::swap(r_cur, r_nxt);
     std::swap(c_cur, c_nxt);
+  }
+  if (relabel) {
+    // back to the caller's ids: r_nxt is free after the last swap
+    hipLaunchKernelGGL(k_unpermute, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, plan->perm.p,
+                       r_cur, V, r_nxt);
+    std::swap(r_cur, r_nxt);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(CG_ERR_HIP, "PageRank kernels: %s", hipGetErrorString(e));

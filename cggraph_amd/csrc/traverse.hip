@@ -79,16 +79,19 @@ enum { ALG_BFS = CG_BFS, ALG_SSSP = CG_SSSP, ALG_CC = CG_CC };
 // the strict-less pre-check: a pre-check that passes means the value is
 // lowered in this iteration by this or another update (or the read was
 // stale), so the active set is a superset and the fixed point is the same.
+template <int ALG>
+__device__ __forceinline__ uint32_t candidate(uint32_t sv, uint64_t e, const uint32_t* __restrict__ w) {
+  if constexpr (ALG == ALG_BFS) return sv + 1;
+  else if constexpr (ALG == ALG_SSSP) return sv + w[e];
+  else return sv;
+}
+
+// The update of destination d with candidate cand, given a value `seen` read
+// earlier (values only decrease, so a stale read is never below the current
+// value and can only let an update through that the atomic then rejects).
 template <int ALG, bool EXACT>
-__device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_t* __restrict__ dst,
-                                          const uint32_t* __restrict__ w, uint32_t* __restrict__ val,
-                                          uint64_t* __restrict__ out) {
-  const uint32_t d = dst[e];
-  uint32_t cand;
-  if constexpr (ALG == ALG_BFS) cand = sv + 1;
-  else if constexpr (ALG == ALG_SSSP) cand = sv + w[e];
-  else cand = sv;
-  const uint32_t seen = val[d];
+__device__ __forceinline__ uint32_t commit(uint32_t cand, uint32_t seen, uint32_t d, uint32_t* __restrict__ val,
+                                           uint64_t* __restrict__ out) {
   if (cand < seen) {
     if constexpr (EXACT) {
       const uint32_t old = atomicMin(val + d, cand);
@@ -103,6 +106,37 @@ __device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_
     }
   }
   return 0u;
+}
+
+template <int ALG, bool EXACT>
+__device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_t* __restrict__ dst,
+                                          const uint32_t* __restrict__ w, uint32_t* __restrict__ val,
+                                          uint64_t* __restrict__ out) {
+  const uint32_t d = dst[e];
+  return commit<ALG, EXACT>(candidate<ALG>(sv, e, w), val[d], d, val, out);
+}
+
+// Edges [s, e) of one source with value sv, lane-strided over a wavefront.
+// Four edges per lane are loaded (destination, weight, destination value)
+// before any of them is committed, so four gathers are in flight per lane.
+template <int ALG, bool EXACT>
+__device__ __forceinline__ uint32_t relax_strided(uint32_t sv, uint64_t s, uint64_t e, uint32_t lane,
+                                                  const uint32_t* __restrict__ dst, const uint32_t* __restrict__ w,
+                                                  uint32_t* __restrict__ val, uint64_t* __restrict__ out) {
+  uint32_t n = 0;
+  uint64_t k = s + lane;
+  for (; k + 3 * kWave < e; k += 4 * kWave) {
+    const uint32_t d0 = dst[k], d1 = dst[k + kWave], d2 = dst[k + 2 * kWave], d3 = dst[k + 3 * kWave];
+    const uint32_t c0 = candidate<ALG>(sv, k, w), c1 = candidate<ALG>(sv, k + kWave, w);
+    const uint32_t c2 = candidate<ALG>(sv, k + 2 * kWave, w), c3 = candidate<ALG>(sv, k + 3 * kWave, w);
+    const uint32_t s0 = val[d0], s1 = val[d1], s2 = val[d2], s3 = val[d3];
+    n += commit<ALG, EXACT>(c0, s0, d0, val, out);
+    n += commit<ALG, EXACT>(c1, s1, d1, val, out);
+    n += commit<ALG, EXACT>(c2, s2, d2, val, out);
+    n += commit<ALG, EXACT>(c3, s3, d3, val, out);
+  }
+  for (; k < e; k += kWave) n += relax<ALG, EXACT>(sv, k, dst, w, val, out);
+  return n;
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
@@ -395,13 +429,10 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
     const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
     const uint64_t end = g.off[v + 1];
     const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
-    const uint32_t sv = val[v];
-    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG, EXACT>(sv, k, g.dst, g.w, val, out);
+    n_new += relax_strided<ALG, EXACT>(val[v], s, e, lane, g.dst, g.w, val, out);
   } else if (item < (uint64_t)n_chunk + n_med) {
     const uint32_t v = q_med[item - n_chunk];
-    const uint64_t s = g.off[v], e = g.off[v + 1];
-    const uint32_t sv = val[v];
-    for (uint64_t k = s + lane; k < e; k += kWave) n_new += relax<ALG, EXACT>(sv, k, g.dst, g.w, val, out);
+    n_new += relax_strided<ALG, EXACT>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out);
   } else {
     // 64 low-degree vertices per wavefront, their edges spread evenly over
     // the lanes: lane j of round r takes edge r*64+j of the concatenated

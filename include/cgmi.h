@@ -104,12 +104,18 @@ typedef struct cg_options {
   uint32_t sssp_delta;            /* SSSP bucket width (delta-stepping): frontier vertices whose
                                      distance is >= the current threshold wait; 0 = relax the
                                      whole frontier every iteration */
-  uint32_t pr_segment_log2;       /* PageRank source-window size 2^k vertices (cache blocking);
-                                     0 = automatic, 31 = one window (plain pull) */
+  uint32_t pr_segment_log2;       /* PageRank source-window size 2^k vertices (cache blocking,
+                                     pr_layout 3); see pr_layout for 0 and 31 */
   uint32_t atomic_return;         /* 1: activate a destination only when its atomic min returned a
                                      larger value (exact first-visit counts); 0: fire-and-forget
                                      atomics, activate on the strict-less pre-check (a superset,
                                      same fixed point) */
+  uint32_t pr_layout;             /* PageRank in-edge layout: 0 automatic, 1 plain pull over the
+                                     transposed graph, 2 pull over a copy renumbered by descending
+                                     out-degree (gather locality on skewed graphs), 3 source
+                                     windows of 2^pr_segment_log2 vertices.  Automatic: a window
+                                     size in [1, 30] selects 3, 31 selects 1, 0 selects 2 above
+                                     65 536 vertices */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

@@ -207,13 +207,14 @@ def test_pagerank_l1_and_determinism(oracle):
     assert st["iterations"] == 20
 
 
-@pytest.mark.parametrize("seg", [31, 8, 11, 14])
-def test_pagerank_segmented(oracle, seg):
-    """Source-window (cache-blocked) pull against the FP64 oracle; run-to-run
-    bit-identical; windows smaller and larger than the graph."""
+@pytest.mark.parametrize("layout,seg", [(0, 31), (0, 8), (0, 11), (0, 14), (0, 0), (1, 0), (2, 0), (3, 9)])
+def test_pagerank_layouts(oracle, layout, seg):
+    """Every in-edge layout (plain, renumbered by out-degree, source windows
+    smaller and larger than the graph) against the FP64 oracle; run-to-run
+    bit-identical."""
     og = oracle.rmat_graph(17, 16, seed=2, weights=None)
     g = cg.Graph.from_csr(og.offsets, og.dest)
-    e = cg.Engine(g, PR, pr_segment_log2=seg)
+    e = cg.Engine(g, PR, pr_segment_log2=seg, pr_layout=layout)
     r1, _ = e.run()
     r2, _ = e.run()
     e.close()
@@ -225,10 +226,26 @@ def test_pagerank_segmented(oracle, seg):
     n = 5000
     star = oracle.csr_build(n, np.arange(1, n, dtype=np.uint32), np.zeros(n - 1, np.uint32))
     g2 = cg.Graph.from_csr(star.offsets, star.dest)
-    e = cg.Engine(g2, PR, pr_segment_log2=seg)
+    e = cg.Engine(g2, PR, pr_segment_log2=seg, pr_layout=layout)
     r, _ = e.run()
     e.close()
     assert float(np.abs(r.astype(np.float64) - oracle.pagerank64(star, 20, 0.85)).sum()) <= 1e-6
+    # a graph whose first and last vertices have no edges at all
+    src = np.array([3, 3, 5, 7, 8, 8, 8], np.uint32)
+    dst = np.array([5, 8, 3, 3, 3, 5, 7], np.uint32)
+    sparse = oracle.csr_build(12, src, dst)
+    g3 = cg.Graph.from_csr(sparse.offsets, sparse.dest)
+    e = cg.Engine(g3, PR, pr_segment_log2=seg, pr_layout=layout)
+    r, _ = e.run()
+    e.close()
+    assert float(np.abs(r.astype(np.float64) - oracle.pagerank64(sparse, 20, 0.85)).sum()) <= 1e-6
+
+
+def test_pagerank_bad_layout(oracle):
+    og = oracle.rmat_graph(8, 4, seed=1, weights=None)
+    g = cg.Graph.from_csr(og.offsets, og.dest)
+    with pytest.raises(cg.CgError):
+        cg.Engine(g, PR, pr_layout=4)
 
 
 def test_error_paths(oracle):
