@@ -52,6 +52,7 @@ class Options(ctypes.Structure):
         ("pr_segment_log2", ctypes.c_uint32),
         ("atomic_return", ctypes.c_uint32),
         ("pr_layout", ctypes.c_uint32),
+        ("relabel", ctypes.c_uint32),
     ]
 
 

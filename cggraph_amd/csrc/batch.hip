@@ -447,7 +447,7 @@ extern "C" int cg_engine_run_batch(cg_engine* const* engines, const uint32_t* ro
     if (!engines[i]) return cg::set_error(CG_ERR_INVALID, "engine %d is NULL", i);
     if (engines[i]->alg != CG_BFS && engines[i]->alg != CG_SSSP)
       return cg::set_error(CG_ERR_UNSUPPORTED, "engine %d: batched queries are BFS or SSSP", i);
-    graphs[i] = engines[i]->g;
+    graphs[i] = engines[i]->caller_g;
     algs[i] = engines[i]->alg;
     if (outs) o[i] = static_cast<uint32_t*>(outs[i]);
   }

@@ -193,6 +193,7 @@ int cg_options_default(cg_options* o) {
   o->pr_segment_log2 = 0;
   o->atomic_return = 1;
   o->pr_layout = 0;
+  o->relabel = 0;
   return CG_OK;
 }
 
@@ -424,6 +425,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   std::unique_ptr<cg_engine> e(new (std::nothrow) cg_engine());
   if (!e) return set_error(CG_ERR_OOM, "engine object");
   e->g = g;
+  e->caller_g = g;
   e->alg = algorithm;
   e->engine_type = engine_type;
   e->use_device = use_device_num;
@@ -445,16 +447,36 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   CG_HIP(hipEventCreate(&e->ev0));
   CG_HIP(hipEventCreate(&e->ev1));
   CG_HIP(hipEventCreateWithFlags(&e->ev_copy, hipEventDisableTiming));
+  if (e->opt.relabel > 2) return set_error(CG_ERR_INVALID, "relabel %u outside [0, 2]", e->opt.relabel);
+  if ((algorithm == CG_BFS || algorithm == CG_SSSP) && e->opt.relabel != 1 && g->V > 0) {
+    // Renumber by descending out-degree when asked, or (automatic) when the
+    // top 1/64 of vertices send at least half of the edges: the values those
+    // edges touch then share few cache lines.
+    cg::DevBuf<uint32_t> new_of_d;
+    double share = 0.0;
+    CG_TRY(degree_order(g, e->perm, new_of_d, &share));
+    if (e->opt.relabel == 2 || (g->V >= (1u << 16) && share >= 0.5)) {
+      e->own_g.reset(new (std::nothrow) cg_graph());
+      if (!e->own_g) return set_error(CG_ERR_OOM, "renumbered graph object");
+      CG_TRY(relabel_graph(g, new_of_d.p, e->own_g.get()));
+      e->new_of.resize(g->V);
+      CG_HIP(hipMemcpy(e->new_of.data(), new_of_d.p, (size_t)g->V * 4, hipMemcpyDeviceToHost));
+      CG_TRY(e->val_out.ensure(g->V));
+      e->g = e->own_g.get();
+      e->renumbered = true;
+      g = e->g;
+    } else {
+      e->perm.reset();
+    }
+  }
   const uint64_t V = g->V;
   const uint64_t words = (V + 63) / 64 + 1;
   if (algorithm == CG_PR) {
-    CG_TRY(build_transpose(g));
     CG_TRY(e->rank.ensure(V));
     CG_TRY(e->rank2.ensure(V));
     CG_TRY(e->contrib.ensure(V));
     CG_TRY(e->contrib2.ensure(V));
-    DevGraph dg{g->d_off.p, g->d_dst.p, nullptr, g->d_in_off.p, g->d_in_src.p, g->V};
-    CG_TRY(pr_plan_create(dg, g->E, e->opt.pr_layout, e->opt.pr_segment_log2, &e->plan, e->stream));
+    CG_TRY(pr_plan_create(g, e->opt.pr_layout, e->opt.pr_segment_log2, &e->plan, e->stream));
   } else {
     CG_TRY(e->val.ensure(V));
     CG_TRY(e->bm[0].ensure(words));
@@ -530,6 +552,12 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
     rs.edges_traversed = g.E;
     rs.edges_relaxed = g.E;
     rs.expand_alg_bytes = 8.0 * ((double)g.V + 1) + 4.0 * (double)g.E + 8.0 * (double)g.V;
+  } else if (e->renumbered) {
+    const auto t0 = std::chrono::steady_clock::now();
+    st = engine_run(e, e->new_of[root], &rs);
+    if (st == CG_OK) st = launch_unpermute_u32(e->perm.p, e->val.p, g.V, e->val_out.p, e->stream);
+    if (st == CG_OK) CG_HIP(hipStreamSynchronize(e->stream));
+    rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   } else {
     st = engine_run(e, root, &rs);
   }
@@ -544,14 +572,16 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
 int cg_engine_result(cg_engine* e, void* out) {
   if (!e || !out) return set_error(CG_ERR_INVALID, "NULL argument");
   if (!e->result_valid) return set_error(CG_ERR_INVALID, "no completed run to read");
-  const void* src = e->alg == CG_PR ? (const void*)e->rank.p : (const void*)e->val.p;
+  const void* src = e->alg == CG_PR ? (const void*)e->rank.p
+                    : e->renumbered   ? (const void*)e->val_out.p
+                                      : (const void*)e->val.p;
   CG_HIP(hipMemcpy(out, src, (size_t)e->g->V * 4, hipMemcpyDeviceToHost));
   return CG_OK;
 }
 
 int cg_engine_result_device(cg_engine* e, void** dev_ptr) {
   if (!e || !dev_ptr) return set_error(CG_ERR_INVALID, "NULL argument");
-  *dev_ptr = e->alg == CG_PR ? (void*)e->rank.p : (void*)e->val.p;
+  *dev_ptr = e->alg == CG_PR ? (void*)e->rank.p : e->renumbered ? (void*)e->val_out.p : (void*)e->val.p;
   return CG_OK;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -10,7 +11,8 @@
 #include "sched.hpp"
 
 struct cg_engine {
-  cg_graph* g = nullptr;
+  cg_graph* g = nullptr;         // the graph the kernels run on
+  cg_graph* caller_g = nullptr;  // the caller's graph (differs from g when renumbered)
   int alg = CG_BFS;
   int engine_type = CG_COORDINATION;
   int use_device = 1;
@@ -43,6 +45,13 @@ struct cg_engine {
   std::vector<uint64_t> chunk_lo, chunk_hi;
   cg::HostTeam team;
   bool team_ready = false;
+
+  // renumbering by out-degree (BFS / SSSP): g is an engine-owned copy whose
+  // vertex new_of[v] is the caller's v; results are scattered back through perm
+  bool renumbered = false;
+  std::unique_ptr<cg_graph> own_g;
+  cg::DevBuf<uint32_t> perm, val_out;
+  std::vector<uint32_t> new_of;
 
   // PageRank
   cg::DevBuf<float> rank, rank2, contrib, contrib2;

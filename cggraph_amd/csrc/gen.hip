@@ -528,4 +528,159 @@ int device_max_outdeg(cg_graph* g) {
   return CG_OK;
 }
 
+// ---- renumbering by descending out-degree ----
+
+namespace {
+
+// Sort key for descending out-degree; the radix sort is stable, so equal
+// degrees keep ascending ids.
+__global__ void k_degree_order_keys(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ key,
+                                    uint32_t* __restrict__ id) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    key[v] = ~(uint32_t)(off[v + 1] - off[v]);
+    id[v] = (uint32_t)v;
+  }
+}
+
+__global__ void k_new_ids(const uint32_t* __restrict__ perm, uint32_t V, uint32_t* __restrict__ new_of) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    new_of[perm[i]] = (uint32_t)i;
+}
+
+__global__ void k_relabel_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
+                               const uint32_t* __restrict__ new_of, uint32_t bits, uint64_t* __restrict__ keys) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    keys[e] = ((uint64_t)new_of[src[e]] << bits) | new_of[dst[e]];
+}
+
+__global__ void k_unpermute_u32(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in, uint32_t V,
+                                uint32_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[perm[i]] = in[i];
+}
+
+}  // namespace
+
+int edge_sources(const uint64_t* off, uint32_t V, uint64_t E, DevBuf<uint32_t>& src, hipStream_t s) {
+  DevBuf<uint32_t> head;
+  CG_TRY(head.ensure(E));
+  CG_TRY(src.ensure(E));
+  if (!E) return CG_OK;
+  CG_HIP(hipMemsetAsync(head.p, 0, E * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_row_heads, dim3(grid_of(V)), dim3(kBlock), 0, s, off, V, head.p);
+  LAUNCH_OK();
+  size_t sb = 0;
+  CG_HIP(rocprim::inclusive_scan(nullptr, sb, head.p, src.p, E, rocprim::maximum<uint32_t>(), s));
+  DevBuf<uint8_t> st;
+  CG_TRY(st.ensure(sb));
+  CG_HIP(rocprim::inclusive_scan(st.p, sb, head.p, src.p, E, rocprim::maximum<uint32_t>(), s));
+  CG_HIP(hipStreamSynchronize(s));
+  return CG_OK;
+}
+
+int degree_order(cg_graph* g, DevBuf<uint32_t>& perm, DevBuf<uint32_t>& new_of, double* top_share) {
+  CG_TRY(g->ensure_device());
+  hipStream_t s = nullptr;
+  const uint32_t V = g->V;
+  CG_TRY(perm.ensure(V));
+  CG_TRY(new_of.ensure(V));
+  DevBuf<uint32_t> key, id, key_out;
+  CG_TRY(key.ensure(V));
+  CG_TRY(id.ensure(V));
+  CG_TRY(key_out.ensure(V));
+  hipLaunchKernelGGL(k_degree_order_keys, dim3(grid_of(V)), dim3(kBlock), 0, s, g->d_off.p, V, key.p, id.p);
+  LAUNCH_OK();
+  size_t tb = 0;
+  CG_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.p, key_out.p, id.p, perm.p, (size_t)V, 0, 32, s));
+  DevBuf<uint8_t> tmp;
+  CG_TRY(tmp.ensure(tb));
+  CG_HIP(rocprim::radix_sort_pairs(tmp.p, tb, key.p, key_out.p, id.p, perm.p, (size_t)V, 0, 32, s));
+  hipLaunchKernelGGL(k_new_ids, dim3(grid_of(V)), dim3(kBlock), 0, s, perm.p, V, new_of.p);
+  LAUNCH_OK();
+  if (top_share) {
+    // share of the edges leaving the top 1/64 of vertices by out-degree
+    const uint32_t k = V / 64 ? V / 64 : 1;
+    std::vector<uint32_t> hk(k);
+    CG_HIP(hipMemcpyAsync(hk.data(), key_out.p, (size_t)k * 4, hipMemcpyDeviceToHost, s));
+    CG_HIP(hipStreamSynchronize(s));
+    uint64_t sum = 0;
+    for (uint32_t x : hk) sum += ~x;
+    *top_share = g->E ? (double)sum / (double)g->E : 0.0;
+  }
+  CG_HIP(hipStreamSynchronize(s));
+  return CG_OK;
+}
+
+int relabel_graph(cg_graph* g, const uint32_t* new_of, cg_graph* rg) {
+  CG_TRY(g->ensure_device());
+  hipStream_t s = nullptr;
+  const uint32_t V = g->V;
+  const uint64_t E = g->E;
+  const uint32_t bits = bits_for(V);
+  rg->V = V;
+  rg->E = E;
+  rg->has_w = g->has_w;
+  rg->symmetric = g->symmetric;
+  rg->zero_tail = 0;
+  CG_TRY(rg->d_off.ensure((uint64_t)V + 1));
+  CG_TRY(rg->d_dst.ensure(E));
+  if (E == 0) {
+    CG_TRY(launch_fill_u64(rg->d_off.p, 0, (uint64_t)V + 1, s));
+  } else {
+    DevBuf<uint64_t> keys, alt;
+    CG_TRY(keys.ensure(E));
+    CG_TRY(alt.ensure(E));
+    {
+      DevBuf<uint32_t> src;
+      CG_TRY(edge_sources(g->d_off.p, V, E, src, s));
+      hipLaunchKernelGGL(k_relabel_keys, dim3(grid_of(E)), dim3(kBlock), 0, s, src.p, g->d_dst.p, E, new_of, bits,
+                         keys.p);
+      LAUNCH_OK();
+      CG_HIP(hipStreamSynchronize(s));
+    }
+    rocprim::double_buffer<uint64_t> db(keys.p, alt.p);
+    size_t tb = 0;
+    DevBuf<uint8_t> tmp;
+    if (g->has_w) {
+      DevBuf<uint32_t> walt;
+      CG_TRY(rg->d_w.ensure(E));
+      CG_TRY(walt.ensure(E));
+      CG_HIP(hipMemcpyAsync(rg->d_w.p, g->d_w.p, E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+      rocprim::double_buffer<uint32_t> vb(rg->d_w.p, walt.p);
+      CG_HIP(rocprim::radix_sort_pairs(nullptr, tb, db, vb, E, 0, 2 * bits, s));
+      CG_TRY(tmp.ensure(tb));
+      CG_HIP(rocprim::radix_sort_pairs(tmp.p, tb, db, vb, E, 0, 2 * bits, s));
+      if (vb.current() != rg->d_w.p)
+        CG_HIP(hipMemcpyAsync(rg->d_w.p, vb.current(), E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+      CG_HIP(hipStreamSynchronize(s));
+    } else {
+      CG_HIP(rocprim::radix_sort_keys(nullptr, tb, db, E, 0, 2 * bits, s));
+      CG_TRY(tmp.ensure(tb));
+      CG_HIP(rocprim::radix_sort_keys(tmp.p, tb, db, E, 0, 2 * bits, s));
+    }
+    uint64_t* sorted = db.current();
+    hipLaunchKernelGGL(k_offsets, dim3(grid_of((uint64_t)V + 1)), dim3(kBlock), 0, s, sorted, E, V, bits,
+                       rg->d_off.p);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_low_bits, dim3(grid_of(E)), dim3(kBlock), 0, s, sorted, E, bits, rg->d_dst.p);
+    LAUNCH_OK();
+  }
+  CG_HIP(hipStreamSynchronize(s));
+  rg->dev_valid = true;
+  rg->host_valid = false;
+  rg->dev_t_valid = false;
+  return device_max_outdeg(rg);
+}
+
+int launch_unpermute_u32(const uint32_t* perm, const uint32_t* in, uint32_t V, uint32_t* out, hipStream_t s) {
+  if (!V) return CG_OK;
+  hipLaunchKernelGGL(k_unpermute_u32, dim3(grid_of(V)), dim3(kBlock), 0, s, perm, in, V, out);
+  LAUNCH_OK();
+  return CG_OK;
+}
+
 }  // namespace cg

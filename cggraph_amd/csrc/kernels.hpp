@@ -134,8 +134,7 @@ struct PrPlan;  // per-graph layout for the pull kernels (tiers, relabelled CSC 
 // 3 source windows of 2^seg_log2 vertices.  Automatic: seg_log2 in [1, 30]
 // asks for windows, 31 for plain pull, 0 picks relabelled pull above 2^16
 // vertices.
-int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t layout, uint32_t seg_log2, PrPlan** plan,
-                   hipStream_t s);
+int pr_plan_create(cg_graph* g, uint32_t layout, uint32_t seg_log2, PrPlan** plan, hipStream_t s);
 void pr_plan_destroy(PrPlan* plan);
 int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* rank, float* tmp_rank,
            float* contrib, float* contrib2, hipStream_t s, int profile, double* kernel_ms,
@@ -149,5 +148,15 @@ int build_transpose(cg_graph* g);
 int build_from_edges(const uint32_t* src, const uint32_t* dst, const uint32_t* w, uint64_t m, uint32_t V,
                      int symmetrize, cg_graph* g);
 int device_max_outdeg(cg_graph* g);
+// Source vertex of every edge of a CSR (u32[E]).
+int edge_sources(const uint64_t* off, uint32_t V, uint64_t E, DevBuf<uint32_t>& src, hipStream_t s);
+// Renumbering by descending out-degree (ties by ascending id): perm[new] = old,
+// new_of[old] = new; top_share = share of the edges leaving the top 1/64 of
+// vertices (NULL: not computed).
+int degree_order(cg_graph* g, DevBuf<uint32_t>& perm, DevBuf<uint32_t>& new_of, double* top_share);
+// rg = g with every vertex v renamed new_of[v] (device CSR, rows sorted).
+int relabel_graph(cg_graph* g, const uint32_t* new_of, cg_graph* rg);
+// out[perm[i]] = in[i]
+int launch_unpermute_u32(const uint32_t* perm, const uint32_t* in, uint32_t V, uint32_t* out, hipStream_t s);
 
 }  // namespace cg

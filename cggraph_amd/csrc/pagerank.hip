@@ -183,58 +183,6 @@ __global__ void k_pr_init(uint32_t V, const uint32_t* __restrict__ deg, float r0
   }
 }
 
-// ---- relabelled layout construction ----
-
-// Sort key for descending out-degree; radix sort is stable, so ties keep
-// ascending ids.
-__global__ void k_degree_keys(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ key,
-                              uint32_t* __restrict__ id) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
-       v += (uint64_t)gridDim.x * blockDim.x) {
-    key[v] = ~(uint32_t)(off[v + 1] - off[v]);
-    id[v] = (uint32_t)v;
-  }
-}
-
-__global__ void k_invert(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ key, uint32_t V,
-                         uint32_t* __restrict__ new_of, uint32_t* __restrict__ deg_new) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    new_of[perm[i]] = (uint32_t)i;
-    deg_new[i] = ~key[i];
-  }
-}
-
-// in-degree of new row i (the caller's vertex perm[i]); entry V is 0 so the
-// exclusive scan over V + 1 entries ends with E.
-__global__ void k_in_degree_new(const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ perm,
-                                uint32_t V, uint64_t* __restrict__ cnt) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= V;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    if (i < V) {
-      const uint32_t v = perm[i];
-      cnt[i] = in_off[v + 1] - in_off[v];
-    } else {
-      cnt[i] = 0;
-    }
-  }
-}
-
-// key = new(v) << b | new(u) for edge u -> v
-__global__ void k_relabel_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
-                               const uint32_t* __restrict__ new_of, uint32_t b, uint64_t* __restrict__ keys) {
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
-       e += (uint64_t)gridDim.x * blockDim.x)
-    keys[e] = ((uint64_t)new_of[dst[e]] << b) | new_of[src[e]];
-}
-
-__global__ void k_low_bits(const uint64_t* __restrict__ keys, uint64_t E, uint32_t b, uint32_t* __restrict__ out) {
-  const uint64_t mask = (1ull << b) - 1;
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E;
-       e += (uint64_t)gridDim.x * blockDim.x)
-    out[e] = (uint32_t)(keys[e] & mask);
-}
-
 __global__ void k_unpermute(const uint32_t* __restrict__ perm, const float* __restrict__ in, uint32_t V,
                             float* __restrict__ out) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
@@ -243,12 +191,6 @@ __global__ void k_unpermute(const uint32_t* __restrict__ perm, const float* __re
 }
 
 // ---- segmented layout construction ----
-
-__global__ void k_row_heads(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ head) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
-       v += (uint64_t)gridDim.x * blockDim.x)
-    if (off[v + 1] > off[v]) head[off[v]] = (uint32_t)v;
-}
 
 // key = window(u) << 2b | v << b | u for edge u -> v
 __global__ void k_seg_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
@@ -394,89 +336,21 @@ int build_degrees(const DevGraph& g, PrPlan* p, hipStream_t s) {
   return CG_OK;
 }
 
-// Per-edge source ids of the forward CSR: heads of non-empty rows, then a
-// running maximum.
-int edge_sources(const DevGraph& g, uint64_t E, DevBuf<uint32_t>& srcv, hipStream_t s) {
-  DevBuf<uint32_t> head;
-  CG_TRY(head.ensure(E));
-  CG_TRY(srcv.ensure(E));
-  CG_HIP(hipMemsetAsync(head.p, 0, E * sizeof(uint32_t), s));
-  hipLaunchKernelGGL(k_row_heads, dim3(grid_of(g.V)), dim3(kBlock), 0, s, g.off, g.V, head.p);
-  PR_LAUNCH_OK();
-  size_t sb = 0;
-  CG_HIP(rocprim::inclusive_scan(nullptr, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
-  DevBuf<uint8_t> st;
-  CG_TRY(st.ensure(sb));
-  CG_HIP(rocprim::inclusive_scan(st.p, sb, head.p, srcv.p, E, rocprim::maximum<uint32_t>(), s));
+// The CSC of the graph renumbered by descending out-degree.
+int build_relabelled(cg_graph* g, PrPlan* p, hipStream_t s) {
+  const uint32_t V = g->V;
+  cg_graph rg;
+  {
+    DevBuf<uint32_t> new_of;
+    CG_TRY(degree_order(g, p->perm, new_of, nullptr));
+    CG_TRY(relabel_graph(g, new_of.p, &rg));
+  }
+  CG_TRY(build_transpose(&rg));
+  const DevGraph rdg{rg.d_off.p, rg.d_dst.p, nullptr, rg.d_in_off.p, rg.d_in_src.p, V};
+  CG_TRY(build_degrees(rdg, p, s));
   CG_HIP(hipStreamSynchronize(s));
-  return CG_OK;
-}
-
-int build_relabelled(const DevGraph& g, uint64_t E, PrPlan* p, hipStream_t s) {
-  const uint32_t V = g.V;
-  const uint32_t b = bits_for(V);
-  DevBuf<uint32_t> new_of;
-  CG_TRY(p->perm.ensure(V));
-  CG_TRY(p->deg.ensure(V));
-  CG_TRY(new_of.ensure(V));
-  {
-    DevBuf<uint32_t> key, id, key_out;
-    CG_TRY(key.ensure(V));
-    CG_TRY(id.ensure(V));
-    CG_TRY(key_out.ensure(V));
-    hipLaunchKernelGGL(k_degree_keys, dim3(grid_of(V)), dim3(kBlock), 0, s, g.off, V, key.p, id.p);
-    PR_LAUNCH_OK();
-    size_t tb = 0;
-    CG_HIP(rocprim::radix_sort_pairs(nullptr, tb, key.p, key_out.p, id.p, p->perm.p, (size_t)V, 0, 32, s));
-    DevBuf<uint8_t> tmp;
-    CG_TRY(tmp.ensure(tb ? tb : 1));
-    CG_HIP(rocprim::radix_sort_pairs(tmp.p, tb, key.p, key_out.p, id.p, p->perm.p, (size_t)V, 0, 32, s));
-    hipLaunchKernelGGL(k_invert, dim3(grid_of(V)), dim3(kBlock), 0, s, p->perm.p, key_out.p, V, new_of.p,
-                       p->deg.p);
-    PR_LAUNCH_OK();
-    CG_HIP(hipStreamSynchronize(s));
-  }
-  // row offsets of the relabelled CSC
-  CG_TRY(p->r_off.ensure((uint64_t)V + 1));
-  {
-    DevBuf<uint64_t> cnt;
-    CG_TRY(cnt.ensure((uint64_t)V + 1));
-    hipLaunchKernelGGL(k_in_degree_new, dim3(grid_of((uint64_t)V + 1)), dim3(kBlock), 0, s, g.in_off, p->perm.p, V,
-                       cnt.p);
-    PR_LAUNCH_OK();
-    size_t sb = 0;
-    CG_HIP(rocprim::exclusive_scan(nullptr, sb, cnt.p, p->r_off.p, (uint64_t)0, (size_t)V + 1,
-                                   rocprim::plus<uint64_t>(), s));
-    DevBuf<uint8_t> st;
-    CG_TRY(st.ensure(sb ? sb : 1));
-    CG_HIP(rocprim::exclusive_scan(st.p, sb, cnt.p, p->r_off.p, (uint64_t)0, (size_t)V + 1,
-                                   rocprim::plus<uint64_t>(), s));
-    CG_HIP(hipStreamSynchronize(s));
-  }
-  // sources of each relabelled row, ascending: sort (new(v), new(u)) keys
-  CG_TRY(p->r_src.ensure(E ? E : 1));
-  if (E) {
-    DevBuf<uint64_t> keys, alt;
-    CG_TRY(keys.ensure(E));
-    CG_TRY(alt.ensure(E));
-    {
-      DevBuf<uint32_t> srcv;
-      CG_TRY(edge_sources(g, E, srcv, s));
-      hipLaunchKernelGGL(k_relabel_keys, dim3(grid_of(E)), dim3(kBlock), 0, s, srcv.p, g.dst, E, new_of.p, b,
-                         keys.p);
-      PR_LAUNCH_OK();
-      CG_HIP(hipStreamSynchronize(s));
-    }
-    rocprim::double_buffer<uint64_t> db(keys.p, alt.p);
-    size_t tb = 0;
-    CG_HIP(rocprim::radix_sort_keys(nullptr, tb, db, (size_t)E, 0, 2 * b, s));
-    DevBuf<uint8_t> tmp;
-    CG_TRY(tmp.ensure(tb ? tb : 1));
-    CG_HIP(rocprim::radix_sort_keys(tmp.p, tb, db, (size_t)E, 0, 2 * b, s));
-    hipLaunchKernelGGL(k_low_bits, dim3(grid_of(E)), dim3(kBlock), 0, s, db.current(), E, b, p->r_src.p);
-    PR_LAUNCH_OK();
-    CG_HIP(hipStreamSynchronize(s));
-  }
+  p->r_off = std::move(rg.d_in_off);
+  p->r_src = std::move(rg.d_in_src);
   CG_TRY(build_tiers(p->r_off.p, V, p, s));
   p->layout = kLayoutRelabel;
   return CG_OK;
@@ -494,7 +368,7 @@ int build_segmented(const DevGraph& g, uint64_t E, uint32_t shift, PrPlan* p, hi
   CG_TRY(alt.ensure(E));
   {
     DevBuf<uint32_t> srcv;
-    CG_TRY(edge_sources(g, E, srcv, s));
+    CG_TRY(edge_sources(g.off, V, E, srcv, s));
     hipLaunchKernelGGL(k_seg_keys, dim3(grid_of(E)), dim3(kBlock), 0, s, srcv.p, g.dst, E, b, shift, keys.p);
     PR_LAUNCH_OK();
     CG_HIP(hipStreamSynchronize(s));
@@ -572,8 +446,10 @@ int build_segmented(const DevGraph& g, uint64_t E, uint32_t shift, PrPlan* p, hi
 
 }  // namespace
 
-int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t layout, uint32_t seg_log2, PrPlan** plan,
-                   hipStream_t s) {
+int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, PrPlan** plan, hipStream_t s) {
+  CG_TRY(build_transpose(graph));
+  const DevGraph g{graph->d_off.p, graph->d_dst.p, nullptr, graph->d_in_off.p, graph->d_in_src.p, graph->V};
+  const uint64_t E = graph->E;
   if (layout > kLayoutSegmented) return set_error(CG_ERR_INVALID, "pr_layout %u outside [0, 3]", layout);
   const uint32_t b = bits_for(g.V);
   if (layout == kLayoutAuto) {
@@ -596,7 +472,7 @@ int pr_plan_create(const DevGraph& g, uint64_t E, uint32_t layout, uint32_t seg_
   if (layout == kLayoutSegmented) {
     st = build_segmented(g, E, shift, p, s);
   } else if (layout == kLayoutRelabel) {
-    st = build_relabelled(g, E, p, s);
+    st = build_relabelled(graph, p, s);
   } else {
     st = build_degrees(g, p, s);
     if (st == CG_OK) st = build_tiers(g.in_off, g.V, p, s);

@@ -116,6 +116,11 @@ typedef struct cg_options {
                                      windows of 2^pr_segment_log2 vertices.  Automatic: a window
                                      size in [1, 30] selects 3, 31 selects 1, 0 selects 2 above
                                      65 536 vertices */
+  uint32_t relabel;               /* BFS / SSSP: run on an engine-owned copy renumbered by
+                                     descending out-degree (results come back in the caller's
+                                     ids): 0 automatic (at least 65 536 vertices and the top 1/64
+                                     of vertices send at least half of the edges), 1 never,
+                                     2 always */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

@@ -78,7 +78,7 @@ def test_engine_handle_batch(oracle):
     engine's algorithm and graph; a PR engine is refused."""
     ogs = [oracle.rmat_graph(10, 8, seed=7, weights=(1, 100)), oracle.grid_graph(24, 0.1, seed=3)]
     gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
-    engs = [cg.Engine(gs[0], 0), cg.Engine(gs[0], 1), cg.Engine(gs[1], 1)]
+    engs = [cg.Engine(gs[0], 0), cg.Engine(gs[0], 1, relabel=2), cg.Engine(gs[1], 1)]
     owner = [0, 1, 2, 1, 0, 2, 2]
     rng = np.random.default_rng(11)
     roots = [int(rng.integers(0, ogs[0 if k < 2 else 1].V)) for k in owner]

@@ -300,3 +300,49 @@ def test_edge_list_ingest(oracle, tmp_path):
         assert np.array_equal(s, oracle.sssp(ref, root))
     g = cg.Graph.from_edges(V, src, dst)  # no weights
     assert np.array_equal(g.export()[1], oracle.csr_build(V, src, dst).dest)
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2])
+@pytest.mark.parametrize("relabel", [1, 2])
+def test_renumbered_engine(oracle, sched, relabel):
+    """BFS / SSSP on the engine-owned copy renumbered by out-degree: values
+    come back in the caller's ids, bit-exact, under the GPU, heuristic
+    (host team on the renumbered copy) and learned schedules."""
+    og = oracle.rmat_graph(14, 16, seed=9, weights=(1, 100))
+    g = cg.Graph.from_csr(og.offsets, og.dest, og.weight)
+    rng = np.random.default_rng(relabel * 10 + sched)
+    roots = [oracle.max_outdeg_vertex(og)] + [int(x) for x in rng.integers(0, og.V, 3)]
+    for alg in (BFS, SSSP):
+        e = cg.Engine(g, alg, sched_mode=sched, relabel=relabel)
+        for r in roots:
+            got, st = e.run(r)
+            ref = oracle.bfs(og, r) if alg == BFS else oracle.sssp(og, r)
+            assert np.array_equal(got, ref), (alg, r)
+            assert np.array_equal(e.result(), ref)
+            assert st["edges_traversed"] == oracle.reached_edges(og, ref)
+        e.close()
+
+
+def test_renumbered_lattice_and_tiny(oracle):
+    """Forced renumbering on a lattice (no skew) and on a graph with isolated
+    vertices; automatic mode on the same inputs."""
+    og = oracle.grid_graph(40, 0.1, seed=3)
+    g = cg.Graph.from_csr(og.offsets, og.dest, og.weight)
+    for relabel in (0, 2):
+        for alg in (BFS, SSSP):
+            got, _ = run(g, alg, 37, relabel=relabel)
+            assert np.array_equal(got, oracle.bfs(og, 37) if alg == BFS else oracle.sssp(og, 37))
+    src = np.array([3, 3, 5, 7, 8, 8, 8], np.uint32)
+    dst = np.array([5, 8, 3, 3, 3, 5, 7], np.uint32)
+    w = np.array([4, 1, 2, 2, 9, 1, 1], np.uint32)
+    sp = oracle.csr_build(12, src, dst)
+    # csr_build sorts rows; weights follow the edge's (src, dst) order
+    order = np.lexsort((dst, src))
+    g2 = cg.Graph.from_csr(sp.offsets, sp.dest, w[order])
+    sp_w = oracle.CSR(sp.offsets, sp.dest, w[order])
+    for root in (0, 3, 8, 11):
+        for alg in (BFS, SSSP):
+            got, _ = run(g2, alg, root, relabel=2)
+            assert np.array_equal(got, oracle.bfs(sp_w, root) if alg == BFS else oracle.sssp(sp_w, root))
+    with pytest.raises(cg.CgError):
+        cg.Engine(g2, BFS, relabel=3)
