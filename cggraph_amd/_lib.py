@@ -53,6 +53,7 @@ class Options(ctypes.Structure):
         ("atomic_return", ctypes.c_uint32),
         ("pr_layout", ctypes.c_uint32),
         ("relabel", ctypes.c_uint32),
+        ("pr_kernel", ctypes.c_uint32),
     ]
 
 

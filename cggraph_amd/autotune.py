@@ -1,5 +1,5 @@
-"""Autotuner for the traversal engine's degree tiers / delta-stepping width and
-the batched kernel's launch shape.
+"""Autotuner for the traversal engine's degree tiers, delta-stepping width and
+renumbering, and the batched kernel's launch shape.
 
 It times real runs through the C ABI (the outputs do not depend on any of the
 tuned values, so any choice is correct) and keeps the fastest configuration
@@ -26,6 +26,7 @@ ENGINE_SPACE = {
     "bin_chunk": [512, 1024, 4096, 16384],
     "chunk_edges": [512, 1024, 2048, 4096, 8192],
     "sssp_delta": [0, 64, 256, 1024, 4096],
+    "relabel": [0, 1],
 }
 BATCH_SPACE = {
     "sssp_delta": [1000, 3000, 10000, 30000, 100000],

@@ -194,6 +194,7 @@ int cg_options_default(cg_options* o) {
   o->atomic_return = 1;
   o->pr_layout = 0;
   o->relabel = 0;
+  o->pr_kernel = 0;
   return CG_OK;
 }
 
@@ -476,7 +477,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->rank2.ensure(V));
     CG_TRY(e->contrib.ensure(V));
     CG_TRY(e->contrib2.ensure(V));
-    CG_TRY(pr_plan_create(g, e->opt.pr_layout, e->opt.pr_segment_log2, &e->plan, e->stream));
+    CG_TRY(pr_plan_create(g, e->opt.pr_layout, e->opt.pr_segment_log2, e->opt.pr_kernel, &e->plan, e->stream));
   } else {
     CG_TRY(e->val.ensure(V));
     CG_TRY(e->bm[0].ensure(words));

@@ -134,7 +134,10 @@ struct PrPlan;  // per-graph layout for the pull kernels (tiers, relabelled CSC 
 // 3 source windows of 2^seg_log2 vertices.  Automatic: seg_log2 in [1, 30]
 // asks for windows, 31 for plain pull, 0 picks relabelled pull above 2^16
 // vertices.
-int pr_plan_create(cg_graph* g, uint32_t layout, uint32_t seg_log2, PrPlan** plan, hipStream_t s);
+// kernel (plain / relabelled layouts): 0 automatic, 1 row tiers, 2 edge tiles, 3 two source
+// bands (relabelled layout; boundary 2^seg_log2, default 2^19).
+int pr_plan_create(cg_graph* g, uint32_t layout, uint32_t seg_log2, uint32_t kernel, PrPlan** plan,
+                   hipStream_t s);
 void pr_plan_destroy(PrPlan* plan);
 int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* rank, float* tmp_rank,
            float* contrib, float* contrib2, hipStream_t s, int profile, double* kernel_ms,

@@ -41,10 +41,24 @@ enum { kLayoutAuto = 0, kLayoutPlain = 1, kLayoutRelabel = 2, kLayoutSegmented =
 
 struct PrPlan {
   int layout = kLayoutPlain;
-  // plain and relabelled pull: tiers over the rows of the CSC in use
-  uint32_t n_med = 0, n_high = 0;
-  DevBuf<uint32_t> med, high;
+  // plain and relabelled pull: tiers over the rows of the CSC in use (one
+  // set per pass; the two-band sweep has two passes)
+  struct Tiers {
+    uint32_t n_med = 0, n_high = 0;
+    DevBuf<uint32_t> med, high;
+  } tiers[2];
+  bool banded = false;
+  uint32_t band = 0;
+  DevBuf<uint32_t> hot;
+  DevBuf<float> part;
   DevBuf<uint32_t> deg;  // out-degree of each vertex (in the numbering in use)
+  // edge tiles over the CSC in use (instead of the row tiers)
+  bool tiled = false;
+  uint64_t edges = 0;
+  uint64_t ntiles = 0;
+  uint32_t lead_rows = 0;  // rows before the first edge (no in-edges)
+  DevBuf<uint32_t> tile_row;
+  DevBuf<float> carry, head_part;
   // relabelled pull
   DevBuf<uint32_t> perm;  // new id -> caller's id
   DevBuf<uint64_t> r_off;
@@ -69,6 +83,7 @@ struct Csc {
   const uint32_t* src;
   const uint32_t* deg;  // out-degree, for contrib = rank / deg
   uint32_t V;
+  const uint32_t* hot;  // two-band sweep: sources of row v below the band boundary (a prefix)
 };
 
 unsigned grid_of(uint64_t n) {
@@ -85,15 +100,6 @@ uint32_t bits_for(uint64_t V) {
 
 __device__ __forceinline__ uint32_t ld_stream(const uint32_t* p) { return __builtin_nontemporal_load(p); }
 
-__global__ void k_classify(const uint64_t* __restrict__ in_off, uint32_t V, uint32_t* __restrict__ med,
-                           uint32_t* __restrict__ high, uint32_t* __restrict__ cnt) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
-       v += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t d = in_off[v + 1] - in_off[v];
-    if (d >= kHigh) high[atomicAdd(cnt + 1, 1u)] = (uint32_t)v;
-    else if (d >= kMed) med[atomicAdd(cnt, 1u)] = (uint32_t)v;
-  }
-}
 
 __global__ void k_out_degree(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ deg) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
@@ -110,17 +116,42 @@ __device__ __forceinline__ void finish(uint32_t v, float sum, float base, float 
   contrib[v] = d ? r / (float)d : 0.0f;
 }
 
-// Rows with fewer than kMed in-edges: one lane per row, sources in order.
+// Row v's in-edges swept by pass P: 0 the whole row; in the two-band sweep,
+// 1 the sources below the band boundary (a prefix, rows being sorted) and 2
+// the rest.
+template <int P>
+__device__ __forceinline__ void row_range(const Csc& g, uint32_t v, uint64_t& s, uint64_t& e) {
+  s = g.off[v];
+  e = g.off[v + 1];
+  if constexpr (P == 1) e = s + g.hot[v];
+  else if constexpr (P == 2) s += g.hot[v];
+}
+
+// Pass 1 keeps the partial sum; pass 2 adds the rest to it, so every row is
+// summed prefix first, in a fixed order.
+template <int P>
+__device__ __forceinline__ void emit(const Csc& g, uint32_t v, float sum, float base, float damping,
+                                     float* __restrict__ part, float* __restrict__ rank,
+                                     float* __restrict__ contrib) {
+  if constexpr (P == 1) part[v] = sum;
+  else if constexpr (P == 2) finish(v, part[v] + sum, base, damping, g.deg, rank, contrib);
+  else finish(v, sum, base, damping, g.deg, rank, contrib);
+}
+
+// Rows with fewer than kMed in-edges in this pass: one lane per row, sources
+// in order.
+template <int P>
 __global__ __launch_bounds__(kBlock) void k_pr_low(Csc g, const float* __restrict__ contrib_in, float base,
-                                                   float damping, float* __restrict__ rank,
+                                                   float damping, float* __restrict__ part, float* __restrict__ rank,
                                                    float* __restrict__ contrib_out) {
   const uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   if (v >= g.V) return;
-  const uint64_t s = g.off[v], e = g.off[v + 1];
+  uint64_t s, e;
+  row_range<P>(g, (uint32_t)v, s, e);
   if (e - s >= kMed) return;
   float sum = 0.0f;
   for (uint64_t k = s; k < e; ++k) sum += contrib_in[ld_stream(g.src + k)];
-  finish((uint32_t)v, sum, base, damping, g.deg, rank, contrib_out);
+  emit<P>(g, (uint32_t)v, sum, base, damping, part, rank, contrib_out);
 }
 
 // Lane-strided partial sum over [s, e): the lane adds sources s+lane,
@@ -144,33 +175,82 @@ __device__ __forceinline__ float strided_sum(const uint32_t* __restrict__ src, c
   return sum;
 }
 
+template <int P>
 __global__ __launch_bounds__(kBlock) void k_pr_med(Csc g, const uint32_t* __restrict__ list, uint32_t n,
                                                    const float* __restrict__ contrib_in, float base,
-                                                   float damping, float* __restrict__ rank,
+                                                   float damping, float* __restrict__ part, float* __restrict__ rank,
                                                    float* __restrict__ contrib_out) {
   const uint64_t i = blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64;
   if (i >= n) return;  // wave-uniform
   const uint32_t lane = __lane_id();
   const uint32_t v = list[i];
-  float sum = strided_sum<64>(g.src, contrib_in, g.off[v], g.off[v + 1], lane);
+  uint64_t s, e;
+  row_range<P>(g, v, s, e);
+  float sum = strided_sum<64>(g.src, contrib_in, s, e, lane);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
-  if (lane == 0) finish(v, sum, base, damping, g.deg, rank, contrib_out);
+  if (lane == 0) emit<P>(g, v, sum, base, damping, part, rank, contrib_out);
 }
 
+template <int P>
 __global__ __launch_bounds__(kBlock) void k_pr_high(Csc g, const uint32_t* __restrict__ list,
                                                     const float* __restrict__ contrib_in, float base,
-                                                    float damping, float* __restrict__ rank,
+                                                    float damping, float* __restrict__ part, float* __restrict__ rank,
                                                     float* __restrict__ contrib_out) {
-  __shared__ float part[kBlock];
+  __shared__ float acc[kBlock];
   const uint32_t v = list[blockIdx.x];
-  part[threadIdx.x] = strided_sum<kBlock>(g.src, contrib_in, g.off[v], g.off[v + 1], threadIdx.x);
+  uint64_t s, e;
+  row_range<P>(g, v, s, e);
+  acc[threadIdx.x] = strided_sum<kBlock>(g.src, contrib_in, s, e, threadIdx.x);
   __syncthreads();
   for (int w = kBlock / 2; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    if ((int)threadIdx.x < w) acc[threadIdx.x] += acc[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) finish(v, part[0], base, damping, g.deg, rank, contrib_out);
+  if (threadIdx.x == 0) emit<P>(g, v, acc[0], base, damping, part, rank, contrib_out);
+}
+
+// hot[v] = sources of row v below 2^band (rows sorted ascending)
+__global__ void k_band_split(const uint64_t* __restrict__ off, const uint32_t* __restrict__ src, uint32_t V,
+                             uint32_t band, uint32_t* __restrict__ hot) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t lo = off[v], hi = off[v + 1];
+    const uint64_t s = lo;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if ((src[mid] >> band) == 0) lo = mid + 1;
+      else hi = mid;
+    }
+    hot[v] = (uint32_t)(lo - s);
+  }
+}
+
+// Row lists of the medium and high tiers of pass P; one atomic per list per
+// wavefront.
+template <int P>
+__global__ void k_classify(Csc g, uint32_t* __restrict__ med, uint32_t* __restrict__ high,
+                           uint32_t* __restrict__ cnt) {
+  const uint32_t lane = __lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~63u); b < g.V; b += stride) {
+    const uint64_t v = b + lane;
+    uint64_t s = 0, e = 0;
+    if (v < g.V) row_range<P>(g, (uint32_t)v, s, e);
+    const uint64_t d = e - s;
+    const bool hi = d >= kHigh, md = d >= kMed && d < kHigh;
+    const uint64_t bh = __ballot(hi), bm = __ballot(md);
+    uint32_t base_h = 0, base_m = 0;
+    if (lane == 0) {
+      if (bh) base_h = atomicAdd(cnt + 1, (uint32_t)__popcll(bh));
+      if (bm) base_m = atomicAdd(cnt, (uint32_t)__popcll(bm));
+    }
+    base_h = __shfl(base_h, 0, 64);
+    base_m = __shfl(base_m, 0, 64);
+    const uint64_t below = (1ull << lane) - 1;
+    if (hi) high[base_h + __popcll(bh & below)] = (uint32_t)v;
+    if (md) med[base_m + __popcll(bm & below)] = (uint32_t)v;
+  }
 }
 
 __global__ void k_pr_init(uint32_t V, const uint32_t* __restrict__ deg, float r0, float* __restrict__ rank,
@@ -181,6 +261,175 @@ __global__ void k_pr_init(uint32_t V, const uint32_t* __restrict__ deg, float r0
     const uint32_t d = deg[v];
     contrib[v] = d ? r0 / (float)d : 0.0f;
   }
+}
+
+// ---- edge tiles (merge-based pull) ----
+//
+// A workgroup takes kTileEdges consecutive in-edges of the CSC, whatever rows
+// they belong to; thread j takes kTileK consecutive edges, loads their sources
+// with two 16-byte loads and has all kTileK gathers in flight at once, then
+// walks its edges adding into the current row.  A row that ends inside the
+// thread's edges is finished there; the part of a row that continues across
+// threads is combined by a segmented scan over the workgroup (fixed shape),
+// and a row that continues across workgroups is finished by k_pr_fixup, which
+// adds the workgroups' carries in workgroup order.  Every summation order is a
+// function of the CSC alone.  (Merge-based sparse matrix-vector product,
+// Merrill and Garland, SC'16.)
+
+constexpr int kTileThreads = 256;
+constexpr int kTileK = 8;
+constexpr uint64_t kTileEdges = (uint64_t)kTileThreads * kTileK;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// tile_row[t] = row holding edge t * kTileEdges (largest r with off[r] <= it),
+// tile_row[ntiles] = V.
+__global__ void k_tile_rows(const uint64_t* __restrict__ off, uint32_t V, uint64_t ntiles,
+                            uint32_t* __restrict__ tile_row) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t <= ntiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    if (t == ntiles) {
+      tile_row[t] = V;
+      continue;
+    }
+    const uint64_t e = t * kTileEdges;
+    uint32_t lo = 0, hi = V;  // off[V] = E > e
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (off[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    tile_row[t] = lo;
+  }
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_pr_tile(Csc g, uint64_t E, const uint32_t* __restrict__ tile_row,
+                                                          const float* __restrict__ contrib_in, float base,
+                                                          float damping, float* __restrict__ rank,
+                                                          float* __restrict__ contrib_out, float* __restrict__ carry,
+                                                          float* __restrict__ head_part) {
+  __shared__ float s_val[kTileThreads];
+  __shared__ uint32_t s_flag[kTileThreads / 64];
+  __shared__ float s_wval[kTileThreads / 64];
+  const uint64_t t = blockIdx.x;
+  const uint64_t tile0 = t * kTileEdges;
+  const uint64_t e0 = tile0 + (uint64_t)threadIdx.x * kTileK;
+  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+  const bool has = e0 < E;
+  const uint32_t nk = has ? (uint32_t)((E - e0) < (uint64_t)kTileK ? (E - e0) : kTileK) : 0u;
+
+  // row holding edge e0
+  uint32_t head = g.V;
+  if (has) {
+    uint32_t lo = tile_row[t], hi = tile_row[t + 1];
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (g.off[mid] <= e0) lo = mid;
+      else hi = mid - 1;
+    }
+    head = lo;
+  }
+  // sources and gathers, all in flight before the first add
+  float c[kTileK];
+  if (nk == (uint32_t)kTileK) {
+    const u32x4* p = reinterpret_cast<const u32x4*>(g.src + e0);
+    const u32x4 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
+    c[0] = contrib_in[a.x];
+    c[1] = contrib_in[a.y];
+    c[2] = contrib_in[a.z];
+    c[3] = contrib_in[a.w];
+    c[4] = contrib_in[b.x];
+    c[5] = contrib_in[b.y];
+    c[6] = contrib_in[b.z];
+    c[7] = contrib_in[b.w];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kTileK; ++i) c[i] = (uint32_t)i < nk ? contrib_in[g.src[e0 + i]] : 0.0f;
+  }
+  const bool inherited = has && g.off[head] < e0;  // the head row began before this thread
+  uint32_t r = head;
+  uint64_t next_end = has ? g.off[r + 1] : ~0ull;
+  float acc = 0.0f, head_acc = 0.0f;
+  bool first = true, head_done = false;
+#pragma unroll
+  for (int i = 0; i < kTileK; ++i) {
+    if ((uint32_t)i < nk) {
+      acc += c[i];
+      const uint64_t e = e0 + i + 1;
+      while (next_end == e) {  // row r ends after this edge (and empty rows after it)
+        if (first && inherited) {
+          head_acc = acc;
+          head_done = true;
+        } else {
+          finish(r, acc, base, damping, g.deg, rank, contrib_out);
+        }
+        first = false;
+        acc = 0.0f;
+        ++r;
+        next_end = r < g.V ? g.off[r + 1] : ~0ull;
+      }
+    }
+  }
+  // segmented inclusive scan of the tail parts: a thread whose tail row began
+  // inside its own edges starts a new segment
+  bool f = !(has && r == head && inherited);
+  float v = acc;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float vu = __shfl_up(v, d, 64);
+    const int fu = __shfl_up((int)f, d, 64);
+    if (lane >= (uint32_t)d) {
+      if (!f) v = vu + v;
+      f = f || fu;
+    }
+  }
+  if (lane == 63) {
+    s_flag[wave] = f;
+    s_wval[wave] = v;
+  }
+  __syncthreads();
+  if (!f) {
+    // no segment start between the workgroup's first thread and this one in
+    // this wavefront: prepend the earlier wavefronts' open segment
+    float pre = 0.0f;
+    for (uint32_t w = 0; w < wave; ++w) pre = s_flag[w] ? s_wval[w] : pre + s_wval[w];
+    v = pre + v;
+  }
+  s_val[threadIdx.x] = v;
+  __syncthreads();
+  const float carry_in = threadIdx.x ? s_val[threadIdx.x - 1] : 0.0f;
+  if (head_done) {
+    const float total = carry_in + head_acc;
+    if (g.off[head] < tile0) head_part[t] = total;  // began in an earlier workgroup
+    else finish(head, total, base, damping, g.deg, rank, contrib_out);
+  }
+  if (threadIdx.x == kTileThreads - 1) carry[t] = v;
+}
+
+// Rows that began in an earlier workgroup, finished in the workgroup where
+// they end: carries of the workgroups they span, in order, then that
+// workgroup's part.
+__global__ void k_pr_fixup(Csc g, uint64_t E, uint64_t ntiles, const uint32_t* __restrict__ tile_row,
+                           const float* __restrict__ carry, const float* __restrict__ head_part, float base,
+                           float damping, float* __restrict__ rank, float* __restrict__ contrib_out) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = tile_row[t];
+    const uint64_t tile0 = t * kTileEdges;
+    const uint64_t tile1 = tile0 + kTileEdges < E ? tile0 + kTileEdges : E;
+    const uint64_t s = g.off[r], e = g.off[r + 1];
+    if (s >= tile0 || e > tile1) continue;  // not inherited, or does not end here
+    float sum = 0.0f;
+    for (uint64_t u = s / kTileEdges; u < t; ++u) sum += carry[u];
+    finish(r, sum + head_part[t], base, damping, g.deg, rank, contrib_out);
+  }
+}
+
+// rows [0, n) have no in-edges and precede the first edge
+__global__ void k_pr_lead(uint32_t n, const uint32_t* __restrict__ deg, float base, float* __restrict__ rank,
+                          float* __restrict__ contrib_out) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    finish((uint32_t)v, 0.0f, base, 0.0f, deg, rank, contrib_out);
 }
 
 __global__ void k_unpermute(const uint32_t* __restrict__ perm, const float* __restrict__ in, uint32_t V,
@@ -308,22 +557,43 @@ __global__ void k_seg_finish(uint32_t V, const uint32_t* __restrict__ deg, float
                        __FILE__, __LINE__);                                             \
   } while (0)
 
-// med / high row lists of a CSC (any order: each row's sum is fixed by the row).
-int build_tiers(const uint64_t* in_off, uint32_t V, PrPlan* p, hipStream_t s) {
+// med / high row lists of pass P over a CSC (any order: each row's sum is
+// fixed by the row).
+template <int P>
+int build_tiers(const Csc& g, PrPlan::Tiers& t, hipStream_t s) {
   DevBuf<uint32_t> cnt;
   CG_TRY(cnt.ensure(2));
-  CG_TRY(p->med.ensure(V ? V : 1));
-  CG_TRY(p->high.ensure(V ? V : 1));
+  CG_TRY(t.med.ensure(g.V ? g.V : 1));
+  CG_TRY(t.high.ensure(g.V ? g.V : 1));
   CG_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(uint32_t), s));
-  if (V) {
-    hipLaunchKernelGGL(k_classify, dim3(grid_of(V)), dim3(kBlock), 0, s, in_off, V, p->med.p, p->high.p, cnt.p);
+  if (g.V) {
+    hipLaunchKernelGGL(k_classify<P>, dim3(grid_of(g.V)), dim3(kBlock), 0, s, g, t.med.p, t.high.p, cnt.p);
     PR_LAUNCH_OK();
   }
   uint32_t h[2] = {0, 0};
   CG_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
   CG_HIP(hipStreamSynchronize(s));
-  p->n_med = h[0];
-  p->n_high = h[1];
+  t.n_med = h[0];
+  t.n_high = h[1];
+  return CG_OK;
+}
+
+// Two-band sweep over the relabelled CSC: pass 1 reads only the sources
+// below 2^band, whose contributions (4 * 2^band bytes) stay in L2 for the
+// whole pass; pass 2 reads the rest.
+int build_bands(uint32_t V, uint32_t band, PrPlan* p, hipStream_t s) {
+  CG_TRY(p->hot.ensure(V ? V : 1));
+  CG_TRY(p->part.ensure(V ? V : 1));
+  if (V) {
+    hipLaunchKernelGGL(k_band_split, dim3(grid_of(V)), dim3(kBlock), 0, s, p->r_off.p, p->r_src.p, V, band,
+                       p->hot.p);
+    PR_LAUNCH_OK();
+  }
+  const Csc g{p->r_off.p, p->r_src.p, p->deg.p, V, p->hot.p};
+  CG_TRY(build_tiers<1>(g, p->tiers[0], s));
+  CG_TRY(build_tiers<2>(g, p->tiers[1], s));
+  p->banded = true;
+  p->band = band;
   return CG_OK;
 }
 
@@ -333,6 +603,23 @@ int build_degrees(const DevGraph& g, PrPlan* p, hipStream_t s) {
     hipLaunchKernelGGL(k_out_degree, dim3(grid_of(g.V)), dim3(kBlock), 0, s, g.off, g.V, p->deg.p);
     PR_LAUNCH_OK();
   }
+  return CG_OK;
+}
+
+int build_tiles(const uint64_t* off, uint32_t V, uint64_t E, PrPlan* p, hipStream_t s) {
+  p->edges = E;
+  p->ntiles = (E + kTileEdges - 1) / kTileEdges;
+  CG_TRY(p->tile_row.ensure(p->ntiles + 1));
+  CG_TRY(p->carry.ensure(p->ntiles ? p->ntiles : 1));
+  CG_TRY(p->head_part.ensure(p->ntiles ? p->ntiles : 1));
+  hipLaunchKernelGGL(k_tile_rows, dim3(grid_of(p->ntiles + 1)), dim3(kBlock), 0, s, off, V, p->ntiles,
+                     p->tile_row.p);
+  PR_LAUNCH_OK();
+  uint32_t lead = 0;
+  CG_HIP(hipMemcpyAsync(&lead, p->tile_row.p, 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  p->lead_rows = lead;
+  p->tiled = true;
   return CG_OK;
 }
 
@@ -351,7 +638,6 @@ int build_relabelled(cg_graph* g, PrPlan* p, hipStream_t s) {
   CG_HIP(hipStreamSynchronize(s));
   p->r_off = std::move(rg.d_in_off);
   p->r_src = std::move(rg.d_in_src);
-  CG_TRY(build_tiers(p->r_off.p, V, p, s));
   p->layout = kLayoutRelabel;
   return CG_OK;
 }
@@ -444,9 +730,24 @@ int build_segmented(const DevGraph& g, uint64_t E, uint32_t shift, PrPlan* p, hi
   return CG_OK;
 }
 
+template <int P>
+void launch_tiers(const Csc& g, const PrPlan::Tiers& t, unsigned vblocks, const float* contrib_in, float base,
+                  float damping, float* part, float* rank, float* contrib_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pr_low<P>, dim3(vblocks), dim3(kBlock), 0, s, g, contrib_in, base, damping, part, rank,
+                     contrib_out);
+  if (t.n_med)
+    hipLaunchKernelGGL(k_pr_med<P>, dim3((t.n_med + 3) / 4), dim3(kBlock), 0, s, g, t.med.p, t.n_med, contrib_in,
+                       base, damping, part, rank, contrib_out);
+  if (t.n_high)
+    hipLaunchKernelGGL(k_pr_high<P>, dim3(t.n_high), dim3(kBlock), 0, s, g, t.high.p, contrib_in, base, damping,
+                       part, rank, contrib_out);
+}
+
 }  // namespace
 
-int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, PrPlan** plan, hipStream_t s) {
+int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, uint32_t kernel, PrPlan** plan,
+                   hipStream_t s) {
+  if (kernel > 3) return set_error(CG_ERR_INVALID, "pr_kernel %u outside [0, 3]", kernel);
   CG_TRY(build_transpose(graph));
   const DevGraph g{graph->d_off.p, graph->d_dst.p, nullptr, graph->d_in_off.p, graph->d_in_src.p, graph->V};
   const uint64_t E = graph->E;
@@ -475,8 +776,20 @@ int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, PrPlan**
     st = build_relabelled(graph, p, s);
   } else {
     st = build_degrees(g, p, s);
-    if (st == CG_OK) st = build_tiers(g.in_off, g.V, p, s);
+    if (st == CG_OK) st = build_tiers<0>(Csc{g.in_off, g.in_src, p->deg.p, g.V, nullptr}, p->tiers[0], s);
     p->layout = kLayoutPlain;
+  }
+  if (st == CG_OK && p->layout == kLayoutRelabel) {
+    // automatic: two source bands of 2^19 vertices (2 MB of contributions)
+    if (kernel == 0 || kernel == 3) {
+      st = build_bands(g.V, seg_log2 >= 1 && seg_log2 <= 30 ? seg_log2 : 19, p, s);
+    } else if (kernel == 1) {
+      st = build_tiers<0>(Csc{p->r_off.p, p->r_src.p, p->deg.p, g.V, nullptr}, p->tiers[0], s);
+    } else {
+      st = build_tiles(p->r_off.p, g.V, E, p, s);
+    }
+  } else if (st == CG_OK && p->layout == kLayoutPlain && kernel == 2) {
+    st = build_tiles(g.in_off, g.V, E, p, s);
   }
   if (st != CG_OK) {
     delete p;
@@ -496,8 +809,9 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
   const float base = (1.0f - damping) / (float)V;
   const unsigned vblocks = (unsigned)(((uint64_t)V + kBlock - 1) / kBlock);
   const bool relabel = plan->layout == kLayoutRelabel;
-  const Csc csc = relabel ? Csc{plan->r_off.p, plan->r_src.p, plan->deg.p, V}
-                          : Csc{g.in_off, g.in_src, plan->deg.p, V};
+  const Csc csc = relabel ? Csc{plan->r_off.p, plan->r_src.p, plan->deg.p, V, plan->hot.p}
+                          : Csc{g.in_off, g.in_src, plan->deg.p, V, nullptr};
+  const uint64_t plan_edges = plan->edges;
   hipLaunchKernelGGL(k_pr_init, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, V, plan->deg.p,
                      1.0f / (float)V, rank, contrib);
   float* r_cur = rank;
@@ -525,14 +839,22 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
       }
       hipLaunchKernelGGL(k_seg_finish, dim3(std::min<unsigned>(vblocks, 65536)), dim3(kBlock), 0, s, V,
                          plan->deg.p, base, damping, plan->sum.p, r_nxt, c_nxt);
+    } else if (plan->tiled) {
+      if (plan->lead_rows)
+        hipLaunchKernelGGL(k_pr_lead, dim3(grid_of(plan->lead_rows)), dim3(kBlock), 0, s, plan->lead_rows,
+                           plan->deg.p, base, r_nxt, c_nxt);
+      if (plan->ntiles) {
+        hipLaunchKernelGGL(k_pr_tile, dim3((unsigned)plan->ntiles), dim3(kTileThreads), 0, s, csc, plan_edges,
+                           plan->tile_row.p, c_cur, base, damping, r_nxt, c_nxt, plan->carry.p, plan->head_part.p);
+        hipLaunchKernelGGL(k_pr_fixup, dim3(grid_of(plan->ntiles)), dim3(kBlock), 0, s, csc, plan_edges,
+                           plan->ntiles, plan->tile_row.p, plan->carry.p, plan->head_part.p, base, damping, r_nxt,
+                           c_nxt);
+      }
+    } else if (plan->banded) {
+      launch_tiers<1>(csc, plan->tiers[0], vblocks, c_cur, base, damping, plan->part.p, r_nxt, c_nxt, s);
+      launch_tiers<2>(csc, plan->tiers[1], vblocks, c_cur, base, damping, plan->part.p, r_nxt, c_nxt, s);
     } else {
-      hipLaunchKernelGGL(k_pr_low, dim3(vblocks), dim3(kBlock), 0, s, csc, c_cur, base, damping, r_nxt, c_nxt);
-      if (plan->n_med)
-        hipLaunchKernelGGL(k_pr_med, dim3((plan->n_med + 3) / 4), dim3(kBlock), 0, s, csc, plan->med.p,
-                           plan->n_med, c_cur, base, damping, r_nxt, c_nxt);
-      if (plan->n_high)
-        hipLaunchKernelGGL(k_pr_high, dim3(plan->n_high), dim3(kBlock), 0, s, csc, plan->high.p, c_cur, base,
-                           damping, r_nxt, c_nxt);
+      launch_tiers<0>(csc, plan->tiers[0], vblocks, c_cur, base, damping, nullptr, r_nxt, c_nxt, s);
     }
     if (profile) {
       CG_HIP(hipEventRecord(b, s));

@@ -207,14 +207,16 @@ def test_pagerank_l1_and_determinism(oracle):
     assert st["iterations"] == 20
 
 
-@pytest.mark.parametrize("layout,seg", [(0, 31), (0, 8), (0, 11), (0, 14), (0, 0), (1, 0), (2, 0), (3, 9)])
-def test_pagerank_layouts(oracle, layout, seg):
+@pytest.mark.parametrize("layout,seg,kernel", [(0, 31, 0), (0, 8, 0), (0, 11, 0), (0, 14, 0), (0, 0, 0), (1, 0, 1),
+                                               (1, 0, 2), (2, 0, 1), (2, 0, 2), (2, 0, 3), (2, 12, 3), (2, 20, 3),
+                                               (3, 9, 0)])
+def test_pagerank_layouts(oracle, layout, seg, kernel):
     """Every in-edge layout (plain, renumbered by out-degree, source windows
-    smaller and larger than the graph) against the FP64 oracle; run-to-run
-    bit-identical."""
+    smaller and larger than the graph) and every sweep (row tiers, edge tiles,
+    two source bands) against the FP64 oracle; run-to-run bit-identical."""
     og = oracle.rmat_graph(17, 16, seed=2, weights=None)
     g = cg.Graph.from_csr(og.offsets, og.dest)
-    e = cg.Engine(g, PR, pr_segment_log2=seg, pr_layout=layout)
+    e = cg.Engine(g, PR, pr_segment_log2=seg, pr_layout=layout, pr_kernel=kernel)
     r1, _ = e.run()
     r2, _ = e.run()
     e.close()
@@ -226,7 +228,7 @@ def test_pagerank_layouts(oracle, layout, seg):
     n = 5000
     star = oracle.csr_build(n, np.arange(1, n, dtype=np.uint32), np.zeros(n - 1, np.uint32))
     g2 = cg.Graph.from_csr(star.offsets, star.dest)
-    e = cg.Engine(g2, PR, pr_segment_log2=seg, pr_layout=layout)
+    e = cg.Engine(g2, PR, pr_segment_log2=seg, pr_layout=layout, pr_kernel=kernel)
     r, _ = e.run()
     e.close()
     assert float(np.abs(r.astype(np.float64) - oracle.pagerank64(star, 20, 0.85)).sum()) <= 1e-6
@@ -235,7 +237,7 @@ def test_pagerank_layouts(oracle, layout, seg):
     dst = np.array([5, 8, 3, 3, 3, 5, 7], np.uint32)
     sparse = oracle.csr_build(12, src, dst)
     g3 = cg.Graph.from_csr(sparse.offsets, sparse.dest)
-    e = cg.Engine(g3, PR, pr_segment_log2=seg, pr_layout=layout)
+    e = cg.Engine(g3, PR, pr_segment_log2=seg, pr_layout=layout, pr_kernel=kernel)
     r, _ = e.run()
     e.close()
     assert float(np.abs(r.astype(np.float64) - oracle.pagerank64(sparse, 20, 0.85)).sum()) <= 1e-6
@@ -246,6 +248,8 @@ def test_pagerank_bad_layout(oracle):
     g = cg.Graph.from_csr(og.offsets, og.dest)
     with pytest.raises(cg.CgError):
         cg.Engine(g, PR, pr_layout=4)
+    with pytest.raises(cg.CgError):
+        cg.Engine(g, PR, pr_kernel=4)
 
 
 def test_error_paths(oracle):
