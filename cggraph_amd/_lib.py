@@ -54,6 +54,7 @@ class Options(ctypes.Structure):
         ("pr_layout", ctypes.c_uint32),
         ("relabel", ctypes.c_uint32),
         ("pr_kernel", ctypes.c_uint32),
+        ("min_pull", ctypes.c_uint32),
     ]
 
 

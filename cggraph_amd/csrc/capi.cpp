@@ -195,6 +195,7 @@ int cg_options_default(cg_options* o) {
   o->pr_layout = 0;
   o->relabel = 0;
   o->pr_kernel = 0;
+  o->min_pull = 0;
   return CG_OK;
 }
 

@@ -200,8 +200,21 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
     ++local->heavy_iterations;
     local->gpu_vertices += sst.n_active;
     if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
-    CG_TRY(launch_pull_bfs(dg, e->val.p, in, out, e->stripes.p, s));
+    const double V1 = (double)dg.V;
+    if (alg == CG_BFS) {
+      CG_TRY(launch_pull_bfs(dg, e->val.p, in, out, e->stripes.p, s));
+      // lower bound: offsets, value check, both bitmaps (the scanned in-edges
+      // depend on where each search stops and are not counted)
+      local->expand_alg_bytes += 8.0 * (V1 + 1) + 4.0 * V1 + 16.0 * (double)words;
+    } else {
+      CG_TRY(launch_pull_min(alg, dg, e->g->E, e->pull_tiles.p, e->val.p, in, out, e->stripes.p, s));
+      // offsets, in-edge sources (+ weights), the frontier sources' values
+      // (one per frontier out-edge), the row values, both bitmaps
+      local->expand_alg_bytes += 8.0 * (V1 + 1) + (alg == CG_SSSP ? 8.0 : 4.0) * (double)e->g->E +
+                                 4.0 * (double)sst.m_active + 4.0 * V1 + 16.0 * (double)words;
+    }
     if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); *timing_pending = true; }
+    ++local->expand_launches;
     local->edges_relaxed += sst.m_active;
     return CG_OK;
   }
@@ -278,7 +291,7 @@ int probe_all(cg_engine* e, const DevGraph& dg, const DevStats& sst, DevStats* d
   double by_key[6] = {-1, -1, -1, -1, -1, -1};
   for (int c = 0; c < CG_NUM_CHOICES; ++c) {
     Step st{decode_choice(c), 0, threshold};
-    if (e->alg != CG_BFS) st.ch.pull = 0;
+    if (e->alg != CG_BFS && !e->pull_tiles_ready) st.ch.pull = 0;
     st.host_share = (st.ch.host && !st.ch.pull) ? 1 : 0;
     const int key = st.ch.pull ? 4 : (st.host_share ? 5 : st.ch.tier);
     if (by_key[key] >= 0) {
@@ -313,6 +326,19 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   const int mode = e->opt.sched_mode;
   if (alg == CG_SSSP && !g.has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
   if (alg == CG_BFS && mode != CG_SCHED_HEURISTIC) CG_TRY(build_transpose(&g));
+  // bottom-up min steps (SSSP, label propagation): transposed graph + edge tiles
+  const bool forced_pull = mode == CG_SCHED_FORCED && decode_choice(e->opt.forced_choice).pull;
+  const bool min_pull = (alg == CG_SSSP || alg == CG_CC) && g.E > 0 &&
+                        ((e->opt.min_pull && (mode == CG_SCHED_GPU || mode == CG_SCHED_LEARNED)) || forced_pull);
+  if (min_pull) {
+    CG_TRY(build_transpose(&g));
+    if (!e->pull_tiles_ready) {
+      CG_TRY(e->pull_tiles.ensure(pull_tile_count(g.E) + 1));
+      CG_TRY(build_pull_tiles(g.d_in_off.p, V, g.E, e->pull_tiles.p, s));
+      CG_HIP(hipStreamSynchronize(s));
+      e->pull_tiles_ready = true;
+    }
+  }
   const bool learned = mode == CG_SCHED_LEARNED;
   const bool probe = mode == CG_SCHED_PROBE;
   if (learned) CG_TRY(mlp_load(e));
@@ -322,7 +348,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   const float* mlp_dev = (learned && e->mlp_valid) ? e->d_mlp.p : nullptr;
 
   DevGraph dg{g.d_off.p, g.d_dst.p, g.has_w ? g.d_w.p : nullptr,
-              g.dev_t_valid ? g.d_in_off.p : nullptr, g.dev_t_valid ? g.d_in_src.p : nullptr, V};
+              g.dev_t_valid ? g.d_in_off.p : nullptr, g.dev_t_valid ? g.d_in_src.p : nullptr, V,
+              g.dev_t_valid && g.has_w ? g.d_in_w.p : nullptr};
 
   cg_run_stats local{};
   const auto t0 = clk::now();
@@ -405,8 +432,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
           ++local.learned_fallbacks;
         }
       }
+      // bottom-up min step when the frontier's edges reach min_pull / 1000 of E
+      if (min_pull) step.ch.pull = sst.m_active * 1000ull >= (uint64_t)e->opt.min_pull * g.E;
     }
-    if (alg != CG_BFS) step.ch.pull = 0;
+    if (alg != CG_BFS && !(min_pull && (mode != CG_SCHED_HEURISTIC))) step.ch.pull = 0;
     if (step.ch.pull) step.host_share = 0;  // bottom-up steps run on the GPU
     pull = step.ch.pull;
 

@@ -53,6 +53,10 @@ struct cg_engine {
   cg::DevBuf<uint32_t> perm, val_out;
   std::vector<uint32_t> new_of;
 
+  // bottom-up min steps: row of the first in-edge of each edge tile
+  cg::DevBuf<uint32_t> pull_tiles;
+  bool pull_tiles_ready = false;
+
   // PageRank
   cg::DevBuf<float> rank, rank2, contrib, contrib2;
   cg::PrPlan* plan = nullptr;

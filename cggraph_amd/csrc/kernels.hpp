@@ -55,6 +55,7 @@ struct DevGraph {
   const uint64_t* in_off;
   const uint32_t* in_src;
   uint32_t V;
+  const uint32_t* in_w = nullptr;  // weights of the in-edges (bottom-up SSSP)
 };
 
 struct Bins {
@@ -107,6 +108,15 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
 // Bottom-up BFS step: unreached vertices look for a parent in `in`.
 int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
                     uint32_t* stripes, hipStream_t s);
+// Bottom-up SSSP / label propagation: every vertex takes the minimum over its
+// in-edges from frontier vertices.  The in-edges are cut into tiles of
+// pull_tile_edges(); tile_row (pull_tile_count() + 1 entries, from
+// build_pull_tiles) holds the row of each tile's first edge.
+uint64_t pull_tile_edges();
+uint64_t pull_tile_count(uint64_t E);
+int build_pull_tiles(const uint64_t* in_off, uint32_t V, uint64_t E, uint32_t* tile_row, hipStream_t s);
+int launch_pull_min(int alg, const DevGraph& g, uint64_t E, const uint32_t* tile_row, uint32_t* val,
+                    const uint64_t* in, uint64_t* out, uint32_t* stripes, hipStream_t s);
 // Applies host-produced (vertex, value) candidates with atomic min and marks
 // improved vertices in `out`.
 int launch_apply_candidates(const uint32_t* ids, const uint32_t* vals, uint32_t n, uint32_t* val,
@@ -134,8 +144,8 @@ struct PrPlan;  // per-graph layout for the pull kernels (tiers, relabelled CSC 
 // 3 source windows of 2^seg_log2 vertices.  Automatic: seg_log2 in [1, 30]
 // asks for windows, 31 for plain pull, 0 picks relabelled pull above 2^16
 // vertices.
-// kernel (plain / relabelled layouts): 0 automatic, 1 row tiers, 2 edge tiles, 3 two source
-// bands (relabelled layout; boundary 2^seg_log2, default 2^19).
+// kernel (plain / relabelled layouts): 0 automatic (row tiers), 1 row tiers, 2 edge tiles,
+// 3 two source bands (relabelled layout; boundary 2^seg_log2, default 2^19).
 int pr_plan_create(cg_graph* g, uint32_t layout, uint32_t seg_log2, uint32_t kernel, PrPlan** plan,
                    hipStream_t s);
 void pr_plan_destroy(PrPlan* plan);

@@ -780,13 +780,12 @@ int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, uint32_t
     p->layout = kLayoutPlain;
   }
   if (st == CG_OK && p->layout == kLayoutRelabel) {
-    // automatic: two source bands of 2^19 vertices (2 MB of contributions)
-    if (kernel == 0 || kernel == 3) {
+    if (kernel == 3) {
       st = build_bands(g.V, seg_log2 >= 1 && seg_log2 <= 30 ? seg_log2 : 19, p, s);
-    } else if (kernel == 1) {
-      st = build_tiers<0>(Csc{p->r_off.p, p->r_src.p, p->deg.p, g.V, nullptr}, p->tiers[0], s);
-    } else {
+    } else if (kernel == 2) {
       st = build_tiles(p->r_off.p, g.V, E, p, s);
+    } else {
+      st = build_tiers<0>(Csc{p->r_off.p, p->r_src.p, p->deg.p, g.V, nullptr}, p->tiers[0], s);
     }
   } else if (st == CG_OK && p->layout == kLayoutPlain && kernel == 2) {
     st = build_tiles(g.in_off, g.V, E, p, s);

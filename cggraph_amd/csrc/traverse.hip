@@ -500,6 +500,116 @@ __global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __res
   }
 }
 
+// Bottom-up SSSP and label propagation: thread j of a workgroup takes kPullK
+// consecutive in-edges of the transposed graph (whatever rows they belong
+// to), keeps the candidates of those whose source is in the frontier, and
+// folds them into one atomic min per row segment.  The minimum is exact and
+// order-free, so no cross-thread combination is needed; the fixed point is
+// the push engine's.
+constexpr int kPullK = 8;
+constexpr uint64_t kPullTile = (uint64_t)kBlock * kPullK;
+typedef uint32_t pull_u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void k_pull_tile_rows(const uint64_t* __restrict__ in_off, uint32_t V, uint64_t ntiles,
+                                 uint32_t* __restrict__ tile_row) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t <= ntiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    if (t == ntiles) {
+      tile_row[t] = V;
+      continue;
+    }
+    const uint64_t e = t * kPullTile;
+    uint32_t lo = 0, hi = V;  // in_off[V] = E > e
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (in_off[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    tile_row[t] = lo;
+  }
+}
+
+__device__ __forceinline__ uint32_t pull_commit(uint32_t v, uint32_t m, uint32_t* __restrict__ val,
+                                                uint64_t* __restrict__ out) {
+  if (m < val[v]) {
+    const uint32_t old = atomicMin(val + v, m);
+    if (m < old) {
+      atomicOr(reinterpret_cast<unsigned long long*>(out) + (v >> 6), 1ull << (v & 63));
+      return old == kUnreached ? 1u : 0u;
+    }
+  }
+  return 0u;
+}
+
+template <int ALG>
+__global__ __launch_bounds__(kBlock) void k_pull_min(DevGraph g, uint64_t E, const uint32_t* __restrict__ tile_row,
+                                                     uint32_t* __restrict__ val, const uint64_t* __restrict__ in,
+                                                     uint64_t* __restrict__ out, uint32_t* __restrict__ stripes) {
+  const uint64_t t = blockIdx.x;
+  const uint64_t e0 = t * kPullTile + (uint64_t)threadIdx.x * kPullK;
+  uint32_t n_new = 0;
+  if (e0 < E) {
+    const uint32_t nk = (E - e0) < (uint64_t)kPullK ? (uint32_t)(E - e0) : (uint32_t)kPullK;
+    uint32_t lo = tile_row[t], hi = tile_row[t + 1];
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (g.in_off[mid] <= e0) lo = mid;
+      else hi = mid - 1;
+    }
+    uint32_t u[kPullK], w[kPullK];
+    if (nk == (uint32_t)kPullK) {
+      const pull_u32x4* ps = reinterpret_cast<const pull_u32x4*>(g.in_src + e0);
+      const pull_u32x4 a = ps[0], b = ps[1];
+      u[0] = a.x; u[1] = a.y; u[2] = a.z; u[3] = a.w;
+      u[4] = b.x; u[5] = b.y; u[6] = b.z; u[7] = b.w;
+      if constexpr (ALG == ALG_SSSP) {
+        const pull_u32x4* pw = reinterpret_cast<const pull_u32x4*>(g.in_w + e0);
+        const pull_u32x4 c = pw[0], d = pw[1];
+        w[0] = c.x; w[1] = c.y; w[2] = c.z; w[3] = c.w;
+        w[4] = d.x; w[5] = d.y; w[6] = d.z; w[7] = d.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kPullK; ++i) {
+        u[i] = (uint32_t)i < nk ? g.in_src[e0 + i] : 0u;
+        if constexpr (ALG == ALG_SSSP) w[i] = (uint32_t)i < nk ? g.in_w[e0 + i] : 0u;
+      }
+    }
+    uint32_t cand[kPullK];
+#pragma unroll
+    for (int i = 0; i < kPullK; ++i) {
+      cand[i] = 0xFFFFFFFFu;
+      if ((uint32_t)i < nk && ((in[u[i] >> 6] >> (u[i] & 63)) & 1ull)) {
+        if constexpr (ALG == ALG_SSSP) cand[i] = val[u[i]] + w[i];
+        else cand[i] = val[u[i]];
+      }
+    }
+    uint32_t r = lo;
+    uint64_t next_end = g.in_off[r + 1];
+    uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < kPullK; ++i) {
+      if ((uint32_t)i < nk) {
+        m = cand[i] < m ? cand[i] : m;
+        const uint64_t e = e0 + i + 1;
+        if (next_end == e) {  // row r ends after this edge (and empty rows after it)
+          if (m != 0xFFFFFFFFu) n_new += pull_commit(r, m, val, out);
+          m = 0xFFFFFFFFu;
+          while (next_end == e) {
+            ++r;
+            next_end = r < g.V ? g.in_off[r + 1] : ~0ull;
+          }
+        }
+      }
+    }
+    if (m != 0xFFFFFFFFu) n_new += pull_commit(r, m, val, out);  // the row goes on past this thread
+  }
+  if constexpr (ALG != ALG_CC) {
+    const uint32_t tot = wave_sum_u32(n_new);
+    if (lane_id() == 0) stripe_add(stripes, tot);
+  }
+}
+
 __global__ void k_apply_candidates(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ vals,
                                    uint32_t n, uint32_t* __restrict__ val, uint64_t* __restrict__ out,
                                    uint32_t* __restrict__ stripes) {
@@ -696,6 +806,37 @@ int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64
   const uint64_t threads = ((uint64_t)g.V + 63) / 64 * 64;
   const uint64_t blocks = (threads + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_pull_bfs, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, stripes);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+uint64_t pull_tile_edges() { return kPullTile; }
+
+uint64_t pull_tile_count(uint64_t E) { return (E + kPullTile - 1) / kPullTile; }
+
+int build_pull_tiles(const uint64_t* in_off, uint32_t V, uint64_t E, uint32_t* tile_row, hipStream_t s) {
+  const uint64_t nt = pull_tile_count(E);
+  const uint64_t blocks = (nt + 1 + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_pull_tile_rows, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(kBlock), 0, s, in_off,
+                     V, nt, tile_row);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_pull_min(int alg, const DevGraph& g, uint64_t E, const uint32_t* tile_row, uint32_t* val,
+                    const uint64_t* in, uint64_t* out, uint32_t* stripes, hipStream_t s) {
+  if (!g.in_off || !g.in_src || !tile_row) return set_error(CG_ERR_INTERNAL, "pull step without in-edges");
+  if (alg == CG_SSSP && !g.in_w) return set_error(CG_ERR_INTERNAL, "pull SSSP step without in-edge weights");
+  const uint64_t nt = pull_tile_count(E);
+  if (!nt) return CG_OK;
+  if (alg == CG_SSSP)
+    hipLaunchKernelGGL(k_pull_min<ALG_SSSP>, dim3((unsigned)nt), dim3(kBlock), 0, s, g, E, tile_row, val, in, out,
+                       stripes);
+  else if (alg == CG_CC)
+    hipLaunchKernelGGL(k_pull_min<ALG_CC>, dim3((unsigned)nt), dim3(kBlock), 0, s, g, E, tile_row, val, in, out,
+                       stripes);
+  else
+    return set_error(CG_ERR_INTERNAL, "min-pull step for algorithm %d", alg);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

@@ -121,13 +121,18 @@ typedef struct cg_options {
                                      ids): 0 automatic (at least 65 536 vertices and the top 1/64
                                      of vertices send at least half of the edges), 1 never,
                                      2 always */
-  uint32_t pr_kernel;             /* PageRank sweep over layouts 1 and 2: 0 automatic (3 on the
-                                     renumbered layout, 1 on the plain one), 1 row tiers (a lane /
+  uint32_t pr_kernel;             /* PageRank sweep over layouts 1 and 2: 0 automatic (row tiers,
+                                     the fastest measured), 1 row tiers (a lane /
                                      wavefront / workgroup per row by in-degree), 2 edge tiles
                                      (2048 consecutive in-edges per workgroup, rows combined by a
                                      segmented scan), 3 two source bands (renumbered layout: the
                                      sources below 2^pr_segment_log2, default 2^19, first, whose
                                      contributions stay in L2, then the rest) */
+  uint32_t min_pull;              /* SSSP / label-propagation CC under the GPU and learned
+                                     schedules: an iteration whose frontier edges reach
+                                     min_pull / 1000 of E runs bottom-up (every vertex takes the
+                                     minimum over its in-edges from the frontier; no per-edge
+                                     atomics); 0 = always push */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

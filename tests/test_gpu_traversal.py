@@ -350,3 +350,23 @@ def test_renumbered_lattice_and_tiny(oracle):
             assert np.array_equal(got, oracle.bfs(sp_w, root) if alg == BFS else oracle.sssp(sp_w, root))
     with pytest.raises(cg.CgError):
         cg.Engine(g2, BFS, relabel=3)
+
+
+@pytest.mark.parametrize("min_pull", [1, 60, 250, 1000])
+def test_min_pull_rule(oracle, min_pull):
+    """Bottom-up SSSP / label-propagation steps chosen by the frontier-edge
+    rule (GPU schedule, and renumbered engines) reach the push fixed point."""
+    og = oracle.rmat_graph(14, 16, seed=4, weights=(1, 100))
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    for relabel in (1, 2):
+        got, st = run(g, SSSP, root, min_pull=min_pull, relabel=relabel)
+        assert np.array_equal(got, oracle.sssp(og, root)), (min_pull, relabel)
+        if min_pull == 1:
+            assert st["pull_iterations"] >= 1
+        got, st = run(g, CC, 0, min_pull=min_pull)
+        assert np.array_equal(got, oracle.cc_directed(og)), min_pull
+    lat = oracle.grid_graph(50, 0.1, seed=8)
+    gl = gpu_graph(lat)
+    got, _ = run(gl, SSSP, 77, min_pull=min_pull, sssp_delta=300)
+    assert np.array_equal(got, oracle.sssp(lat, 77))
