@@ -55,6 +55,7 @@ class Options(ctypes.Structure):
         ("relabel", ctypes.c_uint32),
         ("pr_kernel", ctypes.c_uint32),
         ("min_pull", ctypes.c_uint32),
+        ("expand_grid", ctypes.c_uint32),
     ]
 
 

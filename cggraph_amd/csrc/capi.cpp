@@ -196,6 +196,7 @@ int cg_options_default(cg_options* o) {
   o->relabel = 0;
   o->pr_kernel = 0;
   o->min_pull = 0;
+  o->expand_grid = 0;
   return CG_OK;
 }
 

@@ -231,7 +231,7 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
   }
   // GPU push, optionally co-executed with the host team
   ++local->heavy_iterations;
-  Bins bins{e->opt.bin_wave, e->opt.bin_chunk, e->opt.chunk_edges};
+  Bins bins{e->opt.bin_wave, e->opt.bin_chunk, e->opt.chunk_edges, e->opt.expand_grid};
   if (step.ch.tier) {
     bins.bin_wave = kTierSets[step.ch.tier].bin_wave;
     bins.bin_chunk = kTierSets[step.ch.tier].bin_chunk;

@@ -62,6 +62,7 @@ struct Bins {
   uint32_t bin_wave;
   uint32_t bin_chunk;
   uint32_t chunk_edges;
+  uint32_t grid_cap = 0;  // push expansion: most workgroups launched (0: 2048)
 };
 
 // ---- traversal (traverse.hip) ----

@@ -415,54 +415,58 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
                                                    uint32_t small_limit, uint32_t chunk_edges,
                                                    const DevStats* __restrict__ st,
                                                    uint32_t* __restrict__ stripes) {
-  // The grid is sized from an upper bound; the queue lengths come from the
-  // compaction and surplus wavefronts leave at once.
+  // The queue lengths come from the compaction; the wavefronts of a grid
+  // capped near one chip's worth of resident workgroups stride over the items
+  // (chunks first, the heaviest), so no wavefront is launched only to find
+  // nothing to do.
   const uint32_t n_chunk = st->n_chunk, n_med = st->n_med;
   const uint32_t n_small = st->n_small < small_limit ? st->n_small : small_limit;
+  const uint64_t n_items = (uint64_t)n_chunk + n_med + (n_small + kWave - 1) / kWave;
   const uint32_t lane = lane_id();
-  const uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id();
+  const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
   uint32_t n_new = 0;
-  if (item < n_chunk) {
-    // a slice of a high-degree vertex: lanes stride its edges
-    const uint64_t pk = q_chunk[item];
-    const uint32_t v = (uint32_t)(pk >> 32);
-    const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
-    const uint64_t end = g.off[v + 1];
-    const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
-    n_new += relax_strided<ALG, EXACT>(val[v], s, e, lane, g.dst, g.w, val, out);
-  } else if (item < (uint64_t)n_chunk + n_med) {
-    const uint32_t v = q_med[item - n_chunk];
-    n_new += relax_strided<ALG, EXACT>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out);
-  } else {
-    // 64 low-degree vertices per wavefront, their edges spread evenly over
-    // the lanes: lane j of round r takes edge r*64+j of the concatenated
-    // adjacency lists and finds its owner by binary search over the exclusive
-    // degree prefix held one entry per lane.
-    if (item >= (uint64_t)n_chunk + n_med + (n_small + kWave - 1) / kWave) return;
-    const uint64_t base = (item - n_chunk - n_med) * kWave;
-    const uint32_t cnt = (n_small - base) < (uint64_t)kWave ? (uint32_t)(n_small - base) : kWave;
-    uint32_t deg = 0, sv = 0;
-    uint64_t s = 0;
-    if (lane < cnt) {
-      const uint32_t v = q_small[base + lane];
-      s = g.off[v];
-      deg = (uint32_t)(g.off[v + 1] - s);
-      sv = val[v];
-    }
-    uint32_t total;
-    const uint32_t excl = wave_excl_scan(deg, &total);
-    for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
-      const uint32_t j = j0 + lane;
-      uint32_t lo = 0;
-#pragma unroll
-      for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
-        const uint32_t pm = __shfl(excl, (int)(lo + step), kWave);
-        if (pm <= j) lo += step;
+  for (uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); item < n_items; item += stride) {
+    if (item < n_chunk) {
+      // a slice of a high-degree vertex: lanes stride its edges
+      const uint64_t pk = q_chunk[item];
+      const uint32_t v = (uint32_t)(pk >> 32);
+      const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
+      const uint64_t end = g.off[v + 1];
+      const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
+      n_new += relax_strided<ALG, EXACT>(val[v], s, e, lane, g.dst, g.w, val, out);
+    } else if (item < (uint64_t)n_chunk + n_med) {
+      const uint32_t v = q_med[item - n_chunk];
+      n_new += relax_strided<ALG, EXACT>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out);
+    } else {
+      // 64 low-degree vertices per wavefront, their edges spread evenly over
+      // the lanes: lane j of round r takes edge r*64+j of the concatenated
+      // adjacency lists and finds its owner by binary search over the
+      // exclusive degree prefix held one entry per lane.
+      const uint64_t base = (item - n_chunk - n_med) * kWave;
+      const uint32_t cnt = (n_small - base) < (uint64_t)kWave ? (uint32_t)(n_small - base) : kWave;
+      uint32_t deg = 0, sv = 0;
+      uint64_t s = 0;
+      if (lane < cnt) {
+        const uint32_t v = q_small[base + lane];
+        s = g.off[v];
+        deg = (uint32_t)(g.off[v + 1] - s);
+        sv = val[v];
       }
-      const uint64_t os = __shfl(s, (int)lo, kWave);
-      const uint32_t osv = __shfl(sv, (int)lo, kWave);
-      const uint32_t oex = __shfl(excl, (int)lo, kWave);
-      if (j < total) n_new += relax<ALG, EXACT>(osv, os + (j - oex), g.dst, g.w, val, out);
+      uint32_t total;
+      const uint32_t excl = wave_excl_scan(deg, &total);
+      for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+        const uint32_t j = j0 + lane;
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
+          const uint32_t pm = __shfl(excl, (int)(lo + step), kWave);
+          if (pm <= j) lo += step;
+        }
+        const uint64_t os = __shfl(s, (int)lo, kWave);
+        const uint32_t osv = __shfl(sv, (int)lo, kWave);
+        const uint32_t oex = __shfl(excl, (int)lo, kWave);
+        if (j < total) n_new += relax<ALG, EXACT>(osv, os + (j - oex), g.dst, g.w, val, out);
+      }
     }
   }
   if constexpr (ALG != ALG_CC) {
@@ -779,8 +783,9 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
                        uint32_t* stripes, bool exact, hipStream_t s) {
   if (!item_bound) return CG_OK;
-  const uint64_t blocks = (item_bound + kWavesPerBlock - 1) / kWavesPerBlock;
-  if (blocks >= (1ull << 31)) return set_error(CG_ERR_INTERNAL, "expansion grid too large");
+  uint64_t blocks = (item_bound + kWavesPerBlock - 1) / kWavesPerBlock;
+  const uint64_t cap = bins.grid_cap ? bins.grid_cap : 2048;  // 256 CUs x 32 wavefronts
+  if (blocks > cap) blocks = cap;
   dim3 grid((unsigned)blocks), block(kBlock);
 #define CG_EXPAND(A, X)                                                                          \
   hipLaunchKernelGGL((k_expand<A, X>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk, small_limit, \

@@ -133,6 +133,9 @@ typedef struct cg_options {
                                      min_pull / 1000 of E runs bottom-up (every vertex takes the
                                      minimum over its in-edges from the frontier; no per-edge
                                      atomics); 0 = always push */
+  uint32_t expand_grid;           /* push expansion: most workgroups per launch, whose wavefronts
+                                     stride over the work items; 0 = 2048 (256 CUs x 32
+                                     wavefronts) */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

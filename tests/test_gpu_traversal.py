@@ -370,3 +370,17 @@ def test_min_pull_rule(oracle, min_pull):
     gl = gpu_graph(lat)
     got, _ = run(gl, SSSP, 77, min_pull=min_pull, sssp_delta=300)
     assert np.array_equal(got, oracle.sssp(lat, 77))
+
+
+@pytest.mark.parametrize("cap", [1, 7, 100000])
+def test_expand_grid_caps(oracle, cap):
+    """However few workgroups the push expansion gets, its wavefronts cover
+    every work item."""
+    og = random_graph(oracle, 12, 20000, 150000, hub=20000)
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    for alg, ref in ((BFS, oracle.bfs(og, root)), (SSSP, oracle.sssp(og, root)), (CC, oracle.cc_directed(og))):
+        got, _ = run(g, alg, root, expand_grid=cap, sched_mode=_lib.CG_SCHED_FORCED, forced_choice=2)
+        assert np.array_equal(got, ref), (alg, cap)
+        got, _ = run(g, alg, root, expand_grid=cap)
+        assert np.array_equal(got, ref), (alg, cap)
