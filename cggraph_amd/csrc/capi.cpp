@@ -450,26 +450,28 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   CG_HIP(hipEventCreate(&e->ev0));
   CG_HIP(hipEventCreate(&e->ev1));
   CG_HIP(hipEventCreateWithFlags(&e->ev_copy, hipEventDisableTiming));
+  for (auto& pr : e->ev_pair)
+    for (auto& ev : pr) CG_HIP(hipEventCreate(&ev));
   if (e->opt.relabel > 2) return set_error(CG_ERR_INVALID, "relabel %u outside [0, 2]", e->opt.relabel);
   if ((algorithm == CG_BFS || algorithm == CG_SSSP) && e->opt.relabel != 1 && g->V > 0) {
     // Renumber by descending out-degree when asked, or (automatic) when the
     // top 1/64 of vertices send at least half of the edges: the values those
     // edges touch then share few cache lines.
-    cg::DevBuf<uint32_t> new_of_d;
+    cg::DevBuf<uint32_t> perm;
     double share = 0.0;
-    CG_TRY(degree_order(g, e->perm, new_of_d, &share));
+    CG_TRY(degree_order(g, perm, e->new_of_d, &share));
     if (e->opt.relabel == 2 || (g->V >= (1u << 16) && share >= 0.5)) {
       e->own_g.reset(new (std::nothrow) cg_graph());
       if (!e->own_g) return set_error(CG_ERR_OOM, "renumbered graph object");
-      CG_TRY(relabel_graph(g, new_of_d.p, e->own_g.get()));
+      CG_TRY(relabel_graph(g, e->new_of_d.p, e->own_g.get()));
       e->new_of.resize(g->V);
-      CG_HIP(hipMemcpy(e->new_of.data(), new_of_d.p, (size_t)g->V * 4, hipMemcpyDeviceToHost));
+      CG_HIP(hipMemcpy(e->new_of.data(), e->new_of_d.p, (size_t)g->V * 4, hipMemcpyDeviceToHost));
       CG_TRY(e->val_out.ensure(g->V));
       e->g = e->own_g.get();
       e->renumbered = true;
       g = e->g;
     } else {
-      e->perm.reset();
+      e->new_of_d.reset();
     }
   }
   const uint64_t V = g->V;
@@ -556,11 +558,14 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
     rs.edges_relaxed = g.E;
     rs.expand_alg_bytes = 8.0 * ((double)g.V + 1) + 4.0 * (double)g.E + 8.0 * (double)g.V;
   } else if (e->renumbered) {
-    const auto t0 = std::chrono::steady_clock::now();
+    // the query's time includes bringing the values back to the caller's ids
     st = engine_run(e, e->new_of[root], &rs);
-    if (st == CG_OK) st = launch_unpermute_u32(e->perm.p, e->val.p, g.V, e->val_out.p, e->stream);
-    if (st == CG_OK) CG_HIP(hipStreamSynchronize(e->stream));
-    rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st == CG_OK) {
+      const auto t0 = std::chrono::steady_clock::now();
+      st = launch_renumber_back(e->new_of_d.p, e->val.p, g.V, e->val_out.p, e->stream);
+      if (st == CG_OK) CG_HIP(hipStreamSynchronize(e->stream));
+      rs.ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
   } else {
     st = engine_run(e, root, &rs);
   }

@@ -380,6 +380,22 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   const uint32_t delta = alg == CG_SSSP ? e->opt.sssp_delta : 0;
   uint32_t threshold = delta ? delta : 0xFFFFFFFFu;
   const uint32_t* tval = delta ? e->val.p : nullptr;
+  // GPU schedule for SSSP / CC without bottom-up steps: the tiers are fixed
+  // and nothing waits for the statistics, so compaction and expansion are
+  // queued right behind them and the host reads the statistics while the GPU
+  // goes on (BFS decides push or pull from the statistics first).
+  const bool early = mode == CG_SCHED_GPU && !host_possible && alg != CG_BFS && !min_pull;
+  const Bins fbins{e->opt.bin_wave, e->opt.bin_chunk, e->opt.chunk_edges, e->opt.expand_grid};
+  bool early_timed[2] = {false, false};
+  auto take_timing = [&](int k) -> int {
+    if (!early_timed[k]) return CG_OK;
+    float ms = 0.f;
+    CG_HIP(hipEventSynchronize(e->ev_pair[k][1]));
+    CG_HIP(hipEventElapsedTime(&ms, e->ev_pair[k][0], e->ev_pair[k][1]));
+    local.expand_kernel_ms += ms;
+    early_timed[k] = false;
+    return CG_OK;
+  };
   auto tit = clk::now();
   for (;;) {
     ++it;
@@ -388,10 +404,23 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     CG_TRY(launch_stats(e->bm[cur].p, e->bm[cur ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p,
                         e->run_state.p, mlp_dev, tval, threshold, dst, e->st.p + (slot ^ 1),
                         e->hst_map_dev + slot, s));
+    if (early) {
+      CG_TRY(launch_compact(e->bm[cur].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, tval,
+                            threshold, e->bm[cur ^ 1].p, s));
+      if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[slot][0], s));
+      CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[cur ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
+                                ~0ull >> 8, 0xFFFFFFFFu, fbins, dst, e->stripes.p, e->opt.atomic_return != 0, s));
+      if (e->opt.profile) {
+        CG_HIP(hipEventRecord(e->ev_pair[slot][1], s));
+        early_timed[slot] = true;
+      }
+      ++local.expand_launches;
+    }
     CG_TRY(wait_stats(e, slot, it));
     if (e->opt.fault_flags & CG_FAULT_HIP_ERROR)
       return set_error(CG_ERR_HIP, "injected HIP failure at iteration %u", it);
     const DevStats sst = e->hst_last;
+    if (early) CG_TRY(take_timing(slot ^ 1));  // the previous expansion has finished
     if (it > 1) {
       if (timing_pending) {
         float ms = 0.f;
@@ -455,10 +484,23 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       CG_TRY(probe_all(e, dg, sst, dst, in, out, threshold, us));
       write_probe(e, rec, root, us);
     }
-    CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));
+    if (early) {
+      // already queued behind the statistics
+      ++local.heavy_iterations;
+      local.gpu_vertices += sst.n_active;
+      local.edges_relaxed += sst.m_active;
+      local.expand_alg_bytes += 20.0 * (sst.n_active - sst.n_zero_deg) +
+                                (alg == CG_SSSP ? 12.0 : 8.0) * (double)sst.m_active;
+    } else {
+      CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));
+    }
     cur ^= 1;
   }
   CG_HIP(hipStreamSynchronize(s));
+  if (early) {
+    CG_TRY(take_timing(0));
+    CG_TRY(take_timing(1));
+  }
   local.iterations = it - 1;
   local.ms = ms_since(t0);
   e->cur = cur;
@@ -487,6 +529,9 @@ cg_engine::~cg_engine() {
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (ev_copy) (void)hipEventDestroy(ev_copy);
+  for (auto& pr : ev_pair)
+    for (auto ev : pr)
+      if (ev) (void)hipEventDestroy(ev);
   if (stream) (void)hipStreamDestroy(stream);
   if (trace) std::fclose(trace);
 }

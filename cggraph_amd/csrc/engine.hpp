@@ -19,6 +19,7 @@ struct cg_engine {
   cg_options opt{};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
+  hipEvent_t ev_pair[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // expansions queued ahead, by slot
 
   // values (uint32 for BFS/SSSP/CC, float for PR) and frontier state
   cg::DevBuf<uint32_t> val;
@@ -47,10 +48,10 @@ struct cg_engine {
   bool team_ready = false;
 
   // renumbering by out-degree (BFS / SSSP): g is an engine-owned copy whose
-  // vertex new_of[v] is the caller's v; results are scattered back through perm
+  // vertex new_of[v] is the caller's v; results are gathered back through it
   bool renumbered = false;
   std::unique_ptr<cg_graph> own_g;
-  cg::DevBuf<uint32_t> perm, val_out;
+  cg::DevBuf<uint32_t> new_of_d, val_out;
   std::vector<uint32_t> new_of;
 
   // bottom-up min steps: row of the first in-edge of each edge tile

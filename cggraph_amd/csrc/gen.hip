@@ -556,11 +556,12 @@ __global__ void k_relabel_keys(const uint32_t* __restrict__ src, const uint32_t*
     keys[e] = ((uint64_t)new_of[src[e]] << bits) | new_of[dst[e]];
 }
 
-__global__ void k_unpermute_u32(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in, uint32_t V,
-                                uint32_t* __restrict__ out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    out[perm[i]] = in[i];
+// out[v] = in[new_of[v]]: coalesced writes, gathered reads
+__global__ void k_gather_u32(const uint32_t* __restrict__ new_of, const uint32_t* __restrict__ in, uint32_t V,
+                             uint32_t* __restrict__ out) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    out[v] = in[new_of[v]];
 }
 
 }  // namespace
@@ -676,9 +677,9 @@ int relabel_graph(cg_graph* g, const uint32_t* new_of, cg_graph* rg) {
   return device_max_outdeg(rg);
 }
 
-int launch_unpermute_u32(const uint32_t* perm, const uint32_t* in, uint32_t V, uint32_t* out, hipStream_t s) {
+int launch_renumber_back(const uint32_t* new_of, const uint32_t* in, uint32_t V, uint32_t* out, hipStream_t s) {
   if (!V) return CG_OK;
-  hipLaunchKernelGGL(k_unpermute_u32, dim3(grid_of(V)), dim3(kBlock), 0, s, perm, in, V, out);
+  hipLaunchKernelGGL(k_gather_u32, dim3(grid_of(V)), dim3(kBlock), 0, s, new_of, in, V, out);
   LAUNCH_OK();
   return CG_OK;
 }

@@ -170,7 +170,7 @@ int edge_sources(const uint64_t* off, uint32_t V, uint64_t E, DevBuf<uint32_t>& 
 int degree_order(cg_graph* g, DevBuf<uint32_t>& perm, DevBuf<uint32_t>& new_of, double* top_share);
 // rg = g with every vertex v renamed new_of[v] (device CSR, rows sorted).
 int relabel_graph(cg_graph* g, const uint32_t* new_of, cg_graph* rg);
-// out[perm[i]] = in[i]
-int launch_unpermute_u32(const uint32_t* perm, const uint32_t* in, uint32_t V, uint32_t* out, hipStream_t s);
+// out[v] = in[new_of[v]]: values of the renumbered graph back in the caller's ids
+int launch_renumber_back(const uint32_t* new_of, const uint32_t* in, uint32_t V, uint32_t* out, hipStream_t s);
 
 }  // namespace cg

@@ -183,55 +183,16 @@ __global__ void k_init_run_state(RunState* rs, unsigned long long visited) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ in,
-                                                  uint64_t* __restrict__ zero_me, uint64_t words,
-                                                  DevGraph g, uint64_t E, StatsPart* __restrict__ parts,
-                                                  uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
-                                                  const float* __restrict__ mlp,
-                                                  const uint32_t* __restrict__ val, uint32_t threshold,
-                                                  DevStats* __restrict__ st, DevStats* __restrict__ other,
-                                                  DevStats* __restrict__ host_out) {
+// k_stats grid cap: each workgroup leaves one partial and takes one ticket
+constexpr uint64_t kStatsMaxBlocks = 512;
+
+// Sums the wavefronts' partials of a workgroup, stores the workgroup's
+// partial and takes a ticket; returns true (workgroup-uniform) in the last
+// workgroup to finish.
+__device__ __forceinline__ bool stats_block_done(const StatsPart& p, StatsPart* __restrict__ parts,
+                                                 DevStats* __restrict__ st) {
   __shared__ StatsPart wpart[kWavesPerBlock];
   __shared__ bool is_last;
-  const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
-  uint64_t word = 0;
-  if (wi < words) {
-    word = in[wi];
-    if (zero_me) zero_me[wi] = 0;
-  }
-  StatsPart p{};
-  p.min_val = 0xFFFFFFFFu;
-  if (__ballot(word != 0)) {  // wave-uniform
-    uint32_t c_zero = 0, c_near = 0, mn = 0xFFFFFFFFu;
-    uint64_t m = 0;
-    uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint64_t t = word; t; t &= t - 1) {
-      const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
-      const uint64_t deg = g.off[v + 1] - g.off[v];
-      m += deg;
-      h[hist_bin(deg)]++;
-      c_zero += deg == 0;
-      if (val) {
-        const uint32_t x = val[v];
-        mn = x < mn ? x : mn;
-        c_near += x < threshold;
-      }
-    }
-    p.n_active = wave_sum_u32((uint32_t)__popcll(word));
-    p.m_active = wave_sum_u64(m);
-    p.n_zero = wave_sum_u32(c_zero);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h[k]);
-    if (val) {
-      p.n_near = wave_sum_u32(c_near);
-#pragma unroll
-      for (int d = kWave / 2; d >= 1; d >>= 1) {
-        const uint32_t o = __shfl_xor(mn, d, kWave);
-        mn = o < mn ? o : mn;
-      }
-      p.min_val = mn;
-    }
-  }
   if (lane_id() == 0) wpart[wave_id()] = p;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -250,16 +211,25 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
     is_last = ticket == gridDim.x - 1;
   }
   __syncthreads();
-  if (!is_last) return;  // block-uniform
+  return is_last;
+}
+
+// Last workgroup: fixed-order reduction of the partials, first-visit stripes,
+// run state, features and (with weights) the scheduler network, then the
+// counters of the next compaction and the publication to mapped host memory
+// (`iter` last, so the host can poll it).
+__device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t nparts, uint32_t* __restrict__ stripes,
+                             RunState* __restrict__ rs, uint32_t V, uint64_t E, const float* __restrict__ mlp,
+                             DevStats* __restrict__ st, DevStats* __restrict__ other,
+                             DevStats* __restrict__ host_out) {
   __threadfence();
-  // ---- last workgroup: fixed-order reduction of the partials ----
   __shared__ unsigned long long s_m[kBlock];
   __shared__ uint32_t s_c[kBlock][11];
   __shared__ uint32_t s_min[kBlock];
   unsigned long long m = 0;
   uint32_t c[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t mn = 0xFFFFFFFFu;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+  for (uint32_t b = threadIdx.x; b < nparts; b += kBlock) {
     const StatsPart q = parts[b];
     m += q.m_active;
     c[0] += q.n_active;
@@ -285,51 +255,107 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
   __shared__ float mlp_sh[2 * kMlpH + kMlpOut];
   __shared__ float probs[kMlpOut];
   __shared__ int best;
+  // thread 0 builds the record in registers and stores it once: no read-back
+  // of global memory on the serial path
+  DevStats d{};
   if (threadIdx.x == 0) {
     uint32_t nn = 0;
-    for (int k = 0; k < kStripes; ++k) {
-      nn += stripes[k * kStripeStride];
-      stripes[k * kStripeStride] = 0;
-    }
-    rs->visited += nn;
-    rs->iter += 1;
-    st->n_active = s_c[0][0];
-    st->n_zero_deg = s_c[0][1];
-    st->m_active = s_m[0];
-    for (int k = 0; k < 8; ++k) st->hist[k] = s_c[0][2 + k];
-    st->n_new = nn;
-    st->n_near = s_c[0][10];
-    st->min_val = s_min[0];
-    st->visited = rs->visited;
-    st->iter = rs->iter;
-    st->choice = -1;
-    DevStats tmp = *st;
-    make_features(tmp, g.V, E, rs->visited, rs->iter, feat);
+#pragma unroll
+    for (int k = 0; k < kStripes; ++k) nn += stripes[k * kStripeStride];
+#pragma unroll
+    for (int k = 0; k < kStripes; ++k) stripes[k * kStripeStride] = 0;
+    const unsigned long long visited = rs->visited + nn;
+    const uint32_t iter = rs->iter + 1;
+    rs->visited = visited;
+    rs->iter = iter;
+    d.n_active = s_c[0][0];
+    d.n_zero_deg = s_c[0][1];
+    d.m_active = s_m[0];
+    for (int k = 0; k < 8; ++k) d.hist[k] = s_c[0][2 + k];
+    d.n_new = nn;
+    d.n_near = s_c[0][10];
+    d.min_val = s_min[0];
+    d.visited = visited;
+    d.iter = iter;
+    d.choice = -1;
+    // this iteration's compaction counts from zero (n_small / n_med / n_chunk
+    // stay 0); every workgroup has taken its ticket, and the other slot's
+    // ticket is cleared for the next iteration's statistics
+    *st = d;
+    if (other) other->ticket = 0;
+    make_features(d, V, E, visited, iter, feat);
   }
   __syncthreads();
   if (mlp) {
     mlp_forward_block(mlp, feat, mlp_sh, probs, &best);
     if (threadIdx.x < kMlpOut) st->probs[threadIdx.x] = probs[threadIdx.x];
     if (threadIdx.x == 0) st->choice = best;
+    __syncthreads();
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // this iteration's compaction counts from zero; the other slot's ticket
-    // is ready for the next iteration's statistics
-    st->n_small = 0;
-    st->n_med = 0;
-    st->n_chunk = 0;
-    if (other) other->ticket = 0;
-    if (host_out) {
-      // publish to mapped host memory; `iter` goes last so the host can poll it
-      const uint32_t itv = st->iter;
-      DevStats tmp = *st;
-      tmp.iter = 0xFFFFFFFFu;
-      *host_out = tmp;
-      __threadfence_system();
-      __hip_atomic_store(&host_out->iter, itv, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (threadIdx.x == 0 && host_out) {
+    if (mlp) {
+      for (int k = 0; k < kMlpOut; ++k) d.probs[k] = probs[k];
+      d.choice = best;
+    }
+    const uint32_t itv = d.iter;
+    d.iter = 0xFFFFFFFFu;
+    *host_out = d;
+    // the release store orders the record before `iter` at system scope
+    __hip_atomic_store(&host_out->iter, itv, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ in,
+                                                  uint64_t* __restrict__ zero_me, uint64_t words,
+                                                  DevGraph g, uint64_t E, StatsPart* __restrict__ parts,
+                                                  uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
+                                                  const float* __restrict__ mlp,
+                                                  const uint32_t* __restrict__ val, uint32_t threshold,
+                                                  DevStats* __restrict__ st, DevStats* __restrict__ other,
+                                                  DevStats* __restrict__ host_out) {
+  StatsPart p{};
+  p.min_val = 0xFFFFFFFFu;
+  uint32_t c_act = 0, c_zero = 0, c_near = 0, mn = 0xFFFFFFFFu;
+  uint64_t m = 0;
+  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // a capped grid strides over the words (fewer workgroup partials and tickets)
+  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < words; base += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t wi = base + threadIdx.x;
+    uint64_t word = 0;
+    if (wi < words) {
+      word = in[wi];
+      if (zero_me) zero_me[wi] = 0;
+    }
+    c_act += (uint32_t)__popcll(word);
+    for (uint64_t t = word; t; t &= t - 1) {
+      const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
+      const uint64_t deg = g.off[v + 1] - g.off[v];
+      m += deg;
+      h[hist_bin(deg)]++;
+      c_zero += deg == 0;
+      if (val) {
+        const uint32_t x = val[v];
+        mn = x < mn ? x : mn;
+        c_near += x < threshold;
+      }
     }
   }
+  p.n_active = wave_sum_u32(c_act);
+  p.m_active = wave_sum_u64(m);
+  p.n_zero = wave_sum_u32(c_zero);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h[k]);
+  if (val) {
+    p.n_near = wave_sum_u32(c_near);
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) {
+      const uint32_t o = __shfl_xor(mn, d, kWave);
+      mn = o < mn ? o : mn;
+    }
+    p.min_val = mn;
+  }
+  if (!stats_block_done(p, parts, st)) return;  // workgroup-uniform
+  stats_finish(parts, gridDim.x, stripes, rs, g.V, E, mlp, st, other, host_out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
@@ -755,7 +781,8 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
                  StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* other, DevStats* host_out,
                  hipStream_t s) {
-  const uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
+  uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
+  if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, E, parts,
                      stripes, rs, mlp, val, threshold, st, other, host_out);
   CG_LAUNCH_CHECK();
@@ -784,7 +811,8 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        uint32_t* stripes, bool exact, hipStream_t s) {
   if (!item_bound) return CG_OK;
   uint64_t blocks = (item_bound + kWavesPerBlock - 1) / kWavesPerBlock;
-  const uint64_t cap = bins.grid_cap ? bins.grid_cap : 2048;  // 256 CUs x 32 wavefronts
+  uint64_t cap = bins.grid_cap ? bins.grid_cap : 2048;  // 256 CUs x 32 wavefronts
+  if (cap > (1u << 20)) cap = 1u << 20;                 // the wavefronts stride; more adds nothing
   if (blocks > cap) blocks = cap;
   dim3 grid((unsigned)blocks), block(kBlock);
 #define CG_EXPAND(A, X)                                                                          \

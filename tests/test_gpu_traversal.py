@@ -384,3 +384,23 @@ def test_expand_grid_caps(oracle, cap):
         assert np.array_equal(got, ref), (alg, cap)
         got, _ = run(g, alg, root, expand_grid=cap)
         assert np.array_equal(got, ref), (alg, cap)
+
+
+def test_gpu_schedule_pipelines(oracle):
+    """The GPU schedule's queued-ahead pipeline (statistics, compaction and
+    expansion issued without waiting for the host), with and without
+    delta-stepping."""
+    og = random_graph(oracle, 21, 30000, 200000, hub=30000)
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    for alg, ref in ((BFS, oracle.bfs(og, root)), (SSSP, oracle.sssp(og, root)), (CC, oracle.cc_directed(og))):
+        got, _ = run(g, alg, root)
+        assert np.array_equal(got, ref), alg
+    for delta in (1, 40, 5000):
+        got, _ = run(g, SSSP, root, sssp_delta=delta)
+        assert np.array_equal(got, oracle.sssp(og, root)), delta
+    lat = oracle.grid_graph(70, 0.1, seed=9)
+    gl = gpu_graph(lat)
+    for alg in (BFS, SSSP):
+        got, _ = run(gl, alg, 3, sssp_delta=200)
+        assert np.array_equal(got, oracle.bfs(lat, 3) if alg == BFS else oracle.sssp(lat, 3))
