@@ -153,7 +153,7 @@ bool probs_usable(const DevStats& s) {
     sum += s.probs[k];
   }
   // the network defers to the heuristic when it is not confident
-  return std::fabs(sum - 1.f) < 1e-3f && mx >= 0.35f;
+  return std::fabs(sum - 1.f) < 1e-3f && mx >= kLearnedMinProb;
 }
 
 // Waits for k_stats of iteration `it` to publish into mapped host memory:
@@ -384,7 +384,16 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // and nothing waits for the statistics, so compaction and expansion are
   // queued right behind them and the host reads the statistics while the GPU
   // goes on (BFS decides push or pull from the statistics first).
-  const bool early = mode == CG_SCHED_GPU && !host_possible && alg != CG_BFS && !min_pull;
+  // The learned schedule joins when its network runs on the device: a GPU
+  // push choice is applied there (tiers from the choice), a host choice leaves
+  // the queues empty and the host team runs the iteration afterwards.
+  const bool early_learned = learned && mlp_dev && alg != CG_BFS && !min_pull;
+  const bool early = (mode == CG_SCHED_GPU && !host_possible && alg != CG_BFS && !min_pull) || early_learned;
+  TierTable choice_tiers{};
+  for (int k = 0; k < 4; ++k) {
+    choice_tiers.wave[k] = k ? kTierSets[k].bin_wave : e->opt.bin_wave;
+    choice_tiers.chunk[k] = k ? kTierSets[k].bin_chunk : e->opt.bin_chunk;
+  }
   const Bins fbins{e->opt.bin_wave, e->opt.bin_chunk, e->opt.chunk_edges, e->opt.expand_grid};
   bool early_timed[2] = {false, false};
   auto take_timing = [&](int k) -> int {
@@ -406,7 +415,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
                         e->hst_map_dev + slot, s));
     if (early) {
       CG_TRY(launch_compact(e->bm[cur].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, tval,
-                            threshold, e->bm[cur ^ 1].p, s));
+                            threshold, e->bm[cur ^ 1].p, s, early_learned ? &choice_tiers : nullptr));
       if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[slot][0], s));
       CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[cur ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
                                 ~0ull >> 8, 0xFFFFFFFFu, fbins, dst, e->stripes.p, e->opt.atomic_return != 0, s));
@@ -484,7 +493,11 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       CG_TRY(probe_all(e, dg, sst, dst, in, out, threshold, us));
       write_probe(e, rec, root, us);
     }
-    if (early) {
+    if (early && step.host_share) {
+      // the network gave this iteration to the host team; the queued GPU
+      // expansion found empty queues
+      CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));
+    } else if (early) {
       // already queued behind the statistics
       ++local.heavy_iterations;
       local.gpu_vertices += sst.n_active;

@@ -93,9 +93,17 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
 // Frontier vertices whose value is >= threshold are not queued but carried
 // into `carry` (the next frontier) unchanged.
+// choice_tiers not NULL: the tiers come from the scheduler network's choice
+// that k_stats left in st->choice (tier set k of the table; the table's entry
+// 0 is also used when the network deferred, choice < 0), and a choice that
+// hands the iteration to the host team compacts nothing.
+struct TierTable {
+  uint32_t wave[4], chunk[4];
+};
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                   const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s);
+                   const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s,
+                   const TierTable* choice_tiers = nullptr);
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);

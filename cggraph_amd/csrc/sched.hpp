@@ -24,6 +24,8 @@ struct TierSet {
 // bin_wave: degree from which a vertex gets a whole wavefront;
 // bin_chunk: degree from which a vertex is cut into chunk_edges pieces.
 constexpr TierSet kTierSets[4] = {{32, 4096}, {16, 1024}, {64, 16384}, {8, 256}};
+// The network's choice is taken when its top probability reaches this.
+constexpr float kLearnedMinProb = 0.35f;
 
 struct Choice {
   int pull;

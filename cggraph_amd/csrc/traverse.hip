@@ -289,7 +289,19 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
   if (mlp) {
     mlp_forward_block(mlp, feat, mlp_sh, probs, &best);
     if (threadIdx.x < kMlpOut) st->probs[threadIdx.x] = probs[threadIdx.x];
-    if (threadIdx.x == 0) st->choice = best;
+    if (threadIdx.x == 0) {
+      // the network defers (choice -1) unless its output is a finite
+      // distribution whose top choice has probability >= kLearnedMinProb
+      float sum = 0.f;
+      bool finite = true;
+      for (int k = 0; k < kMlpOut; ++k) {
+        finite = finite && isfinite(probs[k]);
+        sum += probs[k];
+      }
+      const bool usable = finite && fabsf(sum - 1.f) < 1e-3f && probs[best] >= kLearnedMinProb;
+      best = usable ? best : -1;
+      st->choice = best;
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0 && host_out) {
@@ -365,9 +377,18 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
                                                     uint64_t* __restrict__ q_chunk,
                                                     DevStats* __restrict__ st,
                                                     const uint32_t* __restrict__ val, uint32_t threshold,
-                                                    uint64_t* __restrict__ carry) {
+                                                    uint64_t* __restrict__ carry, int use_choice,
+                                                    TierTable tiers) {
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
+  if (use_choice) {
+    // the network's choice, left by k_stats (-1: it deferred to the GPU rule)
+    const int c = st->choice;
+    if (c >= 0 && (c & 8)) return;  // the host team takes this iteration (uniform)
+    const int tier = c > 0 ? (c >> 1) & 3 : 0;
+    bins.bin_wave = tiers.wave[tier];
+    bins.bin_chunk = tiers.chunk[tier];
+  }
   const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   uint64_t word = wi < words ? in[wi] : 0;
   if (val && word) {
@@ -791,11 +812,13 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
 
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                   const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s) {
+                   const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s,
+                   const TierTable* choice_tiers) {
   if (!words) return CG_OK;
   const uint64_t blocks = (words + kBlock - 1) / kBlock;
+  const TierTable tiers = choice_tiers ? *choice_tiers : TierTable{};
   hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, words, g, bins,
-                     q_small, q_med, q_chunk, st, val, threshold, carry);
+                     q_small, q_med, q_chunk, st, val, threshold, carry, choice_tiers ? 1 : 0, tiers);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
