@@ -388,14 +388,24 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // push choice is applied there (tiers from the choice), a host choice leaves
   // the queues empty and the host team runs the iteration afterwards.
   const bool early_learned = learned && mlp_dev && alg != CG_BFS && !min_pull;
-  const bool early = (mode == CG_SCHED_GPU && !host_possible && alg != CG_BFS && !min_pull) || early_learned;
+  // BFS under the GPU schedule: the direction rule runs on the device too, and
+  // both the push pair and the bottom-up kernel are queued; each runs only
+  // when the rule picked it
+  const bool early_bfs = mode == CG_SCHED_GPU && !host_possible && alg == CG_BFS && g.dev_t_valid;
+  const bool early = (mode == CG_SCHED_GPU && !host_possible && alg != CG_BFS && !min_pull) || early_learned ||
+                     early_bfs;
   TierTable choice_tiers{};
   for (int k = 0; k < 4; ++k) {
     choice_tiers.wave[k] = k ? kTierSets[k].bin_wave : e->opt.bin_wave;
     choice_tiers.chunk[k] = k ? kTierSets[k].bin_chunk : e->opt.bin_chunk;
   }
   const Bins fbins{e->opt.bin_wave, e->opt.bin_chunk, e->opt.chunk_edges, e->opt.expand_grid};
-  bool early_timed[2] = {false, false};
+  // Pure GPU schedule: iteration it + 1 is queued before the host reads the
+  // statistics of iteration it, so the host's launch work overlaps the GPU.
+  // The iteration after the last one finds an empty frontier and does nothing;
+  // a delta-stepping threshold moves one iteration later.
+  const bool run_ahead = early && !early_learned;
+  bool early_timed[4] = {false, false, false, false};
   auto take_timing = [&](int k) -> int {
     if (!early_timed[k]) return CG_OK;
     float ms = 0.f;
@@ -405,31 +415,41 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     early_timed[k] = false;
     return CG_OK;
   };
+  // iteration k reads bm[(k - 1) & 1], writes bm[k & 1], uses stats slot k & 1
+  auto enqueue = [&](uint32_t k) -> int {
+    const int sl = k & 1, c = (k - 1) & 1;
+    DevStats* dk = e->st.p + sl;
+    CG_TRY(launch_stats(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p, e->run_state.p,
+                        mlp_dev, tval, threshold, dk, e->st.p + (sl ^ 1), e->hst_map_dev + sl, s,
+                        early_bfs ? 1 : 0));
+    if (!early) return CG_OK;
+    CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, tval,
+                          threshold, e->bm[c ^ 1].p, s, (early_learned || early_bfs) ? &choice_tiers : nullptr,
+                          early_bfs ? 1u : 8u));
+    const int tp = k & 3;
+    if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
+    CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
+                              ~0ull >> 8, 0xFFFFFFFFu, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s));
+    if (early_bfs) CG_TRY(launch_pull_bfs(dg, e->val.p, e->bm[c].p, e->bm[c ^ 1].p, e->stripes.p, s, dk));
+    if (e->opt.profile) {
+      CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));
+      early_timed[tp] = true;
+    }
+    ++local.expand_launches;
+    return CG_OK;
+  };
+  if (run_ahead) CG_TRY(enqueue(1));
   auto tit = clk::now();
   for (;;) {
     ++it;
     const int slot = it & 1;
     DevStats* dst = e->st.p + slot;
-    CG_TRY(launch_stats(e->bm[cur].p, e->bm[cur ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p,
-                        e->run_state.p, mlp_dev, tval, threshold, dst, e->st.p + (slot ^ 1),
-                        e->hst_map_dev + slot, s));
-    if (early) {
-      CG_TRY(launch_compact(e->bm[cur].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, tval,
-                            threshold, e->bm[cur ^ 1].p, s, early_learned ? &choice_tiers : nullptr));
-      if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[slot][0], s));
-      CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[cur ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
-                                ~0ull >> 8, 0xFFFFFFFFu, fbins, dst, e->stripes.p, e->opt.atomic_return != 0, s));
-      if (e->opt.profile) {
-        CG_HIP(hipEventRecord(e->ev_pair[slot][1], s));
-        early_timed[slot] = true;
-      }
-      ++local.expand_launches;
-    }
+    CG_TRY(enqueue(run_ahead ? it + 1 : it));
     CG_TRY(wait_stats(e, slot, it));
     if (e->opt.fault_flags & CG_FAULT_HIP_ERROR)
       return set_error(CG_ERR_HIP, "injected HIP failure at iteration %u", it);
     const DevStats sst = e->hst_last;
-    if (early) CG_TRY(take_timing(slot ^ 1));  // the previous expansion has finished
+    if (early) CG_TRY(take_timing((it - 1) & 3));  // the previous expansion has finished
     if (it > 1) {
       if (timing_pending) {
         float ms = 0.f;
@@ -459,7 +479,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       step.ch = decode_choice(e->opt.forced_choice);
       step.host_share = step.ch.host ? 1 : 0;
     } else {
-      if (alg == CG_BFS) pull = beamer_pull(pull, sst, m_unexplored, V, prev_active);
+      if (early_bfs) pull = sst.choice == 1;  // decided on the device
+      else if (alg == CG_BFS) pull = beamer_pull(pull, sst, m_unexplored, V, prev_active);
       step.ch.pull = pull;
       if (learned) {
         if (mlp_dev && probs_usable(sst)) {
@@ -497,6 +518,13 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       // the network gave this iteration to the host team; the queued GPU
       // expansion found empty queues
       CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));
+    } else if (early && step.ch.pull) {
+      // the queued bottom-up kernel ran (same accounting as exec_step's)
+      ++local.pull_iterations;
+      ++local.heavy_iterations;
+      local.gpu_vertices += sst.n_active;
+      local.edges_relaxed += sst.m_active;
+      local.expand_alg_bytes += 8.0 * ((double)V + 1) + 4.0 * (double)V + 16.0 * (double)words;
     } else if (early) {
       // already queued behind the statistics
       ++local.heavy_iterations;
@@ -510,10 +538,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     cur ^= 1;
   }
   CG_HIP(hipStreamSynchronize(s));
-  if (early) {
-    CG_TRY(take_timing(0));
-    CG_TRY(take_timing(1));
-  }
+  if (early)
+    for (int k = 0; k < 4; ++k) CG_TRY(take_timing(k));
   local.iterations = it - 1;
   local.ms = ms_since(t0);
   e->cur = cur;

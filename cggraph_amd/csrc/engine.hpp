@@ -19,7 +19,7 @@ struct cg_engine {
   cg_options opt{};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
-  hipEvent_t ev_pair[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // expansions queued ahead, by slot
+  hipEvent_t ev_pair[4][2] = {};  // expansions queued ahead, by iteration mod 4
 
   // values (uint32 for BFS/SSSP/CC, float for PR) and frontier state
   cg::DevBuf<uint32_t> val;

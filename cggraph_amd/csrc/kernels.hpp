@@ -37,6 +37,9 @@ constexpr int kStripeStride = 32;  // uint32 per stripe = 128 bytes
 struct RunState {
   unsigned long long visited;
   uint32_t iter;
+  uint32_t pull;                   // device-side direction rule: last step was bottom-up
+  unsigned long long m_explored;   // frontier edges of the iterations so far
+  uint32_t prev_active;            // frontier size of the previous iteration
   uint32_t pad;
 };
 
@@ -86,24 +89,28 @@ uint64_t stats_parts_needed(uint64_t words);
 // The last workgroup also zeroes st's queue counters and other->ticket, and
 // publishes the stats to `host_out` (mapped host memory, may be NULL) with
 // `iter` written last.
+// direction != 0: the last workgroup also takes the direction-optimising BFS
+// decision on the device (the rule of the host's beamer_pull, state in rs) and
+// leaves it in st->choice (1 bottom-up, 0 push), for the kernels queued behind.
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
                  uint64_t E, StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* other,
-                 DevStats* host_out, hipStream_t s);
+                 DevStats* host_out, hipStream_t s, int direction = 0);
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
 // Frontier vertices whose value is >= threshold are not queued but carried
 // into `carry` (the next frontier) unchanged.
-// choice_tiers not NULL: the tiers come from the scheduler network's choice
-// that k_stats left in st->choice (tier set k of the table; the table's entry
-// 0 is also used when the network deferred, choice < 0), and a choice that
-// hands the iteration to the host team compacts nothing.
+// choice_tiers not NULL: the tiers come from the choice k_stats left in
+// st->choice (tier set k of the table; the table's entry 0 is also used when
+// the network deferred, choice < 0), and a choice with any of `skip_bits` set
+// (8: the host team takes the iteration; 1: it goes bottom-up) compacts
+// nothing.
 struct TierTable {
   uint32_t wave[4], chunk[4];
 };
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
                    const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s,
-                   const TierTable* choice_tiers = nullptr);
+                   const TierTable* choice_tiers = nullptr, uint32_t skip_bits = 8);
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);
@@ -115,8 +122,10 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
                        uint32_t* stripes, bool exact, hipStream_t s);
 // Bottom-up BFS step: unreached vertices look for a parent in `in`.
+// cond not NULL: runs only when cond->choice (left by k_stats) asks for a
+// bottom-up step; the grid is capped and wavefronts stride over the words.
 int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
-                    uint32_t* stripes, hipStream_t s);
+                    uint32_t* stripes, hipStream_t s, const DevStats* cond = nullptr);
 // Bottom-up SSSP / label propagation: every vertex takes the minimum over its
 // in-edges from frontier vertices.  The in-edges are cut into tiles of
 // pull_tile_edges(); tile_row (pull_tile_count() + 1 entries, from

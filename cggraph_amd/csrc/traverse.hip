@@ -179,6 +179,9 @@ __global__ void k_init_run_state(RunState* rs, unsigned long long visited) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     rs->visited = visited;
     rs->iter = 0;
+    rs->pull = 0;
+    rs->m_explored = 0;
+    rs->prev_active = 0;
     rs->pad = 0;
   }
 }
@@ -221,7 +224,7 @@ __device__ __forceinline__ bool stats_block_done(const StatsPart& p, StatsPart* 
 __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t nparts, uint32_t* __restrict__ stripes,
                              RunState* __restrict__ rs, uint32_t V, uint64_t E, const float* __restrict__ mlp,
                              DevStats* __restrict__ st, DevStats* __restrict__ other,
-                             DevStats* __restrict__ host_out) {
+                             DevStats* __restrict__ host_out, int direction) {
   __threadfence();
   __shared__ unsigned long long s_m[kBlock];
   __shared__ uint32_t s_c[kBlock][11];
@@ -278,6 +281,19 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
     d.visited = visited;
     d.iter = iter;
     d.choice = -1;
+    if (direction) {
+      // Beamer, Asanovic, Patterson (SC'12): bottom-up when the frontier's
+      // edges exceed the unexplored edges / 15, back to push when the frontier
+      // falls below V / 18 and shrinks
+      const unsigned long long m_unexplored = E > rs->m_explored ? E - rs->m_explored : 0ull;
+      bool pull;
+      if (!rs->pull) pull = 15ull * d.m_active > m_unexplored;
+      else pull = !(18ull * d.n_active < (unsigned long long)V && d.n_active < rs->prev_active);
+      rs->pull = pull ? 1u : 0u;
+      rs->m_explored += d.m_active;
+      rs->prev_active = d.n_active;
+      d.choice = pull ? 1 : 0;
+    }
     // this iteration's compaction counts from zero (n_small / n_med / n_chunk
     // stay 0); every workgroup has taken its ticket, and the other slot's
     // ticket is cleared for the next iteration's statistics
@@ -324,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
                                                   const float* __restrict__ mlp,
                                                   const uint32_t* __restrict__ val, uint32_t threshold,
                                                   DevStats* __restrict__ st, DevStats* __restrict__ other,
-                                                  DevStats* __restrict__ host_out) {
+                                                  DevStats* __restrict__ host_out, int direction) {
   StatsPart p{};
   p.min_val = 0xFFFFFFFFu;
   uint32_t c_act = 0, c_zero = 0, c_near = 0, mn = 0xFFFFFFFFu;
@@ -367,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
     p.min_val = mn;
   }
   if (!stats_block_done(p, parts, st)) return;  // workgroup-uniform
-  stats_finish(parts, gridDim.x, stripes, rs, g.V, E, mlp, st, other, host_out);
+  stats_finish(parts, gridDim.x, stripes, rs, g.V, E, mlp, st, other, host_out, direction);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
@@ -378,13 +394,13 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
                                                     DevStats* __restrict__ st,
                                                     const uint32_t* __restrict__ val, uint32_t threshold,
                                                     uint64_t* __restrict__ carry, int use_choice,
-                                                    TierTable tiers) {
+                                                    TierTable tiers, uint32_t skip_bits) {
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   if (use_choice) {
     // the network's choice, left by k_stats (-1: it deferred to the GPU rule)
     const int c = st->choice;
-    if (c >= 0 && (c & 8)) return;  // the host team takes this iteration (uniform)
+    if (c >= 0 && ((uint32_t)c & skip_bits)) return;  // host team or bottom-up step (uniform)
     const int tier = c > 0 ? (c >> 1) & 3 : 0;
     bins.bin_wave = tiers.wave[tier];
     bins.bin_chunk = tiers.chunk[tier];
@@ -529,26 +545,34 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
 __global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __restrict__ val,
                                                      const uint64_t* __restrict__ in,
                                                      uint64_t* __restrict__ out,
-                                                     uint32_t* __restrict__ stripes) {
-  const uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
-  bool found = false;
-  if (v < g.V && val[v] == kUnreached) {
-    const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
-    for (uint64_t k = s; k < e; ++k) {
-      const uint32_t u = g.in_src[k];
-      if ((in[u >> 6] >> (u & 63)) & 1ull) {
-        val[v] = val[u] + 1;
-        found = true;
-        break;
+                                                     uint32_t* __restrict__ stripes,
+                                                     const DevStats* __restrict__ cond) {
+  if (cond) {
+    const int c = cond->choice;
+    if (!(c >= 0 && (c & 1))) return;  // this iteration pushes (uniform)
+  }
+  const uint64_t words = ((uint64_t)g.V + 63) / 64;
+  const uint64_t nwaves = (uint64_t)gridDim.x * kWavesPerBlock;
+  uint32_t n_new = 0;
+  for (uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); w < words; w += nwaves) {
+    const uint64_t v = w * 64 + lane_id();
+    bool found = false;
+    if (v < g.V && val[v] == kUnreached) {
+      const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
+      for (uint64_t k = s; k < e; ++k) {
+        const uint32_t u = g.in_src[k];
+        if ((in[u >> 6] >> (u & 63)) & 1ull) {
+          val[v] = val[u] + 1;
+          found = true;
+          break;
+        }
       }
     }
+    const uint64_t mask = __ballot(found);
+    if (lane_id() == 0) out[w] = mask;
+    n_new += (uint32_t)__popcll(mask);
   }
-  const uint64_t mask = __ballot(found);
-  const uint64_t words = ((uint64_t)g.V + 63) / 64;
-  if (lane_id() == 0) {
-    if ((v >> 6) < words) out[v >> 6] = mask;
-    stripe_add(stripes, (uint32_t)__popcll(mask));
-  }
+  if (lane_id() == 0) stripe_add(stripes, n_new);
 }
 
 // Bottom-up SSSP and label propagation: thread j of a workgroup takes kPullK
@@ -801,11 +825,11 @@ uint64_t stats_parts_needed(uint64_t words) { return (words + kBlock - 1) / kBlo
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
                  StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* other, DevStats* host_out,
-                 hipStream_t s) {
+                 hipStream_t s, int direction) {
   uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, E, parts,
-                     stripes, rs, mlp, val, threshold, st, other, host_out);
+                     stripes, rs, mlp, val, threshold, st, other, host_out, direction);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
@@ -813,12 +837,12 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
                    const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s,
-                   const TierTable* choice_tiers) {
+                   const TierTable* choice_tiers, uint32_t skip_bits) {
   if (!words) return CG_OK;
   const uint64_t blocks = (words + kBlock - 1) / kBlock;
   const TierTable tiers = choice_tiers ? *choice_tiers : TierTable{};
   hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, words, g, bins,
-                     q_small, q_med, q_chunk, st, val, threshold, carry, choice_tiers ? 1 : 0, tiers);
+                     q_small, q_med, q_chunk, st, val, threshold, carry, choice_tiers ? 1 : 0, tiers, skip_bits);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
@@ -857,11 +881,13 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
 }
 
 int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64_t* out,
-                    uint32_t* stripes, hipStream_t s) {
+                    uint32_t* stripes, hipStream_t s, const DevStats* cond) {
   if (!g.in_off || !g.in_src) return set_error(CG_ERR_INTERNAL, "pull step without in-edges");
-  const uint64_t threads = ((uint64_t)g.V + 63) / 64 * 64;
-  const uint64_t blocks = (threads + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_pull_bfs, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, stripes);
+  const uint64_t words = ((uint64_t)g.V + 63) / 64;
+  uint64_t blocks = (words + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (blocks > 4096) blocks = 4096;  // wavefronts stride over the words
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_pull_bfs, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, stripes, cond);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
