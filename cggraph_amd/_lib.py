@@ -56,6 +56,7 @@ class Options(ctypes.Structure):
         ("pr_kernel", ctypes.c_uint32),
         ("min_pull", ctypes.c_uint32),
         ("expand_grid", ctypes.c_uint32),
+        ("batch_group", ctypes.c_uint32),
     ]
 
 

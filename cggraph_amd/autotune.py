@@ -33,6 +33,7 @@ BATCH_SPACE = {
     "batch_grid": [256, 512, 1024, 2048, 4096],
     "batch_block": [256, 512, 1024],
     "batch_levels": [8, 32, 128],
+    "batch_group": [16, 32, 64],
 }
 
 

@@ -186,7 +186,8 @@ __device__ __forceinline__ void wave_append(bool push, int q, uint32_t v, uint32
 __global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restrict__ qd, int nq,
                                                        const uint32_t* __restrict__ item_off,
                                                        uint32_t* __restrict__ cnt, const uint32_t* __restrict__ qs,
-                                                       uint32_t* __restrict__ qs_mut, BatchState* __restrict__ bs) {
+                                                       uint32_t* __restrict__ qs_mut, BatchState* __restrict__ bs,
+                                                       uint32_t gsz) {
   const uint32_t total = bs->total;
   if (total == 0) return;
   const uint32_t L = bs->cur_level;
@@ -196,14 +197,16 @@ __global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restri
   uint32_t* minv_nxt = qs_mut + nq + np * nq;
   const uint32_t lane = __lane_id();
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
-  // wavefronts stride over 64-item groups (no shared cursor to contend on)
-  for (uint64_t grp = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; grp * 64 < total; grp += waves) {
-    const uint32_t i = (uint32_t)(grp * 64) + lane;
+  // wavefronts stride over groups of gsz items (no shared cursor to contend
+  // on); smaller groups put more wavefronts on a level whose frontier is
+  // small, shortening the level's latency chain
+  for (uint64_t grp = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; grp * gsz < total; grp += waves) {
+    const uint32_t i = (uint32_t)(grp * gsz) + lane;
     int q = 0;
     uint32_t deg = 0, sv = 0, v = 0;
     uint64_t s = 0;
     bool carry = false;
-    if (i < total) {
+    if (lane < gsz && i < total) {
       int lo = 0, hi = nq - 1;  // largest q with item_off[q] <= i
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -293,6 +296,8 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
   const uint32_t block = opt.batch_block ? opt.batch_block : 256;
   const uint32_t grid = opt.batch_grid ? opt.batch_grid : 2048;
   const uint32_t levels = opt.batch_levels ? opt.batch_levels : 16;
+  const uint32_t gsz = opt.batch_group ? opt.batch_group : 32;
+  if (gsz > 64) return set_error(CG_ERR_INVALID, "batch_group must be in [1, 64]");
   if (block % 64 || block > 1024) return set_error(CG_ERR_INVALID, "batch_block must be a multiple of 64, <= 1024");
   for (int i = 0; i < nq; ++i) {
     if (!graphs[i]) return set_error(CG_ERR_INVALID, "query %d has no graph", i);
@@ -339,7 +344,7 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
   for (uint32_t l = 0; l < levels; ++l) {
     hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanBlock), 0, s, dq.p, cnt.p, qs.p, nq, item_off.p, bs.p);
     hipLaunchKernelGGL(k_batch_expand, dim3(grid), dim3(block), 0, s, dq.p, nq, item_off.p, cnt.p, qs.p, qs.p,
-                       bs.p);
+                       bs.p, gsz);
   }
   hipError_t ce = hipStreamEndCapture(s, &ge.graph);
   if (ce != hipSuccess) return set_error(CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));

@@ -197,6 +197,7 @@ int cg_options_default(cg_options* o) {
   o->pr_kernel = 0;
   o->min_pull = 0;
   o->expand_grid = 0;
+  o->batch_group = 0;
   return CG_OK;
 }
 

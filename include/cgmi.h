@@ -136,6 +136,8 @@ typedef struct cg_options {
   uint32_t expand_grid;           /* push expansion: most workgroups per launch, whose wavefronts
                                      stride over the work items; 0 = 2048 (256 CUs x 32
                                      wavefronts) */
+  uint32_t batch_group;           /* cg_batch_run: frontier items per wavefront (1-64, 0 = 32);
+                                     their edges are spread over the 64 lanes */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

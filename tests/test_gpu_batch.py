@@ -33,14 +33,15 @@ def test_batch_matches_oracle(oracle):
     assert st1["edges_traversed"] == st["edges_traversed"]
 
 
-@pytest.mark.parametrize("block,grid,levels,delta", [(64, 1, 1, 0), (1024, 512, 32, 0), (256, 64, 8, 1),
-                                                     (256, 2048, 16, 700), (512, 128, 4, 10**7)])
-def test_batch_launch_shapes(oracle, block, grid, levels, delta):
+@pytest.mark.parametrize("block,grid,levels,delta,group", [(64, 1, 1, 0, 64), (1024, 512, 32, 0, 1),
+                                                           (256, 64, 8, 1, 16), (256, 2048, 16, 700, 32),
+                                                           (512, 128, 4, 10**7, 7)])
+def test_batch_launch_shapes(oracle, block, grid, levels, delta, group):
     o = oracle.grid_graph(30, 0.1, seed=5)
     g = cg.Graph.from_csr(o.offsets, o.dest, o.weight)
     roots = [0, 17, 899, 450]
     outs, _ = batch.run_batch([g] * 4, [0, 1, 0, 1], roots, batch_block=block, batch_grid=grid,
-                              batch_levels=levels, sssp_delta=delta)
+                              batch_levels=levels, sssp_delta=delta, batch_group=group)
     for a, r, got in zip([0, 1, 0, 1], roots, outs):
         assert np.array_equal(got, oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r))
 
