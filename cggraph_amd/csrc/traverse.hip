@@ -186,6 +186,34 @@ __global__ void k_init_run_state(RunState* rs, unsigned long long visited) {
   }
 }
 
+// Calls f(v, out_degree) for the set bits of a lane's frontier word, in
+// vertex order.  Dense 16-vertex slices load their 17 offsets at once (all in
+// flight together) instead of one dependent pair per vertex; slices that
+// would read past off[V] take the per-vertex path.
+template <class F>
+__device__ __forceinline__ void for_each_active(uint64_t word, uint64_t wi, const uint64_t* __restrict__ off,
+                                                uint32_t V, F&& f) {
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    uint32_t bits = (uint32_t)(word >> (16 * c)) & 0xFFFFu;
+    if (!bits) continue;
+    const uint64_t base = wi * 64 + 16 * (uint64_t)c;
+    if (__popc(bits) >= 4 && base + 16 <= V) {
+      uint64_t o[17];
+#pragma unroll
+      for (int j = 0; j < 17; ++j) o[j] = off[base + j];
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if ((bits >> j) & 1u) f((uint32_t)(base + j), o[j + 1] - o[j]);
+    } else {
+      for (; bits; bits &= bits - 1) {
+        const uint32_t v = (uint32_t)(base + (uint32_t)__builtin_ctz(bits));
+        f(v, off[v + 1] - off[v]);
+      }
+    }
+  }
+}
+
 // k_stats grid cap: each workgroup leaves one partial and takes one ticket
 constexpr uint64_t kStatsMaxBlocks = 512;
 
@@ -355,9 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
       if (zero_me) zero_me[wi] = 0;
     }
     c_act += (uint32_t)__popcll(word);
-    for (uint64_t t = word; t; t &= t - 1) {
-      const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
-      const uint64_t deg = g.off[v + 1] - g.off[v];
+    for_each_active(word, wi, g.off, g.V, [&](uint32_t v, uint64_t deg) {
       m += deg;
       h[hist_bin(deg)]++;
       c_zero += deg == 0;
@@ -366,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
         mn = x < mn ? x : mn;
         c_near += x < threshold;
       }
-    }
+    });
   }
   p.n_active = wave_sum_u32(c_act);
   p.m_active = wave_sum_u64(m);
@@ -425,14 +451,12 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   uint32_t o_small = 0, o_med = 0, o_chunk = 0;
   uint32_t t_small = 0, t_med = 0, t_chunk = 0;
   if (any) {
-    for (uint64_t t = word; t; t &= t - 1) {
-      const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
-      const uint64_t deg = g.off[v + 1] - g.off[v];
-      if (deg == 0) continue;
+    for_each_active(word, wi, g.off, g.V, [&](uint32_t, uint64_t deg) {
+      if (deg == 0) return;
       if (deg < bins.bin_wave) c_small++;
       else if (deg < bins.bin_chunk) c_med++;
       else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
-    }
+    });
     o_small = wave_excl_scan(c_small, &t_small);
     o_med = wave_excl_scan(c_med, &t_med);
     o_chunk = wave_excl_scan(c_chunk, &t_chunk);
@@ -456,17 +480,15 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   uint32_t b_small = s_base[wave_id()][0] + o_small;
   uint32_t b_med = s_base[wave_id()][1] + o_med;
   uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
-  for (uint64_t t = word; t; t &= t - 1) {
-    const uint32_t v = (uint32_t)(wi * 64 + __builtin_ctzll(t));
-    const uint64_t deg = g.off[v + 1] - g.off[v];
-    if (deg == 0) continue;
+  for_each_active(word, wi, g.off, g.V, [&](uint32_t v, uint64_t deg) {
+    if (deg == 0) return;
     if (deg < bins.bin_wave) q_small[b_small++] = v;
     else if (deg < bins.bin_chunk) q_med[b_med++] = v;
     else {
       const uint32_t nch = (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
       for (uint32_t c = 0; c < nch; ++c) q_chunk[b_chunk++] = ((uint64_t)v << 32) | c;
     }
-  }
+  });
 }
 
 template <int ALG, bool EXACT>
