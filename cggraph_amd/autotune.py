@@ -25,8 +25,9 @@ ENGINE_SPACE = {
     "bin_wave": [8, 16, 32, 64, 128],
     "bin_chunk": [512, 1024, 4096, 16384],
     "chunk_edges": [512, 1024, 2048, 4096, 8192],
-    "sssp_delta": [0, 64, 256, 1024, 4096],
+    "sssp_delta": [0, 16, 32, 64, 128, 256, 1024, 4096],
     "relabel": [0, 1],
+    "atomic_return": [1, 0],
 }
 BATCH_SPACE = {
     "sssp_delta": [1000, 3000, 10000, 30000, 100000],
