@@ -250,10 +250,11 @@ def run_extra(cg, np, info24):
                         "E": inf["E"], "iterations": st["iterations"], "pull_iterations": st["pull_iterations"]}
     log(f"BFS s18: {extra['bfs_s18']}")
     g.close()
-    # BFS on scale 24 as well
+    # BFS on scale 24 as well (tuned tiers for this graph kind, when the table has them)
+    from cggraph_amd import autotune
     g = cg.Graph.rmat(24, 16, seed=1, weights=None)
     inf = g.info()
-    eng = cg.Engine(g, 0, sched_mode=1)
+    eng = cg.Engine(g, 0, sched_mode=1, **autotune.best_options(autotune.engine_signature("rmat", 0, inf["V"], inf["E"])))
     dt, st = timed_runs(eng, inf["max_outdeg_vertex"], 5, 2)
     eng.close()
     extra["bfs_s24"] = {"gteps": round(st["edges_traversed"] / dt / 1e9, 3), "ms": round(dt * 1e3, 4),
