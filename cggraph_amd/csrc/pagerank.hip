@@ -9,7 +9,8 @@
 //
 // Three layouts of the in-edges (CSC), chosen once per graph:
 //  plain       one sweep over the transposed graph per iteration, rows tiered
-//              by in-degree: one lane / one wavefront / one workgroup per row.
+//              by in-degree: one lane / one wavefront per row, and rows of
+//              8192+ in-edges cut into 8192-edge chunks of one workgroup each.
 //  relabelled  the same sweep over a copy of the CSC whose vertex ids are
 //              renumbered by descending out-degree (ties by id).  On skewed
 //              graphs a small prefix of the ids then carries most edges'
@@ -46,6 +47,12 @@ struct PrPlan {
   struct Tiers {
     uint32_t n_med = 0, n_high = 0;
     DevBuf<uint32_t> med, high;
+    // high rows cut into chunks of kHigh edges, one workgroup each; a row's
+    // chunk sums are added in chunk order
+    uint32_t n_chunks = 0;
+    DevBuf<uint64_t> chunk_lo, chunk_hi;
+    DevBuf<uint32_t> row_chunk;  // first chunk of high row i (n_high + 1 entries)
+    DevBuf<float> partial;
   } tiers[2];
   bool banded = false;
   uint32_t band = 0;
@@ -192,22 +199,47 @@ __global__ __launch_bounds__(kBlock) void k_pr_med(Csc g, const uint32_t* __rest
   if (lane == 0) emit<P>(g, v, sum, base, damping, part, rank, contrib_out);
 }
 
+// Edge ranges of the high rows in pass P (for the chunk table).
 template <int P>
-__global__ __launch_bounds__(kBlock) void k_pr_high(Csc g, const uint32_t* __restrict__ list,
-                                                    const float* __restrict__ contrib_in, float base,
-                                                    float damping, float* __restrict__ part, float* __restrict__ rank,
-                                                    float* __restrict__ contrib_out) {
+__global__ void k_high_ranges(Csc g, const uint32_t* __restrict__ list, uint32_t n, uint64_t* __restrict__ lo,
+                              uint64_t* __restrict__ hi) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t s, e;
+    row_range<P>(g, list[i], s, e);
+    lo[i] = s;
+    hi[i] = e;
+  }
+}
+
+// One workgroup per chunk of a high row: fixed lane-strided partial sums and
+// a fixed tree.
+__global__ __launch_bounds__(kBlock) void k_pr_chunk(const uint32_t* __restrict__ src,
+                                                     const uint64_t* __restrict__ chunk_lo,
+                                                     const uint64_t* __restrict__ chunk_hi,
+                                                     const float* __restrict__ contrib_in,
+                                                     float* __restrict__ partial) {
   __shared__ float acc[kBlock];
-  const uint32_t v = list[blockIdx.x];
-  uint64_t s, e;
-  row_range<P>(g, v, s, e);
-  acc[threadIdx.x] = strided_sum<kBlock>(g.src, contrib_in, s, e, threadIdx.x);
+  const uint64_t c = blockIdx.x;
+  acc[threadIdx.x] = strided_sum<kBlock>(src, contrib_in, chunk_lo[c], chunk_hi[c], threadIdx.x);
   __syncthreads();
   for (int w = kBlock / 2; w >= 1; w >>= 1) {
     if ((int)threadIdx.x < w) acc[threadIdx.x] += acc[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) emit<P>(g, v, acc[0], base, damping, part, rank, contrib_out);
+  if (threadIdx.x == 0) partial[c] = acc[0];
+}
+
+// One lane per high row: its chunk sums in chunk order.
+template <int P>
+__global__ void k_pr_chunk_finish(Csc g, const uint32_t* __restrict__ list, uint32_t n,
+                                  const uint32_t* __restrict__ row_chunk, const float* __restrict__ partial,
+                                  float base, float damping, float* __restrict__ part, float* __restrict__ rank,
+                                  float* __restrict__ contrib_out) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float sum = 0.0f;
+  for (uint32_t c = row_chunk[i]; c < row_chunk[i + 1]; ++c) sum += partial[c];
+  emit<P>(g, list[i], sum, base, damping, part, rank, contrib_out);
 }
 
 // hot[v] = sources of row v below 2^band (rows sorted ascending)
@@ -575,6 +607,39 @@ int build_tiers(const Csc& g, PrPlan::Tiers& t, hipStream_t s) {
   CG_HIP(hipStreamSynchronize(s));
   t.n_med = h[0];
   t.n_high = h[1];
+  if (!t.n_high) return CG_OK;
+  // chunk table of the high rows
+  DevBuf<uint64_t> lo, hi;
+  CG_TRY(lo.ensure(t.n_high));
+  CG_TRY(hi.ensure(t.n_high));
+  hipLaunchKernelGGL(k_high_ranges<P>, dim3(grid_of(t.n_high)), dim3(kBlock), 0, s, g, t.high.p, t.n_high, lo.p,
+                     hi.p);
+  PR_LAUNCH_OK();
+  std::vector<uint64_t> hl(t.n_high), hh(t.n_high);
+  CG_HIP(hipMemcpyAsync(hl.data(), lo.p, t.n_high * 8, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipMemcpyAsync(hh.data(), hi.p, t.n_high * 8, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  std::vector<uint64_t> clo, chi;
+  std::vector<uint32_t> first(t.n_high + 1);
+  for (uint32_t i = 0; i < t.n_high; ++i) {
+    first[i] = (uint32_t)clo.size();
+    for (uint64_t a = hl[i]; a < hh[i]; a += kHigh) {
+      clo.push_back(a);
+      chi.push_back(a + kHigh < hh[i] ? a + kHigh : hh[i]);
+    }
+  }
+  first[t.n_high] = (uint32_t)clo.size();
+  t.n_chunks = (uint32_t)clo.size();
+  CG_TRY(t.chunk_lo.ensure(clo.size() ? clo.size() : 1));
+  CG_TRY(t.chunk_hi.ensure(chi.size() ? chi.size() : 1));
+  CG_TRY(t.row_chunk.ensure(first.size()));
+  CG_TRY(t.partial.ensure(clo.size() ? clo.size() : 1));
+  if (!clo.empty()) {
+    CG_HIP(hipMemcpyAsync(t.chunk_lo.p, clo.data(), clo.size() * 8, hipMemcpyHostToDevice, s));
+    CG_HIP(hipMemcpyAsync(t.chunk_hi.p, chi.data(), chi.size() * 8, hipMemcpyHostToDevice, s));
+  }
+  CG_HIP(hipMemcpyAsync(t.row_chunk.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
+  CG_HIP(hipStreamSynchronize(s));
   return CG_OK;
 }
 
@@ -738,9 +803,15 @@ void launch_tiers(const Csc& g, const PrPlan::Tiers& t, unsigned vblocks, const 
   if (t.n_med)
     hipLaunchKernelGGL(k_pr_med<P>, dim3((t.n_med + 3) / 4), dim3(kBlock), 0, s, g, t.med.p, t.n_med, contrib_in,
                        base, damping, part, rank, contrib_out);
-  if (t.n_high)
-    hipLaunchKernelGGL(k_pr_high<P>, dim3(t.n_high), dim3(kBlock), 0, s, g, t.high.p, contrib_in, base, damping,
-                       part, rank, contrib_out);
+  if (t.n_high) {
+    // the longest rows (up to ~1.5 M in-edges at s25) would keep one workgroup
+    // busy long after the rest: their chunks spread over the chip
+    if (t.n_chunks)
+      hipLaunchKernelGGL(k_pr_chunk, dim3(t.n_chunks), dim3(kBlock), 0, s, g.src, t.chunk_lo.p, t.chunk_hi.p,
+                         contrib_in, t.partial.p);
+    hipLaunchKernelGGL(k_pr_chunk_finish<P>, dim3((t.n_high + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g,
+                       t.high.p, t.n_high, t.row_chunk.p, t.partial.p, base, damping, part, rank, contrib_out);
+  }
 }
 
 }  // namespace
