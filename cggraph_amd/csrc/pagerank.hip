@@ -83,6 +83,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kMed = 32, kHigh = 8192;
 constexpr uint32_t kLongRun = 64;
+constexpr int kMedUnroll = 8;  // gathers in flight per lane in the wavefront-per-row tier
 
 // The in-edges the pull kernels read: row v's sources are src[off[v]..off[v+1]).
 struct Csc {
@@ -164,19 +165,20 @@ __global__ __launch_bounds__(kBlock) void k_pr_low(Csc g, const float* __restric
 // Lane-strided partial sum over [s, e): the lane adds sources s+lane,
 // s+lane+stride, ... in that order; four sources are fetched ahead so four
 // gathers are in flight per lane.
-template <int STRIDE>
+template <int STRIDE, int U = 4>
 __device__ __forceinline__ float strided_sum(const uint32_t* __restrict__ src, const float* __restrict__ contrib,
                                              uint64_t s, uint64_t e, uint32_t lane) {
   float sum = 0.0f;
   uint64_t k = s + lane;
-  for (; k + 3 * STRIDE < e; k += 4 * STRIDE) {
-    const uint32_t u0 = ld_stream(src + k), u1 = ld_stream(src + k + STRIDE);
-    const uint32_t u2 = ld_stream(src + k + 2 * STRIDE), u3 = ld_stream(src + k + 3 * STRIDE);
-    const float c0 = contrib[u0], c1 = contrib[u1], c2 = contrib[u2], c3 = contrib[u3];
-    sum += c0;
-    sum += c1;
-    sum += c2;
-    sum += c3;
+  for (; k + (U - 1) * STRIDE < e; k += U * STRIDE) {
+    uint32_t u[U];
+    float c[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) u[j] = ld_stream(src + k + j * STRIDE);
+#pragma unroll
+    for (int j = 0; j < U; ++j) c[j] = contrib[u[j]];
+#pragma unroll
+    for (int j = 0; j < U; ++j) sum += c[j];
   }
   for (; k < e; k += STRIDE) sum += contrib[ld_stream(src + k)];
   return sum;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_med(Csc g, const uint32_t* __rest
   const uint32_t v = list[i];
   uint64_t s, e;
   row_range<P>(g, v, s, e);
-  float sum = strided_sum<64>(g.src, contrib_in, s, e, lane);
+  float sum = strided_sum<64, kMedUnroll>(g.src, contrib_in, s, e, lane);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
   if (lane == 0) emit<P>(g, v, sum, base, damping, part, rank, contrib_out);
