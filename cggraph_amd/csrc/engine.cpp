@@ -420,7 +420,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     const int sl = k & 1, c = (k - 1) & 1;
     DevStats* dk = e->st.p + sl;
     CG_TRY(launch_stats(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p, e->run_state.p,
-                        mlp_dev, tval, threshold, dk, e->st.p + (sl ^ 1), e->hst_map_dev + sl, s,
+                        mlp_dev, tval, threshold, dk, e->hst_map_dev + sl, s,
                         early_bfs ? 1 : 0));
     if (!early) return CG_OK;
     CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, tval,

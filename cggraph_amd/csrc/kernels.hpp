@@ -19,7 +19,7 @@ struct DevStats {
   uint32_t n_zero_deg;         // frontier vertices with out-degree 0
   unsigned long long visited;  // vertices reached so far (BFS/SSSP), V for CC
   uint32_t n_cand_applied;     // host candidates that improved a value
-  uint32_t ticket;             // k_stats workgroups finished (last one reduces)
+  uint32_t pad0;
   int32_t choice;              // learned-scheduler argmax, -1 when it did not run
   uint32_t iter;               // iteration index (1-based) seen by k_stats
   float probs[16];             // learned-scheduler softmax
@@ -76,25 +76,24 @@ int launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
 int launch_bitmap_prefix(uint64_t* bm, uint64_t words, uint64_t n, hipStream_t s);
 int launch_set_root(uint32_t* val, uint64_t* bm, uint32_t root, hipStream_t s);
 int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s);
-// Frontier statistics of `in` (n_active, m_active, hist, n_zero_deg); zeroes
-// `zero_me` (the bitmap the coming expansion writes).  Workgroup partials go
-// to `parts` (one StatsPart per workgroup, stats_parts_needed(words)); the
-// last workgroup reduces them in a fixed order, folds the striped new-vertex
-// counters of the previous expansion (`stripes`, zeroed again) into
-// rs->visited, and -- when `mlp` is not NULL -- runs the learned scheduler on
-// the iteration's features and writes choice/probs into st.
-// With `val` not NULL it also reports the frontier's smallest value and how
-// many frontier vertices lie below `threshold` (delta-stepping runs).
-uint64_t stats_parts_needed(uint64_t words);
-// The last workgroup also zeroes st's queue counters and other->ticket, and
-// publishes the stats to `host_out` (mapped host memory, may be NULL) with
-// `iter` written last.
-// direction != 0: the last workgroup also takes the direction-optimising BFS
+// Frontier statistics of `in` (n_active, m_active, hist, n_zero_deg), two
+// launches: workgroup partials go to `parts` (one StatsPart per workgroup,
+// stats_parts_needed(words)) while `zero_me` (the bitmap the coming expansion
+// writes) is cleared; then one workgroup reduces them in a fixed order, folds
+// the striped new-vertex counters of the previous expansion (`stripes`,
+// zeroed again) into rs->visited, and -- when `mlp` is not NULL -- runs the
+// learned scheduler on the iteration's features and writes choice/probs into
+// st.  With `val` not NULL it also reports the frontier's smallest value and
+// how many frontier vertices lie below `threshold` (delta-stepping runs).
+// The reduction zeroes st's queue counters and publishes the stats to
+// `host_out` (mapped host memory, may be NULL) with `iter` written last.
+// direction != 0: the reduction also takes the direction-optimising BFS
 // decision on the device (the rule of the host's beamer_pull, state in rs) and
 // leaves it in st->choice (1 bottom-up, 0 push), for the kernels queued behind.
+uint64_t stats_parts_needed(uint64_t words);
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
                  uint64_t E, StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
-                 const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* other,
+                 const uint32_t* val, uint32_t threshold, DevStats* st,
                  DevStats* host_out, hipStream_t s, int direction = 0);
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
 // Frontier vertices whose value is >= threshold are not queued but carried

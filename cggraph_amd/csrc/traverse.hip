@@ -1,12 +1,13 @@
 // Frontier kernels of the traversal engine, written for gfx950 (wave64).
 //
-// One push iteration is three launches over the frontier bitmap:
+// One push iteration is four launches over the frontier bitmap:
 //   k_stats    one lane per 64-bit frontier word: popcount, out-degree sum and
 //              log-degree histogram, reduced per workgroup into a partials
-//              array; the last workgroup (atomic ticket) reduces the partials
-//              in a fixed order and -- in learned mode -- runs the scheduler
-//              network on the iteration's features (inference fused into the
-//              statistics kernel).  It also zeroes the next bitmap.
+//              array.  It also zeroes the next bitmap.
+//   k_stats_reduce  one workgroup reduces the partials in a fixed order,
+//              publishes the record to mapped host memory and -- in learned
+//              mode -- runs the scheduler network on the iteration's features
+//              (inference fused into the statistics pass).
 //   k_compact  splits active vertices by out-degree into three work queues:
 //              small (edges spread evenly over a wavefront's lanes), medium
 //              (one wavefront per vertex) and chunked (a vertex of degree >=
@@ -214,16 +215,16 @@ __device__ __forceinline__ void for_each_active(uint64_t word, uint64_t wi, cons
   }
 }
 
-// k_stats grid cap: each workgroup leaves one partial and takes one ticket
+// k_stats grid cap: each workgroup leaves one partial for the reduction
 constexpr uint64_t kStatsMaxBlocks = 512;
 
-// Sums the wavefronts' partials of a workgroup, stores the workgroup's
-// partial and takes a ticket; returns true (workgroup-uniform) in the last
-// workgroup to finish.
-__device__ __forceinline__ bool stats_block_done(const StatsPart& p, StatsPart* __restrict__ parts,
-                                                 DevStats* __restrict__ st) {
+// Sums the wavefronts' partials of a workgroup and stores the workgroup's
+// partial.  No fence or ticket: the reduction runs in the next kernel, whose
+// launch orders it after every partial.  (An agent-scope fence per workgroup
+// writes back and invalidates the XCD's L2 -- tens of microseconds per
+// statistics pass, and a cold L2 for the expansion behind it.)
+__device__ __forceinline__ void stats_block_part(const StatsPart& p, StatsPart* __restrict__ parts) {
   __shared__ StatsPart wpart[kWavesPerBlock];
-  __shared__ bool is_last;
   if (lane_id() == 0) wpart[wave_id()] = p;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -237,23 +238,16 @@ __device__ __forceinline__ bool stats_block_done(const StatsPart& p, StatsPart* 
       b.min_val = wpart[w].min_val < b.min_val ? wpart[w].min_val : b.min_val;
     }
     parts[blockIdx.x] = b;
-    __threadfence();
-    const uint32_t ticket = atomicAdd(&st->ticket, 1u);
-    is_last = ticket == gridDim.x - 1;
   }
-  __syncthreads();
-  return is_last;
 }
 
-// Last workgroup: fixed-order reduction of the partials, first-visit stripes,
+// One workgroup: fixed-order reduction of the partials, first-visit stripes,
 // run state, features and (with weights) the scheduler network, then the
 // counters of the next compaction and the publication to mapped host memory
 // (`iter` last, so the host can poll it).
 __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t nparts, uint32_t* __restrict__ stripes,
                              RunState* __restrict__ rs, uint32_t V, uint64_t E, const float* __restrict__ mlp,
-                             DevStats* __restrict__ st, DevStats* __restrict__ other,
-                             DevStats* __restrict__ host_out, int direction) {
-  __threadfence();
+                             DevStats* __restrict__ st, DevStats* __restrict__ host_out, int direction) {
   __shared__ unsigned long long s_m[kBlock];
   __shared__ uint32_t s_c[kBlock][11];
   __shared__ uint32_t s_min[kBlock];
@@ -323,10 +317,8 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
       d.choice = pull ? 1 : 0;
     }
     // this iteration's compaction counts from zero (n_small / n_med / n_chunk
-    // stay 0); every workgroup has taken its ticket, and the other slot's
-    // ticket is cleared for the next iteration's statistics
+    // stay 0)
     *st = d;
-    if (other) other->ticket = 0;
     make_features(d, V, E, visited, iter, feat);
   }
   __syncthreads();
@@ -363,18 +355,14 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
 
 __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ in,
                                                   uint64_t* __restrict__ zero_me, uint64_t words,
-                                                  DevGraph g, uint64_t E, StatsPart* __restrict__ parts,
-                                                  uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
-                                                  const float* __restrict__ mlp,
-                                                  const uint32_t* __restrict__ val, uint32_t threshold,
-                                                  DevStats* __restrict__ st, DevStats* __restrict__ other,
-                                                  DevStats* __restrict__ host_out, int direction) {
+                                                  DevGraph g, StatsPart* __restrict__ parts,
+                                                  const uint32_t* __restrict__ val, uint32_t threshold) {
   StatsPart p{};
   p.min_val = 0xFFFFFFFFu;
   uint32_t c_act = 0, c_zero = 0, c_near = 0, mn = 0xFFFFFFFFu;
   uint64_t m = 0;
   uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // a capped grid strides over the words (fewer workgroup partials and tickets)
+  // a capped grid strides over the words (fewer workgroup partials)
   for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < words; base += (uint64_t)gridDim.x * kBlock) {
     const uint64_t wi = base + threadIdx.x;
     uint64_t word = 0;
@@ -408,8 +396,15 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
     }
     p.min_val = mn;
   }
-  if (!stats_block_done(p, parts, st)) return;  // workgroup-uniform
-  stats_finish(parts, gridDim.x, stripes, rs, g.V, E, mlp, st, other, host_out, direction);
+  stats_block_part(p, parts);
+}
+
+__global__ __launch_bounds__(kBlock) void k_stats_reduce(const StatsPart* __restrict__ parts, uint32_t nparts,
+                                                         uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
+                                                         uint32_t V, uint64_t E, const float* __restrict__ mlp,
+                                                         DevStats* __restrict__ st,
+                                                         DevStats* __restrict__ host_out, int direction) {
+  stats_finish(parts, nparts, stripes, rs, V, E, mlp, st, host_out, direction);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
@@ -846,12 +841,15 @@ uint64_t stats_parts_needed(uint64_t words) { return (words + kBlock - 1) / kBlo
 
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
                  StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
-                 const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* other, DevStats* host_out,
+                 const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* host_out,
                  hipStream_t s, int direction) {
   uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
-  hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, E, parts,
-                     stripes, rs, mlp, val, threshold, st, other, host_out, direction);
+  hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, parts, val,
+                     threshold);
+  CG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kBlock), 0, s, parts, (uint32_t)blocks, stripes, rs, g.V, E, mlp,
+                     st, host_out, direction);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
