@@ -216,7 +216,7 @@ __device__ __forceinline__ void for_each_active(uint64_t word, uint64_t wi, cons
 }
 
 // k_stats grid cap: each workgroup leaves one partial for the reduction
-constexpr uint64_t kStatsMaxBlocks = 512;
+constexpr uint64_t kStatsMaxBlocks = 2048;
 
 // Sums the wavefronts' partials of a workgroup and stores the workgroup's
 // partial.  No fence or ticket: the reduction runs in the next kernel, whose
