@@ -3,8 +3,10 @@
 A step is one SSSP query (root = max out-degree vertex, uint32 weights
 U[1,100]) over the whole graph, resident in HBM, through libcgmi's C ABI.
 GTEPS = sum of out-degrees of the reached vertices / seconds per query.
-The line also carries: the roofline of the dominant kernel (push expansion),
-the restated reference CPU engine timed on this host (cpu_baseline), and the
+The line also carries: the roofline of the dominant kernel (push expansion)
+with its practical ceiling (the measured rate of relaxations at data-dependent
+addresses, cggraph_amd/gather_ceiling), the restated reference CPU engine
+timed on this host (cpu_baseline), and the
 other headline workloads (BFS scale 18, WCC scale 24 symmetrized, PageRank
 scale 25 / edge factor 44) in "extra".
 
@@ -13,6 +15,7 @@ scale 25 / edge factor 44) in "extra".
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -25,6 +28,19 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gather_ceiling():
+    """Rates of 4-byte gathers and relaxations at data-dependent addresses on
+    this GPU (cggraph_amd/gather_ceiling, a child process): the practical
+    ceiling of the push expansion and the PageRank pull."""
+    exe = os.path.join(ROOT, "cggraph_amd", "gather_ceiling")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built; run `make -C cggraph_amd`")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"gather_ceiling failed ({r.returncode}): {r.stderr.strip()}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def timed_runs(eng, root, steps, warmup):
@@ -118,6 +134,9 @@ def main():
 
     import cggraph_amd as cg
 
+    ceiling = gather_ceiling()
+    log(f"gather ceiling: {ceiling}")
+
     t = time.perf_counter()
     g = cg.Graph.rmat(args.scale, args.edge_factor, seed=1, weights=(1, 100))
     info = g.info()
@@ -155,6 +174,16 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "avg_launch_ms": round(kms, 5),
                 "alg_bytes_per_launch": round(alg_bytes, 1), "launches": pst["expand_launches"]}
+    # every relaxed edge is one 4-byte gather plus a conditional atomicMin at a
+    # data-dependent address: the measured rate of that pattern is the ceiling
+    relax_rate = pst["edges_relaxed"] / (pst["expand_kernel_ms"] * 1e-3) / 1e9 if pst["expand_kernel_ms"] else 0.0
+    relax_ceiling = ceiling["relax_skewed_v24_G_per_s"]
+    roofline["practical_ceiling"] = {
+        "pattern": "relaxation (read, strict-less check, atomicMin) at degree-skewed addresses over 2^24 "
+                   "values, measured by cggraph_amd/gather_ceiling",
+        "unit": "G edges/s", "achieved": round(relax_rate, 2), "ceiling": relax_ceiling,
+        "frac": round(relax_rate / relax_ceiling, 4), "edges_relaxed": pst["edges_relaxed"]}
+    log(f"practical ceiling: {roofline['practical_ceiling']}")
 
     cpu = None
     parity = None
@@ -173,9 +202,9 @@ def main():
                          f"{ms:.0f} ms", "bit_exact_vs_gpu": parity}
         log(f"CPU baseline: {cpu}")
 
-    extra = {"learned_sssp_s24": learned}
+    extra = {"learned_sssp_s24": learned, "gather_ceiling": ceiling}
     if not args.no_extra:
-        extra.update(run_extra(cg, np, info))
+        extra.update(run_extra(cg, np, info, ceiling))
     g.close()
 
     line = {
@@ -238,7 +267,7 @@ def run_batched(cg, np):
     return res
 
 
-def run_extra(cg, np, info24):
+def run_extra(cg, np, info24, ceiling):
     extra = {"batched_mixed": run_batched(cg, np)}
     # BFS R-MAT scale 18 (the reference's own CPU-runnable configuration)
     g = cg.Graph.rmat(18, 16, seed=1, weights=None)
@@ -285,6 +314,11 @@ def run_extra(cg, np, info24):
     extra["pr_s25_ef44"] = {"ms_per_iteration": round(per_it, 4), "alg_bytes_per_iteration": alg,
                             "GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
                             "E": inf["E"], "iterations": st["iterations"]}
+    # one 4-byte gather per in-edge: against the measured gather rate over 2^25 values
+    gathers = inf["E"] / (per_it * 1e-3) / 1e9
+    pr_ceiling = ceiling["gather_skewed_v25_G_per_s"]
+    extra["pr_s25_ef44"].update({"gathers_G_per_s": round(gathers, 2), "gather_ceiling_G_per_s": pr_ceiling,
+                                 "frac_of_gather_ceiling": round(gathers / pr_ceiling, 4)})
     log(f"PR s25: {extra['pr_s25_ef44']}")
     g.close()
     return extra
