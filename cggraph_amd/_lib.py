@@ -57,6 +57,8 @@ class Options(ctypes.Structure):
         ("min_pull", ctypes.c_uint32),
         ("expand_grid", ctypes.c_uint32),
         ("batch_group", ctypes.c_uint32),
+        ("hub_degree", ctypes.c_uint32),
+        ("hub_delta", ctypes.c_uint32),
     ]
 
 

@@ -1,5 +1,5 @@
-"""Autotuner for the traversal engine's degree tiers, delta-stepping width and
-renumbering, and the batched kernel's launch shape.
+"""Autotuner for the traversal engine's degree tiers, delta-stepping width,
+hub deferral and renumbering, and the batched kernel's launch shape.
 
 It times real runs through the C ABI (the outputs do not depend on any of the
 tuned values, so any choice is correct) and keeps the fastest configuration
@@ -26,6 +26,8 @@ ENGINE_SPACE = {
     "bin_chunk": [512, 1024, 4096, 16384],
     "chunk_edges": [512, 1024, 2048, 4096, 8192],
     "sssp_delta": [0, 16, 32, 64, 128, 256, 1024, 4096],
+    "hub_degree": [0, 64, 128, 256, 512],
+    "hub_delta": [1, 2, 3, 5, 8],
     "relabel": [0, 1],
     "atomic_return": [1, 0],
 }
@@ -84,11 +86,14 @@ def _coordinate_descent(space, measure, start, passes=2, log=None):
 def tune_engine(graph, algorithm, root, kind="rmat", reps=3, space=None, log=None, path=TABLE, save=True):
     space = dict(space or ENGINE_SPACE)
     if int(algorithm) != 1:
-        space.pop("sssp_delta", None)
+        for k in ("sssp_delta", "hub_degree", "hub_delta"):
+            space.pop(k, None)
     o = _lib.default_options()
     start = {k: getattr(o, k) for k in space}
 
     def measure(cfg):
+        if cfg.get("sssp_delta") and cfg.get("hub_degree"):
+            return float("inf")  # alternatives, not combined
         e = Engine(graph, algorithm, sched_mode=_lib.CG_SCHED_GPU, **cfg)
         try:
             e.run(root, copy_out=False)

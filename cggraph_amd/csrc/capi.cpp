@@ -198,6 +198,8 @@ int cg_options_default(cg_options* o) {
   o->min_pull = 0;
   o->expand_grid = 0;
   o->batch_group = 0;
+  o->hub_degree = 0;
+  o->hub_delta = 0;
   return CG_OK;
 }
 
@@ -454,6 +456,8 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   for (auto& pr : e->ev_pair)
     for (auto& ev : pr) CG_HIP(hipEventCreate(&ev));
   if (e->opt.relabel > 2) return set_error(CG_ERR_INVALID, "relabel %u outside [0, 2]", e->opt.relabel);
+  if (e->opt.hub_degree && e->opt.sssp_delta)
+    return set_error(CG_ERR_INVALID, "hub_degree and sssp_delta are alternatives; set one");
   if ((algorithm == CG_BFS || algorithm == CG_SSSP) && e->opt.relabel != 1 && g->V > 0) {
     // Renumber by descending out-degree when asked, or (automatic) when the
     // top 1/64 of vertices send at least half of the edges: the values those

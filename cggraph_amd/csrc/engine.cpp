@@ -394,6 +394,12 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   const bool early_bfs = mode == CG_SCHED_GPU && !host_possible && alg == CG_BFS && g.dev_t_valid;
   const bool early = (mode == CG_SCHED_GPU && !host_possible && alg != CG_BFS && !min_pull) || early_learned ||
                      early_bfs;
+  // SSSP hub deferral (GPU-queued iterations only): the statistics read the
+  // frontier's values for their smallest one, the compaction keeps hubs above
+  // it + delta waiting
+  const bool hub_on = alg == CG_SSSP && e->opt.hub_degree != 0 && early && !delta;
+  const HubDefer hub{hub_on ? e->opt.hub_degree : 0u, e->opt.hub_delta ? e->opt.hub_delta : kHubDeltaDefault};
+  const uint32_t* sval = hub_on ? e->val.p : tval;
   TierTable choice_tiers{};
   for (int k = 0; k < 4; ++k) {
     choice_tiers.wave[k] = k ? kTierSets[k].bin_wave : e->opt.bin_wave;
@@ -420,12 +426,12 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     const int sl = k & 1, c = (k - 1) & 1;
     DevStats* dk = e->st.p + sl;
     CG_TRY(launch_stats(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p, e->run_state.p,
-                        mlp_dev, tval, threshold, dk, e->hst_map_dev + sl, s,
-                        early_bfs ? 1 : 0));
+                        mlp_dev, sval, threshold, dk, e->hst_map_dev + sl, s,
+                        early_bfs ? 1 : 0, hub_on ? e->st.p + (sl ^ 1) : nullptr));
     if (!early) return CG_OK;
-    CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, tval,
+    CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, sval,
                           threshold, e->bm[c ^ 1].p, s, (early_learned || early_bfs) ? &choice_tiers : nullptr,
-                          early_bfs ? 1u : 8u));
+                          early_bfs ? 1u : 8u, hub));
     const int tp = k & 3;
     if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
     CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
@@ -462,6 +468,12 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       if (!probe) write_trace(e, rec, root);
     }
     tit = clk::now();
+    if (hub_on) {
+      // the previous iteration's waiting hubs were counted as expanded
+      local.edges_relaxed -= sst.m_def_prev;
+      local.gpu_vertices -= sst.n_def_prev;
+      local.expand_alg_bytes -= 20.0 * sst.n_def_prev + 12.0 * (double)sst.m_def_prev;
+    }
     if (sst.n_active == 0) break;
     if (delta && sst.n_near == 0) {
       // nothing below the threshold: move it past the smallest waiting value

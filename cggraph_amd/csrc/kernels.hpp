@@ -25,6 +25,11 @@ struct DevStats {
   float probs[16];             // learned-scheduler softmax
   uint32_t min_val;            // smallest value on the frontier (threshold runs only)
   uint32_t n_near;             // frontier vertices below the threshold
+  // hub deferral: frontier vertices the compaction kept waiting (added by
+  // k_compact after the record was published), and the previous iteration's
+  // counts, published with this record so the host can correct its totals
+  uint32_t n_def, n_def_prev;
+  unsigned long long m_def, m_def_prev;
 };
 static_assert(sizeof(DevStats) % 8 == 0, "DevStats layout");
 
@@ -91,10 +96,12 @@ int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s);
 // decision on the device (the rule of the host's beamer_pull, state in rs) and
 // leaves it in st->choice (1 bottom-up, 0 push), for the kernels queued behind.
 uint64_t stats_parts_needed(uint64_t words);
+// prev not NULL: the other statistics slot, whose deferral counts (left by the
+// previous iteration's compaction) are published as n_def_prev / m_def_prev.
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
                  uint64_t E, StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st,
-                 DevStats* host_out, hipStream_t s, int direction = 0);
+                 DevStats* host_out, hipStream_t s, int direction = 0, const DevStats* prev = nullptr);
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
 // Frontier vertices whose value is >= threshold are not queued but carried
 // into `carry` (the next frontier) unchanged.
@@ -103,13 +110,22 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
 // the network deferred, choice < 0), and a choice with any of `skip_bits` set
 // (8: the host team takes the iteration; 1: it goes bottom-up) compacts
 // nothing.
+// hub.degree != 0 (SSSP): instead of `threshold`, a frontier vertex of
+// out-degree >= hub.degree waits when its value is >= st->min_val + hub.delta
+// (the frontier's smallest value, from k_stats, which then needs `val`);
+// lower-degree vertices never wait.  The waiting vertices are counted into
+// st->n_def / st->m_def.
 struct TierTable {
   uint32_t wave[4], chunk[4];
+};
+struct HubDefer {
+  uint32_t degree, delta;
 };
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
                    const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s,
-                   const TierTable* choice_tiers = nullptr, uint32_t skip_bits = 8);
+                   const TierTable* choice_tiers = nullptr, uint32_t skip_bits = 8,
+                   HubDefer hub = HubDefer{0, 0});
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);

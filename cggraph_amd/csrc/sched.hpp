@@ -26,6 +26,8 @@ struct TierSet {
 constexpr TierSet kTierSets[4] = {{32, 4096}, {16, 1024}, {64, 16384}, {8, 256}};
 // The network's choice is taken when its top probability reaches this.
 constexpr float kLearnedMinProb = 0.35f;
+// SSSP hub deferral: window above the frontier's smallest distance
+constexpr uint32_t kHubDeltaDefault = 5;
 
 struct Choice {
   int pull;

@@ -247,7 +247,8 @@ __device__ __forceinline__ void stats_block_part(const StatsPart& p, StatsPart* 
 // (`iter` last, so the host can poll it).
 __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t nparts, uint32_t* __restrict__ stripes,
                              RunState* __restrict__ rs, uint32_t V, uint64_t E, const float* __restrict__ mlp,
-                             DevStats* __restrict__ st, DevStats* __restrict__ host_out, int direction) {
+                             DevStats* __restrict__ st, DevStats* __restrict__ host_out, int direction,
+                             const DevStats* __restrict__ prev) {
   __shared__ unsigned long long s_m[kBlock];
   __shared__ uint32_t s_c[kBlock][11];
   __shared__ uint32_t s_min[kBlock];
@@ -303,6 +304,11 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
     d.visited = visited;
     d.iter = iter;
     d.choice = -1;
+    if (prev && iter > 1) {
+      // written by the previous iteration's compaction, which has finished
+      d.n_def_prev = prev->n_def;
+      d.m_def_prev = prev->m_def;
+    }
     if (direction) {
       // Beamer, Asanovic, Patterson (SC'12): bottom-up when the frontier's
       // edges exceed the unexplored edges / 15, back to push when the frontier
@@ -403,8 +409,9 @@ __global__ __launch_bounds__(kBlock) void k_stats_reduce(const StatsPart* __rest
                                                          uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
                                                          uint32_t V, uint64_t E, const float* __restrict__ mlp,
                                                          DevStats* __restrict__ st,
-                                                         DevStats* __restrict__ host_out, int direction) {
-  stats_finish(parts, nparts, stripes, rs, V, E, mlp, st, host_out, direction);
+                                                         DevStats* __restrict__ host_out, int direction,
+                                                         const DevStats* __restrict__ prev) {
+  stats_finish(parts, nparts, stripes, rs, V, E, mlp, st, host_out, direction, prev);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
@@ -415,9 +422,11 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
                                                     DevStats* __restrict__ st,
                                                     const uint32_t* __restrict__ val, uint32_t threshold,
                                                     uint64_t* __restrict__ carry, int use_choice,
-                                                    TierTable tiers, uint32_t skip_bits) {
+                                                    TierTable tiers, uint32_t skip_bits, HubDefer hub) {
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
+  __shared__ uint32_t s_ndef[kWavesPerBlock];
+  __shared__ unsigned long long s_mdef[kWavesPerBlock];
   if (use_choice) {
     // the network's choice, left by k_stats (-1: it deferred to the GPU rule)
     const int c = st->choice;
@@ -428,17 +437,56 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   }
   const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
   uint64_t word = wi < words ? in[wi] : 0;
+  uint32_t n_def = 0;
+  unsigned long long m_def = 0;
   if (val && word) {
     // delta-stepping: vertices at or above the threshold wait in the next
-    // frontier; this lane owns word wi of both bitmaps during the compaction
+    // frontier; this lane owns word wi of both bitmaps during the compaction.
+    // Hub deferral: only vertices of out-degree >= hub.degree wait, against
+    // a threshold that follows the frontier's smallest value.
+    const uint64_t t_hub = (uint64_t)st->min_val + hub.delta;
+    const uint32_t thr = hub.degree ? (t_hub > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t_hub) : threshold;
     uint64_t far = 0;
     for (uint64_t t = word; t; t &= t - 1) {
       const int b = __builtin_ctzll(t);
-      if (val[wi * 64 + b] >= threshold) far |= 1ull << b;
+      const uint64_t v = wi * 64 + b;
+      if (hub.degree) {
+        // the degree first: offsets are read for the queues anyway, values
+        // are a gather
+        const uint64_t deg = g.off[v + 1] - g.off[v];
+        if (deg < hub.degree || val[v] < thr) continue;
+        ++n_def;
+        m_def += deg;
+      } else if (val[v] < thr) {
+        continue;
+      }
+      far |= 1ull << b;
     }
     if (far) {
       carry[wi] = far;
       word &= ~far;
+    }
+  }
+  if (hub.degree) {
+    // one atomic per workgroup that kept vertices waiting
+    n_def = wave_sum_u32(n_def);
+    m_def = wave_sum_u64(m_def);
+    if (lane_id() == 0) {
+      s_ndef[wave_id()] = n_def;
+      s_mdef[wave_id()] = m_def;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t n = 0;
+      unsigned long long m = 0;
+      for (int w = 0; w < kWavesPerBlock; ++w) {
+        n += s_ndef[w];
+        m += s_mdef[w];
+      }
+      if (n) {
+        atomicAdd(&st->n_def, n);
+        atomicAdd(&st->m_def, m);
+      }
     }
   }
   const bool any = __ballot(word != 0) != 0;  // wave-uniform
@@ -842,14 +890,14 @@ uint64_t stats_parts_needed(uint64_t words) { return (words + kBlock - 1) / kBlo
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
                  StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* host_out,
-                 hipStream_t s, int direction) {
+                 hipStream_t s, int direction, const DevStats* prev) {
   uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, parts, val,
                      threshold);
   CG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kBlock), 0, s, parts, (uint32_t)blocks, stripes, rs, g.V, E, mlp,
-                     st, host_out, direction);
+                     st, host_out, direction, prev);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
@@ -857,12 +905,14 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
                    const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s,
-                   const TierTable* choice_tiers, uint32_t skip_bits) {
+                   const TierTable* choice_tiers, uint32_t skip_bits, HubDefer hub) {
   if (!words) return CG_OK;
+  if (hub.degree && (!val || !hub.delta)) return set_error(CG_ERR_INVALID, "hub deferral needs values and delta >= 1");
   const uint64_t blocks = (words + kBlock - 1) / kBlock;
   const TierTable tiers = choice_tiers ? *choice_tiers : TierTable{};
   hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, words, g, bins,
-                     q_small, q_med, q_chunk, st, val, threshold, carry, choice_tiers ? 1 : 0, tiers, skip_bits);
+                     q_small, q_med, q_chunk, st, val, threshold, carry, choice_tiers ? 1 : 0, tiers, skip_bits,
+                     hub);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

@@ -138,6 +138,15 @@ typedef struct cg_options {
                                      wavefronts) */
   uint32_t batch_group;           /* cg_batch_run: frontier items per wavefront (1-64, 0 = 32);
                                      their edges are spread over the 64 lanes */
+  uint32_t hub_degree;            /* SSSP under the GPU and learned schedules: a frontier vertex
+                                     of out-degree >= hub_degree waits (stays in the frontier)
+                                     while its distance is >= the frontier's smallest distance +
+                                     hub_delta, so a hub is expanded once its distance has
+                                     (nearly) settled instead of at every improvement;
+                                     lower-degree vertices never wait.  0 = off; not combined
+                                     with sssp_delta */
+  uint32_t hub_delta;             /* window of hub_degree above the frontier's smallest distance;
+                                     0 = 5 */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

@@ -372,6 +372,47 @@ def test_min_pull_rule(oracle, min_pull):
     assert np.array_equal(got, oracle.sssp(lat, 77))
 
 
+@pytest.mark.parametrize("hub_degree,hub_delta", [(1, 1), (16, 5), (128, 1), (128, 0), (4096, 50)])
+def test_sssp_hub_deferral(oracle, hub_degree, hub_delta):
+    """High-degree frontier vertices wait until their distance is within
+    hub_delta of the frontier's smallest: the order of relaxations changes,
+    the distances stay the unique fixed point (GPU and learned schedules,
+    plain and renumbered engines)."""
+    cases = [random_graph(oracle, 22, 30000, 200000, hub=30000), oracle.rmat_graph(15, 16, seed=6)]
+    if hub_degree > 1:
+        # (every lattice vertex waiting would take one iteration per distance)
+        cases.append(oracle.grid_graph(60, 0.1, seed=2))
+    for og in cases:
+        g = gpu_graph(og)
+        for root in (oracle.max_outdeg_vertex(og), og.V // 3):
+            ref = oracle.sssp(og, root)
+            for mode, relabel in ((_lib.CG_SCHED_GPU, 1), (_lib.CG_SCHED_GPU, 2), (_lib.CG_SCHED_LEARNED, 1)):
+                got, st = run(g, SSSP, root, sched_mode=mode, relabel=relabel, hub_degree=hub_degree,
+                              hub_delta=hub_delta)
+                assert np.array_equal(got, ref), (hub_degree, hub_delta, root, mode, relabel)
+                assert st["edges_relaxed"] >= st["edges_traversed"]
+
+
+def test_sssp_hub_deferral_accounting(oracle):
+    """With every vertex waiting until its distance is the frontier's smallest
+    (hub_degree 1, hub_delta 1, integer weights >= 1) each vertex is expanded
+    once, at its final distance: the relaxed-edge count the engine reports
+    (expansions minus the vertices kept waiting) equals the out-degree sum of
+    the reached vertices, and the BFS schedule ignores the option."""
+    og = oracle.rmat_graph(13, 16, seed=3, weights=(1, 20))
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    got, st = run(g, SSSP, root, hub_degree=1, hub_delta=1)
+    assert np.array_equal(got, oracle.sssp(og, root))
+    assert st["edges_relaxed"] == st["edges_traversed"] == oracle.reached_edges(og, got)
+    _, plain = run(g, SSSP, root)
+    assert plain["edges_relaxed"] > st["edges_relaxed"]
+    got, _ = run(g, BFS, root, hub_degree=1, hub_delta=1)
+    assert np.array_equal(got, oracle.bfs(og, root))
+    with pytest.raises(cg.CgError):
+        cg.Engine(g, SSSP, hub_degree=64, sssp_delta=10)
+
+
 @pytest.mark.parametrize("cap", [1, 7, 100000])
 def test_expand_grid_caps(oracle, cap):
     """However few workgroups the push expansion gets, its wavefronts cover
