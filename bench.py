@@ -270,9 +270,10 @@ def run_batched(cg, np):
 def run_extra(cg, np, info24, ceiling):
     extra = {"batched_mixed": run_batched(cg, np)}
     # BFS R-MAT scale 18 (the reference's own CPU-runnable configuration)
+    from cggraph_amd import autotune
     g = cg.Graph.rmat(18, 16, seed=1, weights=None)
     inf = g.info()
-    eng = cg.Engine(g, 0, sched_mode=1)
+    eng = cg.Engine(g, 0, sched_mode=1, **autotune.best_options(autotune.engine_signature("rmat", 0, inf["V"], inf["E"])))
     dt, st = timed_runs(eng, inf["max_outdeg_vertex"], 20, 3)
     eng.close()
     extra["bfs_s18"] = {"gteps": round(st["edges_traversed"] / dt / 1e9, 3), "ms": round(dt * 1e3, 4),
@@ -280,7 +281,6 @@ def run_extra(cg, np, info24, ceiling):
     log(f"BFS s18: {extra['bfs_s18']}")
     g.close()
     # BFS on scale 24 as well (tuned tiers for this graph kind, when the table has them)
-    from cggraph_amd import autotune
     g = cg.Graph.rmat(24, 16, seed=1, weights=None)
     inf = g.info()
     eng = cg.Engine(g, 0, sched_mode=1, **autotune.best_options(autotune.engine_signature("rmat", 0, inf["V"], inf["E"])))

@@ -439,33 +439,52 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   uint64_t word = wi < words ? in[wi] : 0;
   uint32_t n_def = 0;
   unsigned long long m_def = 0;
-  if (val && word) {
+  if (val && word && !hub.degree) {
     // delta-stepping: vertices at or above the threshold wait in the next
-    // frontier; this lane owns word wi of both bitmaps during the compaction.
-    // Hub deferral: only vertices of out-degree >= hub.degree wait, against
-    // a threshold that follows the frontier's smallest value.
-    const uint64_t t_hub = (uint64_t)st->min_val + hub.delta;
-    const uint32_t thr = hub.degree ? (t_hub > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t_hub) : threshold;
+    // frontier; this lane owns word wi of both bitmaps during the compaction
     uint64_t far = 0;
     for (uint64_t t = word; t; t &= t - 1) {
       const int b = __builtin_ctzll(t);
-      const uint64_t v = wi * 64 + b;
-      if (hub.degree) {
-        // the degree first: offsets are read for the queues anyway, values
-        // are a gather
-        const uint64_t deg = g.off[v + 1] - g.off[v];
-        if (deg < hub.degree || val[v] < thr) continue;
-        ++n_def;
-        m_def += deg;
-      } else if (val[v] < thr) {
-        continue;
-      }
-      far |= 1ull << b;
+      if (val[wi * 64 + b] >= threshold) far |= 1ull << b;
     }
     if (far) {
       carry[wi] = far;
       word &= ~far;
     }
+  }
+  const bool any = __ballot(word != 0) != 0;  // wave-uniform
+  uint32_t c_small = 0, c_med = 0, c_chunk = 0;
+  uint32_t o_small = 0, o_med = 0, o_chunk = 0;
+  uint32_t t_small = 0, t_med = 0, t_chunk = 0;
+  if (any) {
+    // Hub deferral, decided in the counting pass (the degree comes with the
+    // offsets it loads): a vertex of out-degree >= hub.degree whose value is
+    // >= the frontier's smallest + hub.delta waits in the next frontier
+    uint32_t thr = 0xFFFFFFFFu;
+    if (hub.degree) {
+      const uint64_t t = (uint64_t)st->min_val + hub.delta;
+      thr = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+    }
+    uint64_t wait = 0;
+    for_each_active(word, wi, g.off, g.V, [&](uint32_t v, uint64_t deg) {
+      if (deg == 0) return;
+      if (deg >= hub.degree && hub.degree && val[v] >= thr) {
+        wait |= 1ull << (v & 63);
+        ++n_def;
+        m_def += deg;
+        return;
+      }
+      if (deg < bins.bin_wave) c_small++;
+      else if (deg < bins.bin_chunk) c_med++;
+      else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+    });
+    if (wait) {
+      carry[wi] = wait;
+      word &= ~wait;
+    }
+    o_small = wave_excl_scan(c_small, &t_small);
+    o_med = wave_excl_scan(c_med, &t_med);
+    o_chunk = wave_excl_scan(c_chunk, &t_chunk);
   }
   if (hub.degree) {
     // one atomic per workgroup that kept vertices waiting
@@ -488,21 +507,6 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
         atomicAdd(&st->m_def, m);
       }
     }
-  }
-  const bool any = __ballot(word != 0) != 0;  // wave-uniform
-  uint32_t c_small = 0, c_med = 0, c_chunk = 0;
-  uint32_t o_small = 0, o_med = 0, o_chunk = 0;
-  uint32_t t_small = 0, t_med = 0, t_chunk = 0;
-  if (any) {
-    for_each_active(word, wi, g.off, g.V, [&](uint32_t, uint64_t deg) {
-      if (deg == 0) return;
-      if (deg < bins.bin_wave) c_small++;
-      else if (deg < bins.bin_chunk) c_med++;
-      else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
-    });
-    o_small = wave_excl_scan(c_small, &t_small);
-    o_med = wave_excl_scan(c_med, &t_med);
-    o_chunk = wave_excl_scan(c_chunk, &t_chunk);
   }
   if (lane_id() == 0) {
     s_tot[wave_id()][0] = t_small;

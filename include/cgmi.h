@@ -118,9 +118,9 @@ typedef struct cg_options {
                                      65 536 vertices */
   uint32_t relabel;               /* BFS / SSSP: run on an engine-owned copy renumbered by
                                      descending out-degree (results come back in the caller's
-                                     ids): 0 automatic (at least 65 536 vertices and the top 1/64
-                                     of vertices send at least half of the edges), 1 never,
-                                     2 always */
+                                     ids): 0 automatic (at least 2^21 vertices for SSSP, 2^23
+                                     for BFS, and the top 1/64 of vertices send at least half of
+                                     the edges), 1 never, 2 always */
   uint32_t pr_kernel;             /* PageRank sweep over layouts 1 and 2: 0 automatic (row tiers,
                                      the fastest measured), 1 row tiers (a lane /
                                      wavefront / workgroup per row by in-degree), 2 edge tiles
