@@ -187,32 +187,49 @@ __global__ void k_init_run_state(RunState* rs, unsigned long long visited) {
   }
 }
 
-// Calls f(v, out_degree) for the set bits of a lane's frontier word, in
-// vertex order.  Dense 16-vertex slices load their 17 offsets at once (all in
-// flight together) instead of one dependent pair per vertex; slices that
-// would read past off[V] take the per-vertex path.
+// The statistics and the compaction give each lane one 16-vertex slice of
+// the frontier bitmap (four lanes per 64-bit word, all in one wavefront), so
+// a small graph still spreads over many wavefronts and no lane walks more
+// than 16 vertices.
+constexpr uint64_t kSliceBits = 16;
+constexpr uint64_t kSlicesPerWord = 64 / kSliceBits;
+
+// Calls f(v, out_degree) for the set bits of the slice of vertices [base,
+// base + 16), in vertex order.  A dense slice loads its 17 offsets at once
+// (all in flight together) instead of one dependent pair per vertex; a slice
+// that would read past off[V] takes the per-vertex path.
 template <class F>
-__device__ __forceinline__ void for_each_active(uint64_t word, uint64_t wi, const uint64_t* __restrict__ off,
-                                                uint32_t V, F&& f) {
-#pragma unroll 1
-  for (int c = 0; c < 4; ++c) {
-    uint32_t bits = (uint32_t)(word >> (16 * c)) & 0xFFFFu;
-    if (!bits) continue;
-    const uint64_t base = wi * 64 + 16 * (uint64_t)c;
-    if (__popc(bits) >= 4 && base + 16 <= V) {
-      uint64_t o[17];
+__device__ __forceinline__ void for_each_in_slice(uint32_t bits, uint64_t base, const uint64_t* __restrict__ off,
+                                                  uint32_t V, F&& f) {
+  if (!bits) return;
+  if (__popc(bits) >= 4 && base + 16 <= V) {
+    uint64_t o[17];
 #pragma unroll
-      for (int j = 0; j < 17; ++j) o[j] = off[base + j];
+    for (int j = 0; j < 17; ++j) o[j] = off[base + j];
 #pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if ((bits >> j) & 1u) f((uint32_t)(base + j), o[j + 1] - o[j]);
-    } else {
-      for (; bits; bits &= bits - 1) {
-        const uint32_t v = (uint32_t)(base + (uint32_t)__builtin_ctz(bits));
-        f(v, off[v + 1] - off[v]);
-      }
+    for (int j = 0; j < 16; ++j)
+      if ((bits >> j) & 1u) f((uint32_t)(base + j), o[j + 1] - o[j]);
+  } else {
+    for (; bits; bits &= bits - 1) {
+      const uint32_t v = (uint32_t)(base + (uint32_t)__builtin_ctz(bits));
+      f(v, off[v + 1] - off[v]);
     }
   }
+}
+
+// Slice si of a bitmap: its 16 bits, and its first vertex.
+__device__ __forceinline__ uint32_t slice_bits(uint64_t word, uint64_t si) {
+  return (uint32_t)(word >> (kSliceBits * (si % kSlicesPerWord))) & 0xFFFFu;
+}
+
+// The four lanes of a word combine their 16-bit slices into the word (lane
+// order within the wavefront puts them next to each other); every lane of
+// the wavefront must call this.
+__device__ __forceinline__ uint64_t join_slices(uint32_t bits, uint64_t si) {
+  uint64_t w = (uint64_t)bits << (kSliceBits * (si % kSlicesPerWord));
+  w |= __shfl_xor(w, 1, kWave);
+  w |= __shfl_xor(w, 2, kWave);
+  return w;
 }
 
 // k_stats grid cap: each workgroup leaves one partial for the reduction
@@ -368,16 +385,18 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
   uint32_t c_act = 0, c_zero = 0, c_near = 0, mn = 0xFFFFFFFFu;
   uint64_t m = 0;
   uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // a capped grid strides over the words (fewer workgroup partials)
-  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < words; base += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t wi = base + threadIdx.x;
-    uint64_t word = 0;
-    if (wi < words) {
-      word = in[wi];
-      if (zero_me) zero_me[wi] = 0;
+  // a capped grid strides over the slices (fewer workgroup partials)
+  const uint64_t slices = words * kSlicesPerWord;
+  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t si = base + threadIdx.x;
+    uint32_t bits = 0;
+    if (si < slices) {
+      const uint64_t wi = si / kSlicesPerWord;
+      bits = slice_bits(in[wi], si);
+      if (zero_me && si % kSlicesPerWord == 0) zero_me[wi] = 0;
     }
-    c_act += (uint32_t)__popcll(word);
-    for_each_active(word, wi, g.off, g.V, [&](uint32_t v, uint64_t deg) {
+    c_act += (uint32_t)__popc(bits);
+    for_each_in_slice(bits, si * kSliceBits, g.off, g.V, [&](uint32_t v, uint64_t deg) {
       m += deg;
       h[hist_bin(deg)]++;
       c_zero += deg == 0;
@@ -435,22 +454,24 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
     bins.bin_wave = tiers.wave[tier];
     bins.bin_chunk = tiers.chunk[tier];
   }
-  const uint64_t wi = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
-  uint64_t word = wi < words ? in[wi] : 0;
+  const uint64_t si = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  const uint64_t wi = si / kSlicesPerWord;
+  const uint64_t vbase = si * kSliceBits;
+  uint32_t word = si < words * kSlicesPerWord ? slice_bits(in[wi], si) : 0u;
   uint32_t n_def = 0;
   unsigned long long m_def = 0;
-  if (val && word && !hub.degree) {
+  if (val && !hub.degree) {
     // delta-stepping: vertices at or above the threshold wait in the next
-    // frontier; this lane owns word wi of both bitmaps during the compaction
-    uint64_t far = 0;
-    for (uint64_t t = word; t; t &= t - 1) {
-      const int b = __builtin_ctzll(t);
-      if (val[wi * 64 + b] >= threshold) far |= 1ull << b;
+    // frontier; the four lanes of word wi own it in both bitmaps during the
+    // compaction
+    uint32_t far = 0;
+    for (uint32_t t = word; t; t &= t - 1) {
+      const int b = __builtin_ctz(t);
+      if (val[vbase + b] >= threshold) far |= 1u << b;
     }
-    if (far) {
-      carry[wi] = far;
-      word &= ~far;
-    }
+    const uint64_t far_word = join_slices(far, si);
+    if (far_word && si % kSlicesPerWord == 0) carry[wi] = far_word;
+    word &= ~far;
   }
   const bool any = __ballot(word != 0) != 0;  // wave-uniform
   uint32_t c_small = 0, c_med = 0, c_chunk = 0;
@@ -465,11 +486,11 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
       const uint64_t t = (uint64_t)st->min_val + hub.delta;
       thr = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
     }
-    uint64_t wait = 0;
-    for_each_active(word, wi, g.off, g.V, [&](uint32_t v, uint64_t deg) {
+    uint32_t wait = 0;
+    for_each_in_slice(word, vbase, g.off, g.V, [&](uint32_t v, uint64_t deg) {
       if (deg == 0) return;
       if (deg >= hub.degree && hub.degree && val[v] >= thr) {
-        wait |= 1ull << (v & 63);
+        wait |= 1u << (v - vbase);
         ++n_def;
         m_def += deg;
         return;
@@ -478,8 +499,9 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
       else if (deg < bins.bin_chunk) c_med++;
       else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
     });
-    if (wait) {
-      carry[wi] = wait;
+    if (hub.degree) {
+      const uint64_t wait_word = join_slices(wait, si);
+      if (wait_word && si % kSlicesPerWord == 0) carry[wi] = wait_word;
       word &= ~wait;
     }
     o_small = wave_excl_scan(c_small, &t_small);
@@ -527,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   uint32_t b_small = s_base[wave_id()][0] + o_small;
   uint32_t b_med = s_base[wave_id()][1] + o_med;
   uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
-  for_each_active(word, wi, g.off, g.V, [&](uint32_t v, uint64_t deg) {
+  for_each_in_slice(word, vbase, g.off, g.V, [&](uint32_t v, uint64_t deg) {
     if (deg == 0) return;
     if (deg < bins.bin_wave) q_small[b_small++] = v;
     else if (deg < bins.bin_chunk) q_med[b_med++] = v;
@@ -889,13 +911,17 @@ int launch_init_run_state(RunState* rs, uint64_t visited, hipStream_t s) {
   return CG_OK;
 }
 
-uint64_t stats_parts_needed(uint64_t words) { return (words + kBlock - 1) / kBlock + 1; }
+uint64_t stats_parts_needed(uint64_t words) {
+  const uint64_t blocks = (words * kSlicesPerWord + kBlock - 1) / kBlock;
+  return (blocks < kStatsMaxBlocks ? blocks : kStatsMaxBlocks) + 1;
+}
 
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
                  StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* host_out,
                  hipStream_t s, int direction, const DevStats* prev) {
-  uint64_t blocks = words ? (words + kBlock - 1) / kBlock : 1;
+  const uint64_t slices = words * kSlicesPerWord;
+  uint64_t blocks = slices ? (slices + kBlock - 1) / kBlock : 1;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, zero_me, words, g, parts, val,
                      threshold);
@@ -912,7 +938,7 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
                    const TierTable* choice_tiers, uint32_t skip_bits, HubDefer hub) {
   if (!words) return CG_OK;
   if (hub.degree && (!val || !hub.delta)) return set_error(CG_ERR_INVALID, "hub deferral needs values and delta >= 1");
-  const uint64_t blocks = (words + kBlock - 1) / kBlock;
+  const uint64_t blocks = (words * kSlicesPerWord + kBlock - 1) / kBlock;
   const TierTable tiers = choice_tiers ? *choice_tiers : TierTable{};
   hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, words, g, bins,
                      q_small, q_med, q_chunk, st, val, threshold, carry, choice_tiers ? 1 : 0, tiers, skip_bits,
