@@ -507,7 +507,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     if (!e->hst_map_dev) return set_error(CG_ERR_HIP, "mapped statistics buffer has no device address");
     CG_TRY(e->acc.ensure(kAccSlots));
     CG_TRY(e->parts.ensure(stats_parts_needed(words)));
-    CG_TRY(e->stripes.ensure(kStripes * kStripeStride));
+    CG_TRY(e->stripes.ensure(kStripeWords));
     CG_TRY(e->run_state.ensure(1));
   }
   if (e->opt.trace_path) {

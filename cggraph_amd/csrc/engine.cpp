@@ -308,7 +308,7 @@ int probe_all(cg_engine* e, const DevGraph& dg, const DevStats& sst, DevStats* d
     CG_HIP(hipMemcpyAsync(e->val.p, e->val_snap.p, (size_t)V * 4, hipMemcpyDeviceToDevice, s));
     CG_HIP(hipMemcpyAsync(in, e->bm_snap.p, words * 8, hipMemcpyDeviceToDevice, s));
     CG_HIP(hipMemsetAsync(out, 0, words * 8, s));
-    CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripes * kStripeStride * sizeof(uint32_t), s));
+    CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripeWords * sizeof(uint32_t), s));
     if (st.host_share) e->team.shadow = shadow_snap;
   }
   CG_HIP(hipStreamSynchronize(s));
@@ -356,7 +356,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // ---- initAlgorithm (CGgraph_engine.cuh:1259-1316) ----
   CG_HIP(hipMemsetAsync(e->bm[0].p, 0, words * sizeof(uint64_t), s));
   CG_HIP(hipMemsetAsync(e->bm[1].p, 0, words * sizeof(uint64_t), s));
-  CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripes * kStripeStride * sizeof(uint32_t), s));
+  CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripeWords * sizeof(uint32_t), s));
   CG_HIP(hipMemsetAsync(e->st.p, 0, 2 * sizeof(DevStats), s));
   for (int k = 0; k < 2; ++k) __atomic_store_n(&e->hst_map.p[k].iter, 0xFFFFFFFFu, __ATOMIC_RELEASE);
   if (alg == CG_CC) {
@@ -398,8 +398,9 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // frontier's values for their smallest one, the compaction keeps hubs above
   // it + delta waiting
   const bool hub_on = alg == CG_SSSP && e->opt.hub_degree != 0 && early && !delta;
-  const HubDefer hub{hub_on ? e->opt.hub_degree : 0u, e->opt.hub_delta ? e->opt.hub_delta : kHubDeltaDefault};
-  const uint32_t* sval = hub_on ? e->val.p : tval;
+  const HubDefer hub{hub_on ? e->opt.hub_degree : 0u, e->opt.hub_delta ? e->opt.hub_delta : kHubDeltaDefault,
+                     hub_on ? e->stripes.p : nullptr};
+  const uint32_t* cval = hub_on ? e->val.p : tval;  // values the compaction reads
   TierTable choice_tiers{};
   for (int k = 0; k < 4; ++k) {
     choice_tiers.wave[k] = k ? kTierSets[k].bin_wave : e->opt.bin_wave;
@@ -426,16 +427,17 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     const int sl = k & 1, c = (k - 1) & 1;
     DevStats* dk = e->st.p + sl;
     CG_TRY(launch_stats(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p, e->run_state.p,
-                        mlp_dev, sval, threshold, dk, e->hst_map_dev + sl, s,
-                        early_bfs ? 1 : 0, hub_on ? e->st.p + (sl ^ 1) : nullptr));
+                        mlp_dev, tval, threshold, dk, e->hst_map_dev + sl, s,
+                        early_bfs ? 1 : 0, hub_on ? e->st.p + (sl ^ 1) : nullptr, hub_on));
     if (!early) return CG_OK;
-    CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, sval,
+    CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, cval,
                           threshold, e->bm[c ^ 1].p, s, (early_learned || early_bfs) ? &choice_tiers : nullptr,
                           early_bfs ? 1u : 8u, hub));
     const int tp = k & 3;
     if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
     CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
-                              ~0ull >> 8, 0xFFFFFFFFu, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s));
+                              ~0ull >> 8, 0xFFFFFFFFu, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s,
+                              hub_on));
     if (early_bfs) CG_TRY(launch_pull_bfs(dg, e->val.p, e->bm[c].p, e->bm[c ^ 1].p, e->stripes.p, s, dk));
     if (e->opt.profile) {
       CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));

@@ -37,6 +37,9 @@ static_assert(sizeof(DevStats) % 8 == 0, "DevStats layout");
 // so that their atomics do not serialise on one address.
 constexpr int kStripes = 16;
 constexpr int kStripeStride = 32;  // uint32 per stripe = 128 bytes
+// a second set of kStripes stripes (minima) follows the counters
+constexpr int kStripeMinBase = kStripes * kStripeStride;
+constexpr int kStripeWords = 2 * kStripes * kStripeStride;
 
 // Per-run device state the stats kernel carries from iteration to iteration.
 struct RunState {
@@ -101,7 +104,8 @@ uint64_t stats_parts_needed(uint64_t words);
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g,
                  uint64_t E, StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st,
-                 DevStats* host_out, hipStream_t s, int direction = 0, const DevStats* prev = nullptr);
+                 DevStats* host_out, hipStream_t s, int direction = 0, const DevStats* prev = nullptr,
+                 bool min_from_stripes = false);
 // Writes the three work queues; lengths accumulate in st->n_small/n_med/n_chunk.
 // Frontier vertices whose value is >= threshold are not queued but carried
 // into `carry` (the next frontier) unchanged.
@@ -112,14 +116,16 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
 // nothing.
 // hub.degree != 0 (SSSP): instead of `threshold`, a frontier vertex of
 // out-degree >= hub.degree waits when its value is >= st->min_val + hub.delta
-// (the frontier's smallest value, from k_stats, which then needs `val`);
-// lower-degree vertices never wait.  The waiting vertices are counted into
-// st->n_def / st->m_def.
+// (the frontier's smallest value: from k_stats with `val`, or from the minima
+// stripes the expansion and this kernel keep, hub.track); lower-degree
+// vertices never wait.  The waiting vertices are counted into st->n_def /
+// st->m_def.
 struct TierTable {
   uint32_t wave[4], chunk[4];
 };
 struct HubDefer {
   uint32_t degree, delta;
+  uint32_t* track = nullptr;  // stripes: the waiting vertices' smallest value goes to their minima
 };
 int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const Bins& bins,
                    uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
@@ -135,7 +141,7 @@ uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bin
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
-                       uint32_t* stripes, bool exact, hipStream_t s);
+                       uint32_t* stripes, bool exact, hipStream_t s, bool track_min = false);
 // Bottom-up BFS step: unreached vertices look for a parent in `in`.
 // cond not NULL: runs only when cond->choice (left by k_stats) asks for a
 // bottom-up step; the grid is capped and wavefronts stride over the words.

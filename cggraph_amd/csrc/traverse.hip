@@ -71,6 +71,22 @@ __device__ __forceinline__ void stripe_add(uint32_t* stripes, uint32_t v) {
   if (v) atomicAdd(stripes + (blockIdx.x % kStripes) * kStripeStride, v);
 }
 
+// The second set of stripes holds minima (the next frontier's smallest value
+// under hub deferral); k_stats_reduce reads and resets them.
+__device__ __forceinline__ void stripe_min(uint32_t* stripes, uint32_t v) {
+  if (v != 0xFFFFFFFFu && lane_id() == 0)
+    atomicMin(stripes + kStripeMinBase + (blockIdx.x % kStripes) * kStripeStride, v);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) {
+    const uint32_t o = __shfl_xor(x, d, kWave);
+    x = o < x ? o : x;
+  }
+  return x;
+}
+
 enum { ALG_BFS = CG_BFS, ALG_SSSP = CG_SSSP, ALG_CC = CG_CC };
 
 // Edge operation shared by every push kernel.  Returns 1 when this update
@@ -90,40 +106,47 @@ __device__ __forceinline__ uint32_t candidate(uint32_t sv, uint64_t e, const uin
 // The update of destination d with candidate cand, given a value `seen` read
 // earlier (values only decrease, so a stale read is never below the current
 // value and can only let an update through that the atomic then rejects).
-template <int ALG, bool EXACT>
+// TRACK: mn takes the smallest candidate that activated a destination, so
+// the next frontier's smallest value is known without reading it back (every
+// activated vertex ends at or below a tracked candidate, and the smallest
+// tracked candidate is some vertex's final value).
+template <int ALG, bool EXACT, bool TRACK = false>
 __device__ __forceinline__ uint32_t commit(uint32_t cand, uint32_t seen, uint32_t d, uint32_t* __restrict__ val,
-                                           uint64_t* __restrict__ out) {
+                                           uint64_t* __restrict__ out, uint32_t& mn) {
   if (cand < seen) {
     if constexpr (EXACT) {
       const uint32_t old = atomicMin(val + d, cand);
       if (cand < old) {
         atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
+        if constexpr (TRACK) mn = cand < mn ? cand : mn;
         return old == kUnreached ? 1u : 0u;
       }
     } else {
       (void)atomicMin(val + d, cand);
       (void)atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
+      if constexpr (TRACK) mn = cand < mn ? cand : mn;
       return seen == kUnreached ? 1u : 0u;
     }
   }
   return 0u;
 }
 
-template <int ALG, bool EXACT>
+template <int ALG, bool EXACT, bool TRACK = false>
 __device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_t* __restrict__ dst,
                                           const uint32_t* __restrict__ w, uint32_t* __restrict__ val,
-                                          uint64_t* __restrict__ out) {
+                                          uint64_t* __restrict__ out, uint32_t& mn) {
   const uint32_t d = dst[e];
-  return commit<ALG, EXACT>(candidate<ALG>(sv, e, w), val[d], d, val, out);
+  return commit<ALG, EXACT, TRACK>(candidate<ALG>(sv, e, w), val[d], d, val, out, mn);
 }
 
 // Edges [s, e) of one source with value sv, lane-strided over a wavefront.
 // Four edges per lane are loaded (destination, weight, destination value)
 // before any of them is committed, so four gathers are in flight per lane.
-template <int ALG, bool EXACT>
+template <int ALG, bool EXACT, bool TRACK>
 __device__ __forceinline__ uint32_t relax_strided(uint32_t sv, uint64_t s, uint64_t e, uint32_t lane,
                                                   const uint32_t* __restrict__ dst, const uint32_t* __restrict__ w,
-                                                  uint32_t* __restrict__ val, uint64_t* __restrict__ out) {
+                                                  uint32_t* __restrict__ val, uint64_t* __restrict__ out,
+                                                  uint32_t& mn) {
   uint32_t n = 0;
   uint64_t k = s + lane;
   for (; k + 3 * kWave < e; k += 4 * kWave) {
@@ -131,12 +154,12 @@ __device__ __forceinline__ uint32_t relax_strided(uint32_t sv, uint64_t s, uint6
     const uint32_t c0 = candidate<ALG>(sv, k, w), c1 = candidate<ALG>(sv, k + kWave, w);
     const uint32_t c2 = candidate<ALG>(sv, k + 2 * kWave, w), c3 = candidate<ALG>(sv, k + 3 * kWave, w);
     const uint32_t s0 = val[d0], s1 = val[d1], s2 = val[d2], s3 = val[d3];
-    n += commit<ALG, EXACT>(c0, s0, d0, val, out);
-    n += commit<ALG, EXACT>(c1, s1, d1, val, out);
-    n += commit<ALG, EXACT>(c2, s2, d2, val, out);
-    n += commit<ALG, EXACT>(c3, s3, d3, val, out);
+    n += commit<ALG, EXACT, TRACK>(c0, s0, d0, val, out, mn);
+    n += commit<ALG, EXACT, TRACK>(c1, s1, d1, val, out, mn);
+    n += commit<ALG, EXACT, TRACK>(c2, s2, d2, val, out, mn);
+    n += commit<ALG, EXACT, TRACK>(c3, s3, d3, val, out, mn);
   }
-  for (; k < e; k += kWave) n += relax<ALG, EXACT>(sv, k, dst, w, val, out);
+  for (; k < e; k += kWave) n += relax<ALG, EXACT, TRACK>(sv, k, dst, w, val, out, mn);
   return n;
 }
 
@@ -265,7 +288,7 @@ __device__ __forceinline__ void stats_block_part(const StatsPart& p, StatsPart* 
 __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t nparts, uint32_t* __restrict__ stripes,
                              RunState* __restrict__ rs, uint32_t V, uint64_t E, const float* __restrict__ mlp,
                              DevStats* __restrict__ st, DevStats* __restrict__ host_out, int direction,
-                             const DevStats* __restrict__ prev) {
+                             const DevStats* __restrict__ prev, int min_from_stripes) {
   __shared__ unsigned long long s_m[kBlock];
   __shared__ uint32_t s_c[kBlock][11];
   __shared__ uint32_t s_min[kBlock];
@@ -325,6 +348,18 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
       // written by the previous iteration's compaction, which has finished
       d.n_def_prev = prev->n_def;
       d.m_def_prev = prev->m_def;
+    }
+    if (min_from_stripes) {
+      // the frontier's smallest value, left by the previous expansion and
+      // compaction (the run starts with 0: the root)
+      uint32_t mv = 0xFFFFFFFFu;
+#pragma unroll
+      for (int k = 0; k < kStripes; ++k) {
+        const uint32_t x = stripes[kStripeMinBase + k * kStripeStride];
+        mv = x < mv ? x : mv;
+        stripes[kStripeMinBase + k * kStripeStride] = 0xFFFFFFFFu;
+      }
+      d.min_val = mv;
     }
     if (direction) {
       // Beamer, Asanovic, Patterson (SC'12): bottom-up when the frontier's
@@ -429,8 +464,8 @@ __global__ __launch_bounds__(kBlock) void k_stats_reduce(const StatsPart* __rest
                                                          uint32_t V, uint64_t E, const float* __restrict__ mlp,
                                                          DevStats* __restrict__ st,
                                                          DevStats* __restrict__ host_out, int direction,
-                                                         const DevStats* __restrict__ prev) {
-  stats_finish(parts, nparts, stripes, rs, V, E, mlp, st, host_out, direction, prev);
+                                                         const DevStats* __restrict__ prev, int min_from_stripes) {
+  stats_finish(parts, nparts, stripes, rs, V, E, mlp, st, host_out, direction, prev, min_from_stripes);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
@@ -486,14 +521,18 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
       const uint64_t t = (uint64_t)st->min_val + hub.delta;
       thr = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
     }
-    uint32_t wait = 0;
+    uint32_t wait = 0, wait_min = 0xFFFFFFFFu;
     for_each_in_slice(word, vbase, g.off, g.V, [&](uint32_t v, uint64_t deg) {
       if (deg == 0) return;
-      if (deg >= hub.degree && hub.degree && val[v] >= thr) {
-        wait |= 1u << (v - vbase);
-        ++n_def;
-        m_def += deg;
-        return;
+      if (deg >= hub.degree && hub.degree) {
+        const uint32_t x = val[v];
+        if (x >= thr) {
+          wait |= 1u << (v - vbase);
+          wait_min = x < wait_min ? x : wait_min;
+          ++n_def;
+          m_def += deg;
+          return;
+        }
       }
       if (deg < bins.bin_wave) c_small++;
       else if (deg < bins.bin_chunk) c_med++;
@@ -503,6 +542,8 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
       const uint64_t wait_word = join_slices(wait, si);
       if (wait_word && si % kSlicesPerWord == 0) carry[wi] = wait_word;
       word &= ~wait;
+      // the waiting vertices are part of the next frontier's smallest value
+      if (hub.track) stripe_min(hub.track, wave_min_u32(wait_min));
     }
     o_small = wave_excl_scan(c_small, &t_small);
     o_med = wave_excl_scan(c_med, &t_med);
@@ -560,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   });
 }
 
-template <int ALG, bool EXACT>
+template <int ALG, bool EXACT, bool TRACK>
 __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restrict__ val,
                                                    uint64_t* __restrict__ out,
                                                    const uint32_t* __restrict__ q_small,
@@ -579,6 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
   const uint32_t lane = lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
   uint32_t n_new = 0;
+  uint32_t mn = 0xFFFFFFFFu;  // smallest activating candidate (TRACK)
   for (uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); item < n_items; item += stride) {
     if (item < n_chunk) {
       // a slice of a high-degree vertex: lanes stride its edges
@@ -587,10 +629,10 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
       const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
       const uint64_t end = g.off[v + 1];
       const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
-      n_new += relax_strided<ALG, EXACT>(val[v], s, e, lane, g.dst, g.w, val, out);
+      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], s, e, lane, g.dst, g.w, val, out, mn);
     } else if (item < (uint64_t)n_chunk + n_med) {
       const uint32_t v = q_med[item - n_chunk];
-      n_new += relax_strided<ALG, EXACT>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out);
+      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out, mn);
     } else {
       // 64 low-degree vertices per wavefront, their edges spread evenly over
       // the lanes: lane j of round r takes edge r*64+j of the concatenated
@@ -619,7 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
         const uint64_t os = __shfl(s, (int)lo, kWave);
         const uint32_t osv = __shfl(sv, (int)lo, kWave);
         const uint32_t oex = __shfl(excl, (int)lo, kWave);
-        if (j < total) n_new += relax<ALG, EXACT>(osv, os + (j - oex), g.dst, g.w, val, out);
+        if (j < total) n_new += relax<ALG, EXACT, TRACK>(osv, os + (j - oex), g.dst, g.w, val, out, mn);
       }
     }
   }
@@ -627,6 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
     const uint32_t tot = wave_sum_u32(n_new);
     if (lane == 0) stripe_add(stripes, tot);
   }
+  if constexpr (TRACK) stripe_min(stripes, wave_min_u32(mn));
 }
 
 // Bottom-up BFS (Beamer, Asanovic, Patterson, SC'12): every unreached vertex
@@ -850,7 +893,8 @@ __global__ __launch_bounds__(kBlock) void k_push_bitmap(DevGraph g, uint32_t* __
     const uint64_t os = __shfl(s, (int)lo, kWave);
     const uint32_t osv = __shfl(sv, (int)lo, kWave);
     const uint32_t oex = __shfl(excl, (int)lo, kWave);
-    if (j < total) (void)relax<ALG, true>(osv, os + (j - oex), g.dst, g.w, val, out);
+    uint32_t mn = 0;  // not tracked here
+    if (j < total) (void)relax<ALG, true>(osv, os + (j - oex), g.dst, g.w, val, out, mn);
   }
 }
 
@@ -919,7 +963,7 @@ uint64_t stats_parts_needed(uint64_t words) {
 int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const DevGraph& g, uint64_t E,
                  StatsPart* parts, uint32_t* stripes, RunState* rs, const float* mlp,
                  const uint32_t* val, uint32_t threshold, DevStats* st, DevStats* host_out,
-                 hipStream_t s, int direction, const DevStats* prev) {
+                 hipStream_t s, int direction, const DevStats* prev, bool min_from_stripes) {
   const uint64_t slices = words * kSlicesPerWord;
   uint64_t blocks = slices ? (slices + kBlock - 1) / kBlock : 1;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
@@ -927,7 +971,7 @@ int launch_stats(const uint64_t* in, uint64_t* zero_me, uint64_t words, const De
                      threshold);
   CG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kBlock), 0, s, parts, (uint32_t)blocks, stripes, rs, g.V, E, mlp,
-                     st, host_out, direction, prev);
+                     st, host_out, direction, prev, min_from_stripes ? 1 : 0);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
@@ -955,16 +999,27 @@ uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bin
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
                        uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
-                       uint32_t* stripes, bool exact, hipStream_t s) {
+                       uint32_t* stripes, bool exact, hipStream_t s, bool track_min) {
   if (!item_bound) return CG_OK;
+  if (track_min && alg != CG_SSSP) return set_error(CG_ERR_INTERNAL, "minimum tracking is for SSSP");
   uint64_t blocks = (item_bound + kWavesPerBlock - 1) / kWavesPerBlock;
   uint64_t cap = bins.grid_cap ? bins.grid_cap : 2048;  // 256 CUs x 32 wavefronts
   if (cap > (1u << 20)) cap = 1u << 20;                 // the wavefronts stride; more adds nothing
   if (blocks > cap) blocks = cap;
   dim3 grid((unsigned)blocks), block(kBlock);
 #define CG_EXPAND(A, X)                                                                          \
-  hipLaunchKernelGGL((k_expand<A, X>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk, small_limit, \
+  hipLaunchKernelGGL((k_expand<A, X, false>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk, small_limit, \
                      bins.chunk_edges, st, stripes)
+  if (track_min) {
+    if (exact)
+      hipLaunchKernelGGL((k_expand<ALG_SSSP, true, true>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
+                         small_limit, bins.chunk_edges, st, stripes);
+    else
+      hipLaunchKernelGGL((k_expand<ALG_SSSP, false, true>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
+                         small_limit, bins.chunk_edges, st, stripes);
+    CG_LAUNCH_CHECK();
+    return CG_OK;
+  }
   switch (alg * 2 + (exact ? 1 : 0)) {
     case CG_BFS * 2: CG_EXPAND(ALG_BFS, false); break;
     case CG_BFS * 2 + 1: CG_EXPAND(ALG_BFS, true); break;
