@@ -113,8 +113,11 @@ typedef struct cg_options {
   uint32_t pr_layout;             /* PageRank in-edge layout: 0 automatic, 1 plain pull over the
                                      transposed graph, 2 pull over a copy renumbered by descending
                                      out-degree (gather locality on skewed graphs), 3 source
-                                     windows of 2^pr_segment_log2 vertices.  Automatic: a window
-                                     size in [1, 30] selects 3, 31 selects 1, 0 selects 2 above
+                                     windows of 2^pr_segment_log2 vertices, 4 propagation blocking
+                                     (out-edges binned by 8192-vertex destination blocks, sums in
+                                     64-bit fixed point; at most 2^25 vertices).  Automatic: a
+                                     window size in [1, 30] selects 3, 31 selects 1; 0 selects 4
+                                     from 2^23 to 2^25 vertices (with pr_kernel 0), else 2 above
                                      65 536 vertices */
   uint32_t relabel;               /* BFS / SSSP: run on an engine-owned copy renumbered by
                                      descending out-degree (results come back in the caller's

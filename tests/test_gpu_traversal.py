@@ -209,11 +209,12 @@ def test_pagerank_l1_and_determinism(oracle):
 
 @pytest.mark.parametrize("layout,seg,kernel", [(0, 31, 0), (0, 8, 0), (0, 11, 0), (0, 14, 0), (0, 0, 0), (1, 0, 1),
                                                (1, 0, 2), (2, 0, 1), (2, 0, 2), (2, 0, 3), (2, 12, 3), (2, 20, 3),
-                                               (3, 9, 0)])
+                                               (3, 9, 0), (4, 0, 0)])
 def test_pagerank_layouts(oracle, layout, seg, kernel):
     """Every in-edge layout (plain, renumbered by out-degree, source windows
-    smaller and larger than the graph) and every sweep (row tiers, edge tiles,
-    two source bands) against the FP64 oracle; run-to-run bit-identical."""
+    smaller and larger than the graph, propagation blocking) and every sweep
+    (row tiers, edge tiles, two source bands) against the FP64 oracle;
+    run-to-run bit-identical."""
     og = oracle.rmat_graph(17, 16, seed=2, weights=None)
     g = cg.Graph.from_csr(og.offsets, og.dest)
     e = cg.Engine(g, PR, pr_segment_log2=seg, pr_layout=layout, pr_kernel=kernel)
@@ -243,11 +244,30 @@ def test_pagerank_layouts(oracle, layout, seg, kernel):
     assert float(np.abs(r.astype(np.float64) - oracle.pagerank64(sparse, 20, 0.85)).sum()) <= 1e-6
 
 
+def test_pagerank_blocked_builds(oracle):
+    """Propagation blocking sums in 64-bit fixed point, so the ranks do not
+    depend on the order of the additions: two engines (whose edge slots are
+    placed by atomics, so their layouts may differ) give bit-identical ranks,
+    on a graph whose size is not a multiple of the 8192-vertex blocks and
+    whose hot blocks are cut into several tiles."""
+    og = random_graph(oracle, 5, 20001, 600000, hub=300000)
+    og.weight = None
+    ref = oracle.pagerank64(og, 20, 0.85)
+    g = cg.Graph.from_csr(og.offsets, og.dest)
+    runs = []
+    for _ in range(2):
+        e = cg.Engine(g, PR, pr_layout=4)
+        runs.append(e.run()[0])
+        e.close()
+    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+    assert float(np.abs(runs[0].astype(np.float64) - ref).sum()) <= 1e-6
+
+
 def test_pagerank_bad_layout(oracle):
     og = oracle.rmat_graph(8, 4, seed=1, weights=None)
     g = cg.Graph.from_csr(og.offsets, og.dest)
     with pytest.raises(cg.CgError):
-        cg.Engine(g, PR, pr_layout=4)
+        cg.Engine(g, PR, pr_layout=5)
     with pytest.raises(cg.CgError):
         cg.Engine(g, PR, pr_kernel=4)
 
