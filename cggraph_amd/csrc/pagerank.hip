@@ -843,7 +843,9 @@ constexpr uint32_t kPbShift = 13;        // destinations per bin: 2^13 (the LDS 
 constexpr uint32_t kPbSize = 1u << kPbShift;
 constexpr uint32_t kPbChunkShift = 13;   // sources per chunk: 2^13 (their contributions fill 32 KB of LDS)
 constexpr uint32_t kPbChunk = 1u << kPbChunkShift;
-constexpr uint64_t kPbTile = 1ull << 18;  // entries per workgroup in both passes
+constexpr uint64_t kPbTile = 1ull << 15;     // scatter: entries per workgroup (a slice of one
+                                             // chunk's bins; fewer pages written per workgroup)
+constexpr uint64_t kPbBinTile = 1ull << 18;  // gather: entries per workgroup
 constexpr uint32_t kPbMaxBins = 4096;     // per-tile LDS tables of 12 B per bin stay within 48 KB
 constexpr float kFixScale = 72057594037927936.0f;  // 2^56
 constexpr double kFixInv = 1.0 / 72057594037927936.0;
@@ -982,13 +984,16 @@ __global__ __launch_bounds__(kBlock) void k_pb_scatter(const uint32_t* __restric
                                                        const uint64_t* __restrict__ t_hi, uint32_t nb, uint32_t V,
                                                        const int64_t* __restrict__ delta,
                                                        const float* __restrict__ contrib, float* __restrict__ val) {
-  extern __shared__ float cc[];  // kPbChunk contributions, then nb deltas
+  extern __shared__ float cc[];  // kPbChunk contributions, then the deltas of the tile's bins
   D* dr = reinterpret_cast<D*>(cc + kPbChunk);
   const uint32_t t = blockIdx.x;
   const uint32_t c = t_chunk[t];
   const uint64_t lo = t_lo[t], hi = t_hi[t];
   const uint64_t v0 = (uint64_t)c << kPbChunkShift;
-  for (uint32_t k = threadIdx.x; k < nb; k += kBlock) dr[k] = (D)delta[(uint64_t)c * nb + k];
+  // the tile is a slice of the chunk's bin-sorted entries: its bins are
+  // [first, last]
+  const uint32_t b0 = ent[lo] >> kPbChunkShift, b1 = ent[hi - 1] >> kPbChunkShift;
+  for (uint32_t k = threadIdx.x; k <= b1 - b0; k += kBlock) dr[k] = (D)delta[(uint64_t)c * nb + b0 + k];
   for (uint32_t k = threadIdx.x; k < kPbChunk; k += kBlock) cc[k] = v0 + k < V ? contrib[v0 + k] : 0.0f;
   __syncthreads();
   uint64_t i = lo + threadIdx.x;
@@ -998,13 +1003,14 @@ __global__ __launch_bounds__(kBlock) void k_pb_scatter(const uint32_t* __restric
     for (int j = 0; j < kPbUnroll; ++j) x[j] = __builtin_nontemporal_load(ent + i + j * kBlock);
 #pragma unroll
     for (int j = 0; j < kPbUnroll; ++j)
-      __builtin_nontemporal_store(cc[x[j] & (kPbChunk - 1)],
-                                  val + (uint64_t)((int64_t)(i + j * kBlock) + (int64_t)dr[x[j] >> kPbChunkShift]));
+      __builtin_nontemporal_store(
+          cc[x[j] & (kPbChunk - 1)],
+          val + (uint64_t)((int64_t)(i + j * kBlock) + (int64_t)dr[(x[j] >> kPbChunkShift) - b0]));
   }
   for (; i < hi; i += kBlock) {
     const uint32_t x = __builtin_nontemporal_load(ent + i);
     __builtin_nontemporal_store(cc[x & (kPbChunk - 1)],
-                                val + (uint64_t)((int64_t)i + (int64_t)dr[x >> kPbChunkShift]));
+                                val + (uint64_t)((int64_t)i + (int64_t)dr[(x >> kPbChunkShift) - b0]));
   }
 }
 
@@ -1125,7 +1131,7 @@ int build_blocked(const DevGraph& g, uint64_t E, PrPlan* p, hipStream_t s) {
                        p->pb_t1_lo.p, p->pb_t1_hi.p, nb, reinterpret_cast<unsigned long long*>(in_base.p),
                        p->pb_delta.p, p->pb_ent.p, p->pb_dloc.p);
   CG_HIP(hipGetLastError());
-  // bins -> tiles of at most kPbTile entries; every bin has one (its ranks
+  // bins -> tiles of at most kPbBinTile entries; every bin has one (its ranks
   // are written by it even when it has no in-edges)
   std::vector<uint64_t> ho(nb + 1);
   CG_HIP(hipMemcpyAsync(ho.data(), bin_off.p, ho.size() * 8, hipMemcpyDeviceToHost, s));
@@ -1134,12 +1140,12 @@ int build_blocked(const DevGraph& g, uint64_t E, PrPlan* p, hipStream_t s) {
   std::vector<uint64_t> t2l, t2h;
   for (uint32_t b = 0; b < nb; ++b) {
     const uint64_t n = ho[b + 1] - ho[b];
-    const uint32_t k = n > kPbTile ? (uint32_t)((n + kPbTile - 1) / kPbTile) : 1u;
+    const uint32_t k = n > kPbBinTile ? (uint32_t)((n + kPbBinTile - 1) / kPbBinTile) : 1u;
     if (k > 1) multi.push_back(b);
     for (uint32_t i = 0; i < k; ++i) {
       t2b.push_back(b);
-      t2l.push_back(ho[b] + (uint64_t)i * kPbTile);
-      t2h.push_back(std::min(ho[b + 1], ho[b] + (uint64_t)(i + 1) * kPbTile));
+      t2l.push_back(ho[b] + (uint64_t)i * kPbBinTile);
+      t2h.push_back(std::min(ho[b + 1], ho[b] + (uint64_t)(i + 1) * kPbBinTile));
       t2m.push_back(k > 1 ? 1u : 0u);
     }
   }
