@@ -1027,19 +1027,33 @@ __global__ __launch_bounds__(kBlock) void k_pb_gather(const float* __restrict__ 
   for (uint32_t k = threadIdx.x; k < kPbSize; k += kBlock) acc[k] = 0;
   __syncthreads();
   const uint64_t lo = t_lo[t], hi = t_hi[t];
-  uint64_t j = lo + threadIdx.x;
-  for (; j + (kPbUnroll - 1) * kBlock < hi; j += kPbUnroll * kBlock) {
-    float x[kPbUnroll];
-    uint16_t d[kPbUnroll];
+  // pairs of entries from an even index: one 8-byte value load and one
+  // 4-byte destination load per pair; entries outside [lo, hi) are skipped
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  const uint64_t p0 = lo >> 1, p1 = (hi + 1) >> 1;
+  const f32x2* val2 = reinterpret_cast<const f32x2*>(val);
+  const uint32_t* dloc2 = reinterpret_cast<const uint32_t*>(dloc);
+  uint64_t pj = p0 + threadIdx.x;
+  for (; pj + (kPbUnroll / 2 - 1) * kBlock < p1; pj += (kPbUnroll / 2) * kBlock) {
+    f32x2 x[kPbUnroll / 2];
+    uint32_t d[kPbUnroll / 2];
 #pragma unroll
-    for (int u = 0; u < kPbUnroll; ++u) {
-      x[u] = __builtin_nontemporal_load(val + j + u * kBlock);
-      d[u] = __builtin_nontemporal_load(dloc + j + u * kBlock);
+    for (int u = 0; u < kPbUnroll / 2; ++u) {
+      x[u] = __builtin_nontemporal_load(val2 + pj + u * kBlock);
+      d[u] = __builtin_nontemporal_load(dloc2 + pj + u * kBlock);
     }
 #pragma unroll
-    for (int u = 0; u < kPbUnroll; ++u) atomicAdd(&acc[d[u]], to_fixed(x[u]));
+    for (int u = 0; u < kPbUnroll / 2; ++u) {
+      const uint64_t e = 2 * (pj + u * kBlock);
+      if (e >= lo) atomicAdd(&acc[d[u] & 0xFFFFu], to_fixed(x[u].x));
+      if (e + 1 < hi) atomicAdd(&acc[d[u] >> 16], to_fixed(x[u].y));
+    }
   }
-  for (; j < hi; j += kBlock) atomicAdd(&acc[dloc[j]], to_fixed(val[j]));
+  for (; pj < p1; pj += kBlock) {
+    const uint64_t e = 2 * pj;
+    if (e >= lo) atomicAdd(&acc[dloc[e]], to_fixed(val[e]));
+    if (e + 1 < hi) atomicAdd(&acc[dloc[e + 1]], to_fixed(val[e + 1]));
+  }
   __syncthreads();
   const uint64_t v0 = (uint64_t)t_bin[t] << kPbShift;
   if (t_multi[t]) {
@@ -1124,8 +1138,9 @@ int build_blocked(const DevGraph& g, uint64_t E, PrPlan* p, hipStream_t s) {
   DevBuf<uint32_t> src;
   CG_TRY(edge_sources(g.off, V, E, src, s));
   CG_TRY(p->pb_ent.ensure(E));
-  CG_TRY(p->pb_dloc.ensure(E));
-  CG_TRY(p->pb_val.ensure(E));
+  // + 2: the gather reads entries in aligned pairs, one past an odd end
+  CG_TRY(p->pb_dloc.ensure(E + 2));
+  CG_TRY(p->pb_val.ensure(E + 2));
   if (p->pb_t1)
     hipLaunchKernelGGL(k_pb_fill, dim3(p->pb_t1), dim3(kBlock), nb * 12, s, src.p, g.dst, p->pb_t1_chunk.p,
                        p->pb_t1_lo.p, p->pb_t1_hi.p, nb, reinterpret_cast<unsigned long long*>(in_base.p),
