@@ -126,8 +126,8 @@ def main():
             print(json.dumps({"profile_pr_ms": st["ms"]}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
-              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one MI355X; runs on more than one GPU are not implemented "
+              "(sharding across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

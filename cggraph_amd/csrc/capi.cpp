@@ -423,7 +423,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   if (use_device_num < 1)
     return set_error(CG_ERR_NO_DEVICE, "use_device_num = %d: the engine needs one GPU", use_device_num);
   if (use_device_num > 1)
-    return set_error(CG_ERR_UNSUPPORTED, "use_device_num = %d: this build drives one GPU", use_device_num);
+    return set_error(CG_ERR_UNSUPPORTED, "use_device_num = %d: libcgmi drives one GPU per engine", use_device_num);
   CG_TRY(require_device());
   cg_graph* g = const_cast<cg_graph*>(cg);
   if (algorithm == CG_SSSP && !g->has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");

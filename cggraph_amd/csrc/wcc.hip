@@ -1,4 +1,4 @@
-// WCC by min-label hooking on a symmetric graph (the north-star "CC").
+// WCC (weakly connected components) by min-label hooking on a symmetric graph.
 //
 // The reference computes CC by pushing labels: value[i] = i, then every
 // active vertex lowers its neighbours' labels with write_min / atomicMin

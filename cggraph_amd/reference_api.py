@@ -220,7 +220,7 @@ class TwoLevelPartition:
 
     def __init__(self, csrResult, zeroOutDegreeNum, deviceNum, algorithm):
         if deviceNum != 1:
-            raise CgError(_lib.CG_ERR_UNSUPPORTED, "this build drives one GPU")
+            raise CgError(_lib.CG_ERR_UNSUPPORTED, "libcgmi drives one GPU per engine")
         self.csr = csrResult
         self.algorithm = Algorithm_type(algorithm)
         w = csrResult.csr_weight if self.algorithm == Algorithm_type.SSSP else None

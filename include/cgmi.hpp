@@ -62,7 +62,7 @@ class TwoLevelPartition {
 
   TwoLevelPartition(const CSR_Result_type& csr, uint32_t zeroOutDegreeNum, size_t deviceNum,
                     Algorithm_type algorithm) {
-    if (deviceNum != 1) throw Error(CG_ERR_UNSUPPORTED, "this build drives one GPU");
+    if (deviceNum != 1) throw Error(CG_ERR_UNSUPPORTED, "libcgmi drives one GPU per engine");
     cg_graph* g = nullptr;
     const void* off = csr.csr_offset64 ? static_cast<const void*>(csr.csr_offset64) : csr.csr_offset;
     check(cg_graph_from_csr(csr.vertexNum, csr.edgeNum, off, csr.csr_offset64 != nullptr, csr.csr_dest,
