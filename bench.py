@@ -236,6 +236,16 @@ def run_learned(cg, g, root, tuned, ref_vals):
     res = {"ms": round(dt * 1e3, 4), "gteps": round(st["edges_traversed"] / dt / 1e9, 3),
            "iterations": st["iterations"], "learned_fallbacks": st["learned_fallbacks"],
            "light_iterations": st["light_iterations"], "bit_exact_vs_plain": same}
+    # the reference's own LIGHT / HEAVY rule (host team for light iterations
+    # and the small-degree tail), the plain heuristic the network replaces
+    eng = cg.Engine(g, 1, sched_mode=0)
+    hdt, hst = timed_runs(eng, root, 3, 1)
+    hsame = bool(np.array_equal(eng.result(), ref_vals))
+    eng.close()
+    res["heuristic"] = {"ms": round(hdt * 1e3, 4), "gteps": round(hst["edges_traversed"] / hdt / 1e9, 3),
+                        "iterations": hst["iterations"], "light_iterations": hst["light_iterations"],
+                        "bit_exact_vs_plain": hsame}
+    res["speedup_vs_heuristic"] = round(hdt / dt, 2)
     log(f"learned SSSP s24: {res}")
     return res
 
