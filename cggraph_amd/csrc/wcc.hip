@@ -84,16 +84,29 @@ __global__ void k_sample_roots(const uint32_t* __restrict__ parent, uint32_t V, 
   }
 }
 
-// Remaining edges of vertices outside the large component.  One wavefront
-// per vertex so hub rows spread over 64 lanes.
+// Remaining edges of vertices outside the large component.  A wavefront
+// takes 64 consecutive vertices, tests them one per lane (parent outside the
+// large component, edges left after the sampled ones), then links the rows
+// that need it one at a time with the row spread over the 64 lanes -- most
+// vertices are in the large component or have no edges left, so the test is
+// the bulk of the work and runs 64 wide.
 __global__ __launch_bounds__(kBlock) void k_link_rest(DevGraph g, uint32_t r, uint32_t skip,
                                                       uint32_t* __restrict__ parent) {
   const uint32_t lane = __lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
-  for (uint64_t v = blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64; v < g.V; v += waves) {
-    if (ld_relaxed(parent + v) == skip) continue;  // wave-uniform
-    const uint64_t s = g.off[v] + r, e = g.off[v + 1];
-    for (uint64_t k = s + lane; k < e; k += 64) link((uint32_t)v, g.dst[k], parent);
+  for (uint64_t base = (blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64) * 64; base < g.V;
+       base += waves * 64) {
+    const uint64_t v = base + lane;
+    uint64_t s = 0, e = 0;
+    if (v < g.V && ld_relaxed(parent + v) != skip) {
+      s = g.off[v] + r;
+      e = g.off[v + 1];
+    }
+    for (uint64_t need = __ballot(s < e); need; need &= need - 1) {
+      const int l = __builtin_ctzll(need);
+      const uint64_t rs = __shfl(s, l, 64), re = __shfl(e, l, 64);
+      for (uint64_t k = rs + lane; k < re; k += 64) link((uint32_t)(base + l), g.dst[k], parent);
+    }
   }
 }
 
@@ -130,7 +143,7 @@ int wcc_hooking(const DevGraph& g, uint32_t* parent, uint32_t sample_rounds, hip
     const uint32_t c = ++cnt[x];
     if (c > bc || (c == bc && x < best)) { bc = c; best = x; }
   }
-  const uint64_t waves_grid = std::min<uint64_t>(((uint64_t)V + 3) / 4, 65536);
+  const uint64_t waves_grid = std::min<uint64_t>(((uint64_t)V + 255) / 256, 65536);
   hipLaunchKernelGGL(k_link_rest, dim3((unsigned)(waves_grid ? waves_grid : 1)), dim3(kBlock), 0, s, g,
                      sample_rounds, best, parent);
   hipLaunchKernelGGL(k_compress, dim3(grid_of(V)), dim3(kBlock), 0, s, V, parent);
