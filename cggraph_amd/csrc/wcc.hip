@@ -15,7 +15,6 @@
 // link their remaining edges -- an edge between the large component and
 // another vertex is seen from the other side, because the graph is symmetric.
 #include <algorithm>
-#include <unordered_map>
 #include <vector>
 
 #include "kernels.hpp"
@@ -84,14 +83,45 @@ __global__ void k_sample_roots(const uint32_t* __restrict__ parent, uint32_t V, 
   }
 }
 
+// The most frequent root among the samples (ties to the smaller id), on the
+// device so the host never waits: one workgroup of kSamples threads, each
+// counting its sample's occurrences in LDS.  Any choice gives the same
+// labels; a good one lets most vertices skip their remaining edges.
+constexpr uint32_t kSamples = 1024;
+
+__global__ __launch_bounds__(kSamples) void k_sample_mode(const uint32_t* __restrict__ samp,
+                                                          uint32_t* __restrict__ mode) {
+  __shared__ uint32_t x[kSamples];
+  __shared__ unsigned long long best[kSamples / 64];
+  const uint32_t t = threadIdx.x;
+  x[t] = samp[t];
+  __syncthreads();
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < kSamples; ++j) c += x[j] == x[t];
+  // key: larger count first, then smaller id
+  unsigned long long key = ((unsigned long long)c << 32) | (0xFFFFFFFFu - x[t]);
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(key, d, 64);
+    key = o > key ? o : key;
+  }
+  if (__lane_id() == 0) best[t / 64] = key;
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long k = best[0];
+    for (uint32_t w = 1; w < kSamples / 64; ++w) k = best[w] > k ? best[w] : k;
+    *mode = 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFu);
+  }
+}
+
 // Remaining edges of vertices outside the large component.  A wavefront
 // takes 64 consecutive vertices, tests them one per lane (parent outside the
 // large component, edges left after the sampled ones), then links the rows
 // that need it one at a time with the row spread over the 64 lanes -- most
 // vertices are in the large component or have no edges left, so the test is
 // the bulk of the work and runs 64 wide.
-__global__ __launch_bounds__(kBlock) void k_link_rest(DevGraph g, uint32_t r, uint32_t skip,
+__global__ __launch_bounds__(kBlock) void k_link_rest(DevGraph g, uint32_t r, const uint32_t* __restrict__ mode,
                                                       uint32_t* __restrict__ parent) {
+  const uint32_t skip = *mode;
   const uint32_t lane = __lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
   for (uint64_t base = (blockIdx.x * (uint64_t)(kBlock / 64) + threadIdx.x / 64) * 64; base < g.V;
@@ -128,27 +158,20 @@ int wcc_hooking(const DevGraph& g, uint32_t* parent, uint32_t sample_rounds, hip
     hipLaunchKernelGGL(k_link_round, dim3(grid_of(V)), dim3(kBlock), 0, s, g, r, parent);
     hipLaunchKernelGGL(k_compress, dim3(grid_of(V)), dim3(kBlock), 0, s, V, parent);
   }
-  // most frequent root among 1024 samples
-  constexpr uint32_t kSamples = 1024;
+  // most frequent root among kSamples samples, found on the device
   DevBuf<uint32_t> samp;
-  CG_TRY(samp.ensure(kSamples));
+  CG_TRY(samp.ensure(kSamples + 1));
   hipLaunchKernelGGL(k_sample_roots, dim3(kSamples / kBlock), dim3(kBlock), 0, s, parent, V,
                      (uint64_t)0x5EEDull, kSamples, samp.p);
-  std::vector<uint32_t> h(kSamples);
-  CG_HIP(hipMemcpyAsync(h.data(), samp.p, kSamples * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  CG_HIP(hipStreamSynchronize(s));
-  std::unordered_map<uint32_t, uint32_t> cnt;
-  uint32_t best = h[0], bc = 0;
-  for (uint32_t x : h) {
-    const uint32_t c = ++cnt[x];
-    if (c > bc || (c == bc && x < best)) { bc = c; best = x; }
-  }
+  hipLaunchKernelGGL(k_sample_mode, dim3(1), dim3(kSamples), 0, s, samp.p, samp.p + kSamples);
   const uint64_t waves_grid = std::min<uint64_t>(((uint64_t)V + 255) / 256, 65536);
   hipLaunchKernelGGL(k_link_rest, dim3((unsigned)(waves_grid ? waves_grid : 1)), dim3(kBlock), 0, s, g,
-                     sample_rounds, best, parent);
+                     sample_rounds, samp.p + kSamples, parent);
   hipLaunchKernelGGL(k_compress, dim3(grid_of(V)), dim3(kBlock), 0, s, V, parent);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(CG_ERR_HIP, "WCC kernels: %s", hipGetErrorString(e));
+  // the sample buffer is freed on return: the kernels reading it end first
+  CG_HIP(hipStreamSynchronize(s));
   if (edges_relaxed) *edges_relaxed = 0;
   return CG_OK;
 }
