@@ -14,7 +14,8 @@
  *   tsan_check            run the three algorithms, compare each with the
  *                         sequential oracle, print "mismatches N" per algorithm
  *   tsan_check plant      additionally run a deliberate plain-int race, so the
- *                         test can show that this build of TSan reports races
+ *                         test can show that the sanitizer, set up this way,
+ *                         does report races
  */
 #include <omp.h>
 #include <stdio.h>
