@@ -1217,10 +1217,12 @@ int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, uint32_t
                    hipStream_t s) {
   if (kernel > 3) return set_error(CG_ERR_INVALID, "pr_kernel %u outside [0, 3]", kernel);
   if (layout > kLayoutBlocked) return set_error(CG_ERR_INVALID, "pr_layout %u outside [0, 4]", layout);
-  // automatic: propagation blocking from 2^23 vertices (measured on R-MAT:
-  // 10-20 % faster than the renumbered pull at 2^23-2^24, 27 % at 2^25 with
-  // edge factor 44; equal at 2^22, slower below) up to its 2^25 limit
-  if (layout == kLayoutAuto && seg_log2 == 0 && kernel == 0 && graph->V >= (1u << 23) &&
+  // automatic: propagation blocking from 2^20 vertices up to its 2^25 limit
+  // (measured per iteration on R-MAT edge factor 16 against the renumbered
+  // pull: 0.12 vs 0.13 ms at 2^20, 0.17 vs 0.22 at 2^21, 0.30 vs 0.43 at
+  // 2^22, 0.54 vs 0.93 at 2^23; at 2^19 and below the pull wins, 0.07 vs
+  // 0.09 ms, the blocked passes' fixed cost)
+  if (layout == kLayoutAuto && seg_log2 == 0 && kernel == 0 && graph->V >= (1u << 20) &&
       ((uint64_t)graph->V + kPbSize - 1) / kPbSize <= kPbMaxBins && graph->E > 0)
     layout = kLayoutBlocked;
   if (layout == kLayoutBlocked) {

@@ -10,6 +10,9 @@ do not need a full-size textbook run:
 * R-MAT scale 25 / ef44 PageRank: one float64 step from the engine's 19th
   iterate reproduces its 20th (L1 <= 1e-6), and the total rank mass equals
   (1 - d) + d * (mass on vertices with out-edges) of the previous iterate.
+* R-MAT scales 20 and 22 / ef16 PageRank under the automatic layout
+  (propagation blocking from 2^20 vertices): L1 <= 1e-6 against the FP64
+  oracle, as is the renumbered pull on the same graphs.
 The GPU generator's CSR is compared with the oracle's at scale 18 bit for bit.
 """
 import numpy as np
@@ -77,3 +80,20 @@ def test_pagerank_rmat25_ef44_step_consistency(oracle):
     deg = np.diff(og.offsets)
     mass = (1 - 0.85) + 0.85 * float(r19[deg > 0].astype(np.float64).sum())
     assert abs(float(r20.astype(np.float64).sum()) - mass) <= 1e-5
+
+
+@pytest.mark.parametrize("scale", [20, 22])
+def test_pagerank_automatic_blocking_mid_size(oracle, scale):
+    """From 2^20 vertices the automatic layout is propagation blocking: its
+    ranks are within FP32 rounding of the FP64 oracle and of the renumbered
+    pull's."""
+    g = cg.Graph.rmat(scale, 16, seed=3, weights=None)
+    e = cg.Engine(g, 3)
+    auto, _ = e.run()
+    e.close()
+    e = cg.Engine(g, 3, pr_layout=2)
+    pull, _ = e.run()
+    e.close()
+    ref = oracle.pagerank64(host_csr(oracle, g), 20, 0.85)
+    assert float(np.abs(auto.astype(np.float64) - ref).sum()) <= 1e-6
+    assert float(np.abs(pull.astype(np.float64) - ref).sum()) <= 1e-6
