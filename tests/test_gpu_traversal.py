@@ -249,9 +249,14 @@ def test_pagerank_blocked_builds(oracle):
     depend on the order of the additions: two engines (whose edge slots are
     placed by atomics, so their layouts may differ) give bit-identical ranks,
     on a graph whose size is not a multiple of the 8192-vertex blocks and
-    whose hot blocks are cut into several tiles."""
-    og = random_graph(oracle, 5, 20001, 600000, hub=300000)
-    og.weight = None
+    whose hot blocks are cut into several tiles (the first block receives
+    about 1.2 M in-edges: three gather tiles of at most 2^19 entries)."""
+    rng = np.random.default_rng(5)
+    m, hot = 600000, 1000000
+    src = np.concatenate([rng.integers(0, 20001, m), np.zeros(300000, np.int64), rng.integers(0, 20001, hot)])
+    dst = np.concatenate([rng.integers(0, 20001, m), rng.integers(0, 20001, 300000), rng.integers(0, 8192, hot)])
+    og = oracle.csr_build(20001, src.astype(np.uint32), dst.astype(np.uint32))
+    assert int(np.count_nonzero(og.dest < 8192)) > 2 * 2**19
     ref = oracle.pagerank64(og, 20, 0.85)
     g = cg.Graph.from_csr(og.offsets, og.dest)
     runs = []
