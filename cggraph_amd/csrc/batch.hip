@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "engine.hpp"
@@ -290,6 +291,33 @@ struct GraphExec {
   }
 };
 
+// Batched queries on a graph of at least 2^20 vertices whose top 1/64 of
+// vertices send at least half of the edges run on a copy renumbered by
+// descending out-degree (the rule an engine applies to its own graph): the
+// values those edges touch then share cache lines.  Built once per graph.
+constexpr uint32_t kBatchRelabelMinV = 1u << 20;
+std::mutex batch_relabel_mu;  // first uses of one graph from several threads
+
+static int build_batch_relabel(cg_graph* g) {
+  std::lock_guard<std::mutex> lock(batch_relabel_mu);
+  if (g->batch_relabel_done) return CG_OK;
+  g->batch_relabel_done = true;
+  DevBuf<uint32_t> perm, new_of;
+  double share = 0.0;
+  CG_TRY(degree_order(g, perm, new_of, &share));
+  if (share < 0.5) return CG_OK;
+  std::unique_ptr<cg_graph> rg(new (std::nothrow) cg_graph());
+  if (!rg) return set_error(CG_ERR_OOM, "renumbered graph object");
+  CG_TRY(relabel_graph(g, new_of.p, rg.get()));
+  std::vector<uint32_t> h(g->V);
+  CG_HIP(hipMemcpy(h.data(), new_of.p, (size_t)g->V * 4, hipMemcpyDeviceToHost));
+  CG_HIP(hipDeviceSynchronize());
+  g->batch_new_of_h = std::move(h);
+  g->batch_new_of = std::move(new_of);
+  g->batch_rg = std::move(rg);
+  return CG_OK;
+}
+
 int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots, int nq, uint32_t* const* outs,
                 const cg_options& opt, cg_run_stats* rs) {
   CG_TRY(require_device());
@@ -307,22 +335,37 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
     if (roots[i] >= graphs[i]->V) return set_error(CG_ERR_INVALID, "query %d: root %u >= V", i, roots[i]);
     CG_TRY(graphs[i]->ensure_device());
   }
+  // the graph each query runs on, its root there, and the map back
+  std::vector<cg_graph*> qgraph(graphs, graphs + nq);
+  std::vector<uint32_t> qroot(roots, roots + nq);
+  std::vector<const uint32_t*> back(nq, nullptr);
+  uint32_t back_v = 0;
+  for (int i = 0; i < nq; ++i) {
+    cg_graph* g = graphs[i];
+    if (g->V >= kBatchRelabelMinV) CG_TRY(build_batch_relabel(g));
+    if (g->batch_rg) {
+      qgraph[i] = g->batch_rg.get();
+      qroot[i] = g->batch_new_of_h[roots[i]];
+      back[i] = g->batch_new_of.p;
+      back_v = std::max(back_v, g->V);
+    }
+  }
   hipStream_t s = nullptr;
   CG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   std::unique_ptr<std::remove_pointer<hipStream_t>::type, decltype(&hipStreamDestroy)> sguard(s, hipStreamDestroy);
   // per-query device arrays: values, stamps and two frontier queues
   std::vector<uint64_t> vpre(nq + 1, 0);
-  for (int i = 0; i < nq; ++i) vpre[i + 1] = vpre[i] + graphs[i]->V;
+  for (int i = 0; i < nq; ++i) vpre[i + 1] = vpre[i] + qgraph[i]->V;
   const uint64_t totalV = vpre[nq];
   DevBuf<uint32_t> pool;
   CG_TRY(pool.ensure(4 * totalV));
   std::vector<QueryDesc> hq(nq);
   for (int i = 0; i < nq; ++i) {
-    cg_graph* g = graphs[i];
+    cg_graph* g = qgraph[i];
     uint32_t* b = pool.p + 4 * vpre[i];
     const uint32_t delta = algs[i] == CG_SSSP ? opt.sssp_delta : 0;
     hq[i] = QueryDesc{g->d_off.p, g->d_dst.p, g->has_w ? g->d_w.p : nullptr, b, b + g->V,
-                      {b + 2 * (uint64_t)g->V, b + 3 * (uint64_t)g->V}, g->V, algs[i], roots[i], delta};
+                      {b + 2 * (uint64_t)g->V, b + 3 * (uint64_t)g->V}, g->V, algs[i], qroot[i], delta};
   }
   DevBuf<QueryDesc> dq;
   DevBuf<uint64_t> dvpre;
@@ -378,9 +421,18 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
   BATCH_LAUNCH_OK();
   std::vector<unsigned long long> hacc(64 * 16);
   CG_HIP(hipMemcpyAsync(hacc.data(), acc.p, hacc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  for (int i = 0; i < nq; ++i)
-    if (outs && outs[i])
-      CG_HIP(hipMemcpyAsync(outs[i], hq[i].val, (size_t)hq[i].V * 4, hipMemcpyDeviceToHost, s));
+  DevBuf<uint32_t> unmapped;  // values of a renumbered query in the caller's ids
+  if (outs && back_v) CG_TRY(unmapped.ensure(back_v));
+  for (int i = 0; i < nq; ++i) {
+    if (!outs || !outs[i]) continue;
+    const uint32_t* src = hq[i].val;
+    if (back[i]) {
+      // stream order: the previous query's copy out of `unmapped` is done
+      CG_TRY(launch_renumber_back(back[i], hq[i].val, hq[i].V, unmapped.p, s));
+      src = unmapped.p;
+    }
+    CG_HIP(hipMemcpyAsync(outs[i], src, (size_t)hq[i].V * 4, hipMemcpyDeviceToHost, s));
+  }
   CG_HIP(hipStreamSynchronize(s));
   if (rs) {
     std::memset(rs, 0, sizeof(*rs));

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -141,6 +142,13 @@ struct cg_graph {
   bool dev_t_valid = false;
   cg::DevBuf<uint64_t> d_in_off;
   cg::DevBuf<uint32_t> d_in_src, d_in_w;
+  // batched queries: a copy renumbered by descending out-degree, built on
+  // first use when the graph is large and skewed (vertex batch_new_of[v] of
+  // the copy is the caller's v)
+  bool batch_relabel_done = false;
+  std::unique_ptr<cg_graph> batch_rg;
+  cg::DevBuf<uint32_t> batch_new_of;
+  std::vector<uint32_t> batch_new_of_h;
 
   int ensure_host();    // D2H when only the device copy exists
   int ensure_device();  // H2D when only the host copy exists

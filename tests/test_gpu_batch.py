@@ -92,3 +92,27 @@ def test_engine_handle_batch(oracle):
     pr = cg.Engine(gs[0], 3)
     with pytest.raises(cg.CgError):
         batch.run_engine_batch([engs[0], pr], [0, 0])
+
+
+def test_batch_renumbered_graph(oracle):
+    """A skewed graph of 2^20 vertices runs its batched queries on a copy
+    renumbered by out-degree (built on first use, kept with the graph); roots
+    map in and values map back, so the outputs and traversed-edge counts are
+    the oracle's, on the first batch and on a second one, mixed with a small
+    graph that is not renumbered."""
+    big = oracle.rmat_graph(20, 16, seed=4, weights=(1, 1000))
+    small = oracle.rmat_graph(12, 16, seed=4, weights=(1, 1000))
+    gb = cg.Graph.from_csr(big.offsets, big.dest, big.weight)
+    gs = cg.Graph.from_csr(small.offsets, small.dest, small.weight)
+    rng = np.random.default_rng(7)
+    roots = [oracle.max_outdeg_vertex(big)] + [int(x) for x in rng.integers(0, big.V, 3)]
+    for rep in range(2):
+        qg = [gb] * 4 + [gs] * 2
+        qo = [big] * 4 + [small] * 2
+        qa = [0, 1, 0, 1, 0, 1]
+        qr = (roots if rep == 0 else roots[::-1]) + [3, 5]
+        outs, st = batch.run_batch(qg, qa, qr, mode=batch.BATCHED)
+        for o, a, r, got in zip(qo, qa, qr, outs):
+            ref = oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)
+            assert np.array_equal(got, ref), (rep, a, r)
+        assert st["edges_traversed"] == sum(oracle.reached_edges(o, x) for o, x in zip(qo, outs))
