@@ -123,7 +123,10 @@ uint32_t bits_for(uint64_t V) {
   return b;
 }
 
-__device__ __forceinline__ uint32_t ld_stream(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+// In-edge source ids, read once per sweep.  Ordinary (cached) loads: the
+// non-temporal form measured 25 % slower per iteration at 2^19 vertices and
+// 2-3 % at 2^25 (R-MAT, renumbered pull).
+__device__ __forceinline__ uint32_t ld_src(const uint32_t* p) { return *p; }
 
 
 __global__ void k_out_degree(const uint64_t* __restrict__ off, uint32_t V, uint32_t* __restrict__ deg) {
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_low(Csc g, const float* __restric
   row_range<P>(g, (uint32_t)v, s, e);
   if (e - s >= kMed) return;
   float sum = 0.0f;
-  for (uint64_t k = s; k < e; ++k) sum += contrib_in[ld_stream(g.src + k)];
+  for (uint64_t k = s; k < e; ++k) sum += contrib_in[ld_src(g.src + k)];
   emit<P>(g, (uint32_t)v, sum, base, damping, part, rank, contrib_out);
 }
 
@@ -191,13 +194,13 @@ __device__ __forceinline__ float strided_sum(const uint32_t* __restrict__ src, c
     uint32_t u[U];
     float c[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) u[j] = ld_stream(src + k + j * STRIDE);
+    for (int j = 0; j < U; ++j) u[j] = ld_src(src + k + j * STRIDE);
 #pragma unroll
     for (int j = 0; j < U; ++j) c[j] = contrib[u[j]];
 #pragma unroll
     for (int j = 0; j < U; ++j) sum += c[j];
   }
-  for (; k < e; k += STRIDE) sum += contrib[ld_stream(src + k)];
+  for (; k < e; k += STRIDE) sum += contrib[ld_src(src + k)];
   return sum;
 }
 
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pr_tile(Csc g, uint64_t E, con
   float c[kTileK];
   if (nk == (uint32_t)kTileK) {
     const u32x4* p = reinterpret_cast<const u32x4*>(g.src + e0);
-    const u32x4 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
+    const u32x4 a = p[0], b = p[1];
     c[0] = contrib_in[a.x];
     c[1] = contrib_in[a.y];
     c[2] = contrib_in[a.z];
@@ -1219,12 +1222,12 @@ int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, uint32_t
                    hipStream_t s) {
   if (kernel > 3) return set_error(CG_ERR_INVALID, "pr_kernel %u outside [0, 3]", kernel);
   if (layout > kLayoutBlocked) return set_error(CG_ERR_INVALID, "pr_layout %u outside [0, 4]", layout);
-  // automatic: propagation blocking from 2^20 vertices up to its 2^25 limit
+  // automatic: propagation blocking from 2^22 vertices up to its 2^25 limit
   // (measured per iteration on R-MAT edge factor 16 against the renumbered
-  // pull: 0.12 vs 0.13 ms at 2^20, 0.17 vs 0.22 at 2^21, 0.30 vs 0.43 at
-  // 2^22, 0.54 vs 0.93 at 2^23; at 2^19 and below the pull wins, 0.07 vs
-  // 0.09 ms, the blocked passes' fixed cost)
-  if (layout == kLayoutAuto && seg_log2 == 0 && kernel == 0 && graph->V >= (1u << 20) &&
+  // pull: 0.30-0.35 vs 0.39-0.43 ms at 2^22, 0.54 vs 0.93 at 2^23; at 2^21
+  // the two are within box-to-box spread, 0.17-0.20 vs 0.17 ms, and below it
+  // the pull wins: 0.10 vs 0.12-0.16 ms at 2^20)
+  if (layout == kLayoutAuto && seg_log2 == 0 && kernel == 0 && graph->V >= (1u << 22) &&
       ((uint64_t)graph->V + kPbSize - 1) / kPbSize <= kPbMaxBins && graph->E > 0)
     layout = kLayoutBlocked;
   if (layout == kLayoutBlocked) {

@@ -117,7 +117,7 @@ typedef struct cg_options {
                                      (out-edges binned by 8192-vertex destination blocks, sums in
                                      64-bit fixed point; at most 2^25 vertices).  Automatic: a
                                      window size in [1, 30] selects 3, 31 selects 1; 0 selects 4
-                                     from 2^20 to 2^25 vertices (with pr_kernel 0), else 2 above
+                                     from 2^22 to 2^25 vertices (with pr_kernel 0), else 2 above
                                      65 536 vertices */
   uint32_t relabel;               /* BFS / SSSP: run on an engine-owned copy renumbered by
                                      descending out-degree (results come back in the caller's

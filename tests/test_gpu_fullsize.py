@@ -10,9 +10,9 @@ do not need a full-size textbook run:
 * R-MAT scale 25 / ef44 PageRank: one float64 step from the engine's 19th
   iterate reproduces its 20th (L1 <= 1e-6), and the total rank mass equals
   (1 - d) + d * (mass on vertices with out-edges) of the previous iterate.
-* R-MAT scales 20 and 22 / ef16 PageRank under the automatic layout
-  (propagation blocking from 2^20 vertices): L1 <= 1e-6 against the FP64
-  oracle, as is the renumbered pull on the same graphs.
+* R-MAT scales 20 and 22 / ef16 PageRank on either side of the automatic
+  switch to propagation blocking (2^22 vertices): L1 <= 1e-6 against the
+  FP64 oracle for the automatic layout and for the renumbered pull.
 The GPU generator's CSR is compared with the oracle's at scale 18 bit for bit.
 """
 import numpy as np
@@ -84,9 +84,9 @@ def test_pagerank_rmat25_ef44_step_consistency(oracle):
 
 @pytest.mark.parametrize("scale", [20, 22])
 def test_pagerank_automatic_blocking_mid_size(oracle, scale):
-    """From 2^20 vertices the automatic layout is propagation blocking: its
-    ranks are within FP32 rounding of the FP64 oracle and of the renumbered
-    pull's."""
+    """The automatic layout (the renumbered pull below 2^22 vertices,
+    propagation blocking from there) and the renumbered pull: both within
+    FP32 rounding of the FP64 oracle."""
     g = cg.Graph.rmat(scale, 16, seed=3, weights=None)
     e = cg.Engine(g, 3)
     auto, _ = e.run()
