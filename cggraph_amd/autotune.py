@@ -30,6 +30,7 @@ ENGINE_SPACE = {
     "hub_delta": [1, 2, 3, 5, 8],
     "relabel": [0, 1],
     "atomic_return": [1, 0],
+    "expand_grid": [0, 4096, 8192, 16384],
 }
 BATCH_SPACE = {
     "sssp_delta": [1000, 3000, 10000, 30000, 100000],
