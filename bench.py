@@ -43,6 +43,38 @@ def gather_ceiling():
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def host_cores():
+    """The host cores this process may use: its CPU affinity, capped by a
+    cgroup CPU quota when one is set, and the socket count."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    sockets = set()
+    base = "/sys/devices/system/cpu"
+    for c in (os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else range(nproc)):
+        try:
+            with open(f"{base}/cpu{c}/topology/physical_package_id") as f:
+                sockets.add(f.read().strip())
+        except OSError:
+            pass
+    usable = min(nproc, quota) if quota else nproc
+    return {"nproc": nproc, "os_cpu_count": os.cpu_count(), "cgroup_cpu_quota": quota,
+            "sockets": len(sockets) or 1, "usable": usable}
+
+
+def trimmed_mean(times):
+    """The reference's run average: sorted, min and max dropped when there
+    are more than two runs (main_partition.hpp:209-227)."""
+    t = sorted(times)
+    return sum(t[1:-1]) / (len(t) - 2) if len(t) > 2 else sum(t) / len(t)
+
+
 def timed_runs(eng, root, steps, warmup):
     for _ in range(warmup):
         eng.run(root, copy_out=False)
@@ -102,7 +134,10 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (default: every host core this process may use)")
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="CPU baseline runs; min and max are dropped from the mean when more than two")
     ap.add_argument("--profile-batch", action="store_true",
                     help="run the batched mixed workload once (for rocprofv3 kernel traces)")
     ap.add_argument("--profile-pr", action="store_true",
@@ -191,15 +226,24 @@ def main():
         from oracle import oracle as orc
         off, dst, w = g.export()
         og = orc.CSR(off, dst, w)
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        vals, ms, iters = orc.multicore_run(1, og, root, threads)
-        parity = bool(np.array_equal(vals, gpu_vals))
+        hc = host_cores()
+        threads = args.cpu_threads or hc["usable"]
+        times, parity = [], True
+        for _ in range(max(1, args.cpu_runs)):
+            vals, ms, iters = orc.multicore_run(1, og, root, threads)
+            times.append(ms)
+            parity = parity and bool(np.array_equal(vals, gpu_vals))
+        ms = trimmed_mean(times)
         cpu_m = orc.reached_edges(og, vals)
         cpu = {"value": round(cpu_m / (ms * 1e-3) / 1e9, 4), "unit": "GTEPS", "cores": threads,
                "kind": "port",
-               "sample": f"one SSSP query on the same R-MAT s{args.scale} graph, restated reference "
-                         f"MULTI_CORE engine (Graph_Numa + threeStage_taskSteal), {iters} iterations, "
-                         f"{ms:.0f} ms", "bit_exact_vs_gpu": parity}
+               "sample": f"{len(times)} SSSP queries from the bench root on the same R-MAT s{args.scale} graph "
+                         f"(restated reference MULTI_CORE engine: Graph_Numa + threeStage_taskSteal, one socket), "
+                         f"{iters} iterations each; value from the mean of the runs without the fastest and "
+                         f"slowest", "ms_per_query": round(ms, 2), "run_ms": [round(t, 2) for t in times],
+               "omp_max_threads": orc.omp_max_threads(), "nproc": hc["nproc"],
+               "os_cpu_count": hc["os_cpu_count"], "cgroup_cpu_quota": hc["cgroup_cpu_quota"],
+               "sockets": hc["sockets"], "bit_exact_vs_gpu": parity}
         log(f"CPU baseline: {cpu}")
 
     extra = {"learned_sssp_s24": learned, "gather_ceiling": ceiling}

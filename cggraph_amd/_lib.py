@@ -25,6 +25,7 @@ CG_NUM_FEATURES = 12
 
 CG_SCHED_HEURISTIC, CG_SCHED_GPU, CG_SCHED_LEARNED, CG_SCHED_FORCED, CG_SCHED_PROBE = 0, 1, 2, 3, 4
 MLP_PARAMS = 6032
+CG_NOALLOC_SCRATCH_BYTES = 64 << 20
 CG_FAULT_HIP_ERROR, CG_FAULT_MLP_NAN, CG_FAULT_SLOW_GPU = 1, 2, 4
 
 
@@ -123,6 +124,8 @@ SIGNATURES = {
     "cg_batch_run": (_int, [_vp, _vp, _vp, _int, _vp, _int, ctypes.POINTER(Options),
                             ctypes.POINTER(RunStats)]),
     "cg_engine_run_batch": (_int, [_vp, _vp, _int, _vp, ctypes.POINTER(RunStats)]),
+    "cg_traverse_device_noalloc": (_int, [_int, _vp, _vp, _vp, _u32, _u64, _u32, _vp, _vp,
+                                          ctypes.POINTER(_u32)]),
     "cg_mlp_train": (_int, [_cp, _cp, _int, _flt, _u64, ctypes.POINTER(_flt)]),
     "cg_mlp_forward": (_int, [_vp, _vp, _int, _vp]),
     "cg_mlp_loss_grad": (_int, [_vp, _vp, _vp, _int, _vp, _vp]),

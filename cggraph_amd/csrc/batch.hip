@@ -294,18 +294,23 @@ struct GraphExec {
 // Batched queries on a graph of at least 2^20 vertices whose top 1/64 of
 // vertices send at least half of the edges run on a copy renumbered by
 // descending out-degree (the rule an engine applies to its own graph): the
-// values those edges touch then share cache lines.  Built once per graph.
+// values those edges touch then share cache lines.  Built once per graph, on
+// its first batched use; the copy (CSR, weights and the id map, about the
+// size of the graph again) stays in HBM until the graph is destroyed.  A
+// failed build is reported by that batch and tried again by the next one.
 constexpr uint32_t kBatchRelabelMinV = 1u << 20;
 std::mutex batch_relabel_mu;  // first uses of one graph from several threads
 
 static int build_batch_relabel(cg_graph* g) {
   std::lock_guard<std::mutex> lock(batch_relabel_mu);
   if (g->batch_relabel_done) return CG_OK;
-  g->batch_relabel_done = true;
   DevBuf<uint32_t> perm, new_of;
   double share = 0.0;
   CG_TRY(degree_order(g, perm, new_of, &share));
-  if (share < 0.5) return CG_OK;
+  if (share < 0.5) {
+    g->batch_relabel_done = true;  // not skewed: the batch runs on the graph itself
+    return CG_OK;
+  }
   std::unique_ptr<cg_graph> rg(new (std::nothrow) cg_graph());
   if (!rg) return set_error(CG_ERR_OOM, "renumbered graph object");
   CG_TRY(relabel_graph(g, new_of.p, rg.get()));
@@ -315,6 +320,7 @@ static int build_batch_relabel(cg_graph* g) {
   g->batch_new_of_h = std::move(h);
   g->batch_new_of = std::move(new_of);
   g->batch_rg = std::move(rg);
+  g->batch_relabel_done = true;
   return CG_OK;
 }
 
@@ -357,6 +363,10 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
   std::vector<uint64_t> vpre(nq + 1, 0);
   for (int i = 0; i < nq; ++i) vpre[i + 1] = vpre[i] + qgraph[i]->V;
   const uint64_t totalV = vpre[nq];
+  // the batch indexes its items (vertex slots of all queries) with 32 bits
+  if (totalV > 0xFFFFFFFFull)
+    return set_error(CG_ERR_INVALID, "the batch holds %llu vertex slots over its queries; at most 2^32 - 1",
+                     (unsigned long long)totalV);
   DevBuf<uint32_t> pool;
   CG_TRY(pool.ensure(4 * totalV));
   std::vector<QueryDesc> hq(nq);
@@ -502,6 +512,8 @@ extern "C" int cg_engine_run_batch(cg_engine* const* engines, const uint32_t* ro
   std::vector<uint32_t*> o(nq, nullptr);
   for (int i = 0; i < nq; ++i) {
     if (!engines[i]) return cg::set_error(CG_ERR_INVALID, "engine %d is NULL", i);
+    if (engines[i]->host_only)
+      return cg::set_error(CG_ERR_UNSUPPORTED, "engine %d runs without a GPU; batched queries run on the GPU", i);
     if (engines[i]->alg != CG_BFS && engines[i]->alg != CG_SSSP)
       return cg::set_error(CG_ERR_UNSUPPORTED, "engine %d: batched queries are BFS or SSSP", i);
     graphs[i] = engines[i]->caller_g;

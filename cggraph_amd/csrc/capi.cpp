@@ -416,14 +416,47 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   clear_error();
   if (!cg || !out) return set_error(CG_ERR_INVALID, "NULL argument");
   if (algorithm < CG_BFS || algorithm > CG_PR) return set_error(CG_ERR_INVALID, "unknown algorithm %d", algorithm);
-  if (engine_type != CG_COORDINATION && engine_type != CG_SINGLE_GPU)
+  if (engine_type != CG_COORDINATION && engine_type != CG_SINGLE_GPU && engine_type != CG_MULTI_CORE &&
+      engine_type != CG_SINGLE_CORE)
     return set_error(CG_ERR_UNSUPPORTED,
-                     "engine type %d is not provided; this library runs COORDINATION (0) and SINGLE_GPU (4)",
+                     "engine type %d is not provided; this library runs COORDINATION (0), SINGLE_CORE (2), "
+                     "MULTI_CORE (3) and SINGLE_GPU (4)",
                      engine_type);
-  if (use_device_num < 1)
-    return set_error(CG_ERR_NO_DEVICE, "use_device_num = %d: the engine needs one GPU", use_device_num);
+  if (use_device_num < 0) return set_error(CG_ERR_INVALID, "use_device_num = %d", use_device_num);
   if (use_device_num > 1)
     return set_error(CG_ERR_UNSUPPORTED, "use_device_num = %d: libcgmi drives one GPU per engine", use_device_num);
+  // Without a GPU (use_device_num 0, or a CPU engine type) every iteration
+  // runs on the host team; the caller asked for it, so this is not a fallback:
+  // a GPU engine on a machine without one still fails with CG_ERR_NO_DEVICE.
+  const bool host_only = engine_type == CG_MULTI_CORE || engine_type == CG_SINGLE_CORE || use_device_num == 0;
+  if (host_only) {
+    if (engine_type == CG_SINGLE_GPU)
+      return set_error(CG_ERR_INVALID, "SINGLE_GPU with use_device_num = 0");
+    if (algorithm == CG_PR)
+      return set_error(CG_ERR_UNSUPPORTED, "PageRank runs on the GPU engines only");
+    cg_graph* g = const_cast<cg_graph*>(cg);
+    if (algorithm == CG_SSSP && !g->has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
+    CG_TRY(g->ensure_host());
+    std::unique_ptr<cg_engine> e(new (std::nothrow) cg_engine());
+    if (!e) return set_error(CG_ERR_OOM, "engine object");
+    e->g = g;
+    e->caller_g = g;
+    e->alg = algorithm;
+    e->engine_type = engine_type;
+    e->use_device = 0;
+    e->host_only = true;
+    if (opt) e->opt = *opt;
+    else cg_options_default(&e->opt);
+    if (e->opt.trace_path) { e->trace_path = e->opt.trace_path; e->opt.trace_path = e->trace_path.c_str(); }
+    e->opt.mlp_path = nullptr;
+    if (engine_type == CG_SINGLE_CORE) e->opt.cpu_threads = 1;
+    if (e->opt.trace_path) {
+      e->trace = std::fopen(e->opt.trace_path, "a");
+      if (!e->trace) return set_error(CG_ERR_IO, "cannot open trace file %s", e->opt.trace_path);
+    }
+    *out = e.release();
+    return CG_OK;
+  }
   CG_TRY(require_device());
   cg_graph* g = const_cast<cg_graph*>(cg);
   if (algorithm == CG_SSSP && !g->has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
@@ -455,6 +488,8 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   CG_HIP(hipEventCreateWithFlags(&e->ev_copy, hipEventDisableTiming));
   for (auto& pr : e->ev_pair)
     for (auto& ev : pr) CG_HIP(hipEventCreate(&ev));
+  for (auto& ev : e->ev_piece) CG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  for (auto& ev : e->ev_pcie) CG_HIP(hipEventCreate(&ev));
   if (e->opt.relabel > 2) return set_error(CG_ERR_INVALID, "relabel %u outside [0, 2]", e->opt.relabel);
   if (e->opt.hub_degree && e->opt.sssp_delta)
     return set_error(CG_ERR_INVALID, "hub_degree and sssp_delta are alternatives; set one");
@@ -524,10 +559,18 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
   cg_graph& g = *e->g;
   if (e->alg != CG_CC && e->alg != CG_PR && root >= g.V)
     return set_error(CG_ERR_INVALID, "root %u >= V = %u", root, g.V);
-  CG_HIP(hipSetDevice(0));
   e->result_valid = false;
   cg_run_stats rs{};
   int st = CG_OK;
+  if (e->host_only) {
+    CG_TRY(engine_run_host(e, root, &rs));
+    e->result_valid = true;
+    e->last_root = root;
+    if (stats) *stats = rs;
+    if (out) return cg_engine_result(e, out);
+    return CG_OK;
+  }
+  CG_HIP(hipSetDevice(0));
   if (e->alg == CG_PR) {
     DevGraph dg{g.d_off.p, g.d_dst.p, nullptr, g.d_in_off.p, g.d_in_src.p, g.V};
     const auto t0 = std::chrono::steady_clock::now();
@@ -590,6 +633,10 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
 int cg_engine_result(cg_engine* e, void* out) {
   if (!e || !out) return set_error(CG_ERR_INVALID, "NULL argument");
   if (!e->result_valid) return set_error(CG_ERR_INVALID, "no completed run to read");
+  if (e->host_only) {
+    std::memcpy(out, e->team.shadow.data(), (size_t)e->g->V * 4);
+    return CG_OK;
+  }
   const void* src = e->alg == CG_PR ? (const void*)e->rank.p
                     : e->renumbered   ? (const void*)e->val_out.p
                                       : (const void*)e->val.p;
@@ -599,6 +646,7 @@ int cg_engine_result(cg_engine* e, void* out) {
 
 int cg_engine_result_device(cg_engine* e, void** dev_ptr) {
   if (!e || !dev_ptr) return set_error(CG_ERR_INVALID, "NULL argument");
+  if (e->host_only) return set_error(CG_ERR_UNSUPPORTED, "an engine without a GPU keeps its values in host memory");
   *dev_ptr = e->alg == CG_PR ? (void*)e->rank.p : e->renumbered ? (void*)e->val_out.p : (void*)e->val.p;
   return CG_OK;
 }

@@ -29,6 +29,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <new>
+#include <string>
 
 namespace cg {
 
@@ -49,8 +51,10 @@ int ensure_team(cg_engine* e) {
 }
 
 // Reference heuristic (CGgraph_engine.cuh:312-348).
+// V' is the caller's: a renumbered copy puts every zero-out-degree vertex at
+// the end, which can be more than the caller's declared tail.
 bool heuristic_heavy(const cg_engine* e, const DevStats& s) {
-  const uint64_t Vp = (uint64_t)e->g->V - e->g->zero_tail;
+  const uint64_t Vp = (uint64_t)e->caller_g->V - e->caller_g->zero_tail;
   if (s.n_active == 1) return false;
   if (s.n_active == Vp && e->g->E >= e->opt.heavy_active_edges) return true;
   return s.n_active >= e->opt.heavy_active_vertices && s.m_active >= e->opt.heavy_active_edges;
@@ -101,19 +105,26 @@ void write_probe(cg_engine* e, const IterRecord& r, uint32_t root, const double*
   std::fprintf(e->trace, "]}\n");
 }
 
+int apply_host_candidates(cg_engine* e, uint64_t nc, uint64_t* out, cg_run_stats* rs);
+
 // Relaxes the host team's share (ids/vals already in pinned memory) and
 // applies its candidates on the GPU.
 int host_relax_and_apply(cg_engine* e, uint32_t n, uint64_t* out, cg_run_stats* rs) {
-  hipStream_t s = e->stream;
   uint64_t edges = 0;
   const auto th = clk::now();
-  const uint64_t nc = e->team.relax(e->alg, e->g, e->p_ids.p, e->p_vals.p, nullptr, nullptr, n,
-                                    e->cand_id, e->cand_val, &edges);
+  const uint64_t nc = e->team.relax(e->alg, e->g, e->p_ids.p, e->p_vals.p, n, e->cand_id, e->cand_val, &edges);
   const double sec = ms_since(th) * 1e-3;
   if (edges && sec > 0) e->rate_cpu = 0.7 * e->rate_cpu + 0.3 * ((double)edges / sec);
   rs->edges_relaxed += edges;
   rs->cpu_vertices += n;
   rs->bytes_d2h += 8ull * n;
+  return apply_host_candidates(e, nc, out, rs);
+}
+
+// Copies the host team's candidates (e->cand_id / cand_val) to the GPU and
+// min-applies them.
+int apply_host_candidates(cg_engine* e, uint64_t nc, uint64_t* out, cg_run_stats* rs) {
+  hipStream_t s = e->stream;
   if (!nc) return CG_OK;
   if (nc > 0xFFFFFFFFull) return set_error(CG_ERR_INTERNAL, "host candidate list too long");
   CG_TRY(e->d_cand_id.ensure(nc));
@@ -185,6 +196,116 @@ struct Step {
   uint32_t threshold;  // delta-stepping: GPU pushes expand only values below it
 };
 
+// A HEAVY iteration shared with the host team (CGgraph_engine.cuh:368-855,
+// 1499-1650).  The compaction has written the degree-tier queues.  The host
+// receives the tail of the small tier -- twice the share that the
+// COOPERATION engine's split predicts, so a slow GPU can be relieved --
+// and this thread becomes the GPU agent: it claims pieces of the small tier
+// from the head, half of what is left first and shrinking to a floor
+// (GPU_WORK_GRANULARITY / 2^k, :1550), and launches one expansion per piece
+// with at most two in flight, while the other threads of the team claim
+// 64-entry pieces from the tail (the CPU steal loop, :692-855).  Neither
+// share is fixed before the launch: the two ends meet wherever the faster
+// side got to.  The host's candidates are applied on the GPU afterwards.
+int co_execute(cg_engine* e, const DevGraph& dg, const Bins& bins, const DevStats& sst, DevStats* dst,
+               uint64_t* out, cg_run_stats* local) {
+  hipStream_t s = e->stream;
+  const int alg = e->alg;
+  CG_HIP(hipMemcpyAsync(e->hst.p + 2, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  const DevStats q = e->hst.p[2];
+  const uint32_t n_small = q.n_small;
+  // GPU share of the frontier's edges W (GraphCooperation.cuh:1076-1086):
+  // alpha = (W - t_pcie * R_cpu) / (W * (1 + R_cpu / R_gpu))
+  double rg = e->rate_gpu;
+  if (e->opt.fault_flags & CG_FAULT_SLOW_GPU) rg = e->rate_cpu * 0.25;
+  const double W = (double)sst.m_active;
+  double alpha = W > 0 ? (W - e->pcie_ms * 1e-3 * e->rate_cpu) / (W * (1.0 + e->rate_cpu / rg)) : 1.0;
+  alpha = std::min(1.0, std::max(0.0, alpha));
+  const int T = e->team.threads;
+  uint32_t cap = (uint32_t)std::min<double>(n_small, std::ceil(2.0 * (1.0 - alpha) * n_small));
+  cap = std::min<uint32_t>(n_small, std::max<uint32_t>(cap, (uint32_t)(kVertexStealChunk * T)));
+  const uint32_t floor_ = n_small - cap;
+  CG_HIP(hipEventRecord(e->ev_pcie[0], s));
+  CG_TRY(queue_share_copy(e, floor_, cap));
+  CG_HIP(hipEventRecord(e->ev_pcie[1], s));
+  QueueEnds ends;
+  ends.reset(0, n_small, floor_);
+  e->team.clear_candidates();
+  const uint32_t min_piece = std::max<uint32_t>(4096u, n_small / 64);
+  uint64_t host_entries = 0, host_edges = 0;
+  int agent_status = CG_OK;
+  std::string agent_error;
+  uint32_t pieces = 0;
+  double gpu_sec = 0, host_sec = 0;
+#pragma omp parallel num_threads(T + 1)
+  {
+    const int tid = omp_get_thread_num();
+    if (tid == 0) {
+      // the GPU agent
+      const auto t0 = clk::now();
+      uint32_t want = std::max<uint32_t>(n_small / 2, min_piece);
+      uint32_t lo = 0, hi = 0;
+      bool first = true;
+      for (;;) {
+        const bool got = ends.claim_head(want, &lo, &hi);
+        if (!got && !first) break;
+        if (!got) lo = hi = 0;  // the big tiers still run
+        if (pieces >= 2 && hipEventSynchronize(e->ev_piece[pieces & 1]) != hipSuccess) {
+          agent_status = set_error(CG_ERR_HIP, "expansion piece failed");
+          break;
+        }
+        const uint64_t items = (first ? (uint64_t)q.n_chunk + q.n_med : 0) + (hi - lo + kWave - 1) / kWave + 1;
+        int st = launch_expand_push(alg, dg, e->val.p, out, e->q_small.p, e->q_med.p, e->q_chunk.p, items,
+                                    SmallRange{lo, hi, first ? 1u : 0u}, bins, dst, e->stripes.p,
+                                    e->opt.atomic_return != 0, s);
+        if (st == CG_OK && hipEventRecord(e->ev_piece[pieces & 1], s) != hipSuccess)
+          st = set_error(CG_ERR_HIP, "hipEventRecord failed");
+        if (st != CG_OK) {
+          agent_status = st;
+          agent_error = last_error();
+          break;
+        }
+        ++pieces;
+        first = false;
+        if (!got) break;
+        want = std::max<uint32_t>(want / 2, min_piece);
+      }
+      if (agent_status == CG_OK && hipStreamSynchronize(s) != hipSuccess)
+        agent_status = set_error(CG_ERR_HIP, "expansion pieces failed");
+      gpu_sec = ms_since(t0) * 1e-3;
+    } else {
+      const auto t0 = clk::now();
+      if (hipEventSynchronize(e->ev_copy) == hipSuccess) {
+        uint64_t ed = 0, en = 0;
+        e->team.tail_worker(tid - 1, alg, e->g, e->p_ids.p, e->p_vals.p, &ends, &ed, &en);
+#pragma omp atomic
+        host_entries += en;
+#pragma omp atomic
+        host_edges += ed;
+      }
+      const double sec = ms_since(t0) * 1e-3;
+#pragma omp critical
+      host_sec = std::max(host_sec, sec);
+    }
+  }
+  if (agent_status != CG_OK) return set_error(agent_status, "%s", agent_error.empty() ? last_error() : agent_error.c_str());
+  local->expand_launches += pieces;
+  local->gpu_vertices += sst.n_active - host_entries;
+  local->cpu_vertices += host_entries;
+  local->edges_relaxed += sst.m_active;
+  local->bytes_d2h += 8ull * cap;
+  float pms = 0.f;
+  if (hipEventElapsedTime(&pms, e->ev_pcie[0], e->ev_pcie[1]) == hipSuccess) e->pcie_ms = pms;
+  else (void)hipGetLastError();
+  // measured rates (edges/s) of both sides feed the next split
+  const double gpu_edges = W - (double)host_edges;
+  if (gpu_sec > 0 && gpu_edges > 0) e->rate_gpu = 0.7 * e->rate_gpu + 0.3 * (gpu_edges / gpu_sec);
+  if (host_sec > 0 && host_edges > 0) e->rate_cpu = 0.7 * e->rate_cpu + 0.3 * ((double)host_edges / host_sec);
+  const uint64_t nc = e->team.gather(e->cand_id, e->cand_val);
+  return apply_host_candidates(e, nc, out, local);
+}
+
 // Executes one iteration's expansion from frontier `in` into `out`.
 int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats& sst, DevStats* dst,
               uint64_t* in, uint64_t* out, bool* timing_pending, cg_run_stats* local) {
@@ -238,38 +359,17 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
   }
   CG_TRY(launch_compact(in, words, dg, bins, e->q_small.p, e->q_med.p, e->q_chunk.p, dst, tval,
                         step.threshold, out, s));
-  uint32_t small_limit = 0xFFFFFFFFu, n_host = 0;
-  if (step.host_share == 2) {
-    // Split the small tier by measured throughput (the analytical split of
-    // GraphCooperation.cuh:1036-1086): host share of the edges
-    // R_cpu / (R_cpu + R_gpu), taken from the queue tail.
-    CG_HIP(hipMemcpyAsync(e->hst.p + 2, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
-    CG_HIP(hipStreamSynchronize(s));
-    const uint32_t n_small = e->hst.p[2].n_small;
-    double rg = e->rate_gpu;
-    if (e->opt.fault_flags & CG_FAULT_SLOW_GPU) rg = e->rate_cpu * 0.25;
-    const double frac = e->rate_cpu / (e->rate_cpu + rg);
-    n_host = (uint32_t)std::min<double>(n_small, std::floor(frac * n_small));
-    small_limit = n_small - n_host;
-    if (n_host) CG_TRY(queue_share_copy(e, small_limit, n_host));
-  }
-  const uint64_t bound = expand_item_bound(sst.n_active, sst.m_active, bins);
-  if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
-  const auto tg = clk::now();
-  CG_TRY(launch_expand_push(alg, dg, e->val.p, out, e->q_small.p, e->q_med.p, e->q_chunk.p, bound,
-                            small_limit, bins, dst, e->stripes.p, e->opt.atomic_return != 0, s));
-  if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); *timing_pending = true; }
-  ++local->expand_launches;
   local->expand_alg_bytes += 20.0 * (sst.n_active - sst.n_zero_deg) +
                              (alg == CG_SSSP ? 12.0 : 8.0) * (double)sst.m_active;
-  local->gpu_vertices += sst.n_active - n_host;
+  if (step.host_share == 2) return co_execute(e, dg, bins, sst, dst, out, local);
+  const uint64_t bound = expand_item_bound(sst.n_active, sst.m_active, bins);
+  if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
+  CG_TRY(launch_expand_push(alg, dg, e->val.p, out, e->q_small.p, e->q_med.p, e->q_chunk.p, bound,
+                            SmallRange{}, bins, dst, e->stripes.p, e->opt.atomic_return != 0, s));
+  if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); *timing_pending = true; }
+  ++local->expand_launches;
+  local->gpu_vertices += sst.n_active;
   local->edges_relaxed += sst.m_active;
-  if (n_host) {
-    CG_HIP(hipEventSynchronize(e->ev_copy));
-    CG_TRY(host_relax_and_apply(e, n_host, out, local));
-    const double sec = ms_since(tg) * 1e-3;
-    if (sec > 0) e->rate_gpu = 0.7 * e->rate_gpu + 0.3 * ((double)sst.m_active / sec);
-  }
   return CG_OK;
 }
 
@@ -325,6 +425,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   const int alg = e->alg;
   const int mode = e->opt.sched_mode;
   if (alg == CG_SSSP && !g.has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
+  // A run that failed part-way can leave queued-ahead kernels on the stream;
+  // drain them before the host resets the statistics records they publish
+  // into (their error, if any, was reported by that run).
+  if (hipStreamSynchronize(s) != hipSuccess) (void)hipGetLastError();
   if (alg == CG_BFS && mode != CG_SCHED_HEURISTIC) CG_TRY(build_transpose(&g));
   // bottom-up min steps (SSSP, label propagation): transposed graph + edge tiles
   const bool forced_pull = mode == CG_SCHED_FORCED && decode_choice(e->opt.forced_choice).pull;
@@ -436,7 +540,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     const int tp = k & 3;
     if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
     CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
-                              ~0ull >> 8, 0xFFFFFFFFu, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s,
+                              ~0ull >> 8, SmallRange{}, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s,
                               hub_on));
     if (early_bfs) CG_TRY(launch_pull_bfs(dg, e->val.p, e->bm[c].p, e->bm[c ^ 1].p, e->stripes.p, s, dk));
     if (e->opt.profile) {
@@ -575,6 +679,71 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   return CG_OK;
 }
 
+int engine_run_host(cg_engine* e, uint32_t root, cg_run_stats* rs) {
+  cg_graph& g = *e->g;
+  const uint32_t V = g.V;
+  const int alg = e->alg;
+  if (alg == CG_SSSP && !g.has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
+  CG_TRY(ensure_team(e));
+  const uint64_t words = ((uint64_t)V + 63) / 64;
+  std::vector<uint64_t> in, out;
+  try {
+    in.assign(words + 1, 0);
+    out.assign(words + 1, 0);
+  } catch (const std::bad_alloc&) {
+    return set_error(CG_ERR_OOM, "host frontier bitmaps");
+  }
+  cg_run_stats local{};
+  const auto t0 = clk::now();
+  // ---- initAlgorithm (CGgraph_engine.cuh:1259-1316) ----
+  e->team.reset_values(alg, root);
+  if (alg == CG_CC) {
+    const uint64_t Vp = (uint64_t)V - g.zero_tail;
+    for (uint64_t i = 0; i < Vp / 64; ++i) in[i] = ~0ull;
+    if (Vp % 64) in[Vp / 64] = (1ull << (Vp % 64)) - 1;
+  } else {
+    in[root >> 6] |= 1ull << (root & 63);
+  }
+  uint32_t it = 0;
+  for (;;) {
+    ++it;
+    uint64_t frontier = 0;
+    for (uint64_t i = 0; i < words; ++i) frontier += (uint64_t)__builtin_popcountll(in[i]);
+    std::fill(out.begin(), out.end(), 0ull);
+    uint64_t edges = 0;
+    const uint64_t updates = e->team.step_bitmap(alg, &g, in.data(), out.data(), words, &edges);
+    ++local.light_iterations;
+    local.cpu_vertices += frontier;
+    local.edges_relaxed += edges;
+    if (e->trace) {
+      std::fprintf(e->trace,
+                   "{\"alg\":%d,\"V\":%u,\"E\":%llu,\"root\":%u,\"iter\":%u,\"host\":1,\"n_active\":%llu,"
+                   "\"edges\":%llu,\"updates\":%llu,\"stolen_units\":%llu,\"donated_vertices\":%llu}\n",
+                   alg, V, (unsigned long long)g.E, root, it, (unsigned long long)frontier,
+                   (unsigned long long)edges, (unsigned long long)updates,
+                   (unsigned long long)e->team.last_stolen_units, (unsigned long long)e->team.last_donated_vertices);
+    }
+    if (updates == 0) break;
+    in.swap(out);
+  }
+  local.iterations = it;
+  local.ms = ms_since(t0);
+  if (e->trace) std::fflush(e->trace);
+  // TEPS numerator, outside the timed region
+  if (alg != CG_CC) {
+    uint64_t m = 0;
+    const uint32_t* val = e->team.shadow.data();
+#pragma omp parallel for num_threads(e->team.threads) reduction(+ : m) schedule(static)
+    for (int64_t v = 0; v < (int64_t)V; ++v)
+      if (val[v] != kUnreached) m += g.h_off[v + 1] - g.h_off[v];
+    local.edges_traversed = m;
+  } else {
+    local.edges_traversed = g.E;
+  }
+  if (rs) *rs = local;
+  return CG_OK;
+}
+
 }  // namespace cg
 
 cg_engine::~cg_engine() {
@@ -585,6 +754,10 @@ cg_engine::~cg_engine() {
   for (auto& pr : ev_pair)
     for (auto ev : pr)
       if (ev) (void)hipEventDestroy(ev);
+  for (auto ev : ev_piece)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto ev : ev_pcie)
+    if (ev) (void)hipEventDestroy(ev);
   if (stream) (void)hipStreamDestroy(stream);
   if (trace) std::fclose(trace);
 }

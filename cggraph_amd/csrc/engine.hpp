@@ -20,6 +20,10 @@ struct cg_engine {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
   hipEvent_t ev_pair[4][2] = {};  // expansions queued ahead, by iteration mod 4
+  hipEvent_t ev_piece[2] = {};    // co-executed iterations: the agent's last two pieces
+  hipEvent_t ev_pcie[2] = {};     // co-executed iterations: around the host share's copy
+  // no GPU: every iteration runs on the host team (use_device_num 0, MULTI_CORE, SINGLE_CORE)
+  bool host_only = false;
 
   // values (uint32 for BFS/SSSP/CC, float for PR) and frontier state
   cg::DevBuf<uint32_t> val;
@@ -67,8 +71,9 @@ struct cg_engine {
   bool mlp_valid = false;
   cg::DevBuf<float> d_mlp;
 
-  // measured throughputs (edges/s) for the host/GPU split
-  double rate_gpu = 2.0e10, rate_cpu = 2.0e8;
+  // measured throughputs (edges/s) and PCIe time (ms) of the last shared
+  // iteration, for the host/GPU split
+  double rate_gpu = 2.0e10, rate_cpu = 2.0e8, pcie_ms = 0.0;
 
   std::string trace_path, mlp_path;  // owned copies of the option strings
   FILE* trace = nullptr;
@@ -81,6 +86,10 @@ struct cg_engine {
 
 namespace cg {
 int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs);
+// The engine without a GPU: every iteration is relaxed by the host team over
+// a host frontier bitmap, on host values (the reference's LIGHT path,
+// CGgraph_engine.cuh:907-912, and the MULTI_CORE engine, GraphNuma.cuh:166-314).
+int engine_run_host(cg_engine* e, uint32_t root, cg_run_stats* rs);
 int mlp_load(cg_engine* e);
 // Learned-scheduler inference on the device, fused into the stats epilogue.
 int launch_stats_mlp(const float* params, const DevStats* st, uint32_t V, uint64_t E,

@@ -135,12 +135,19 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);
+// The part of the small tier one expansion launch takes: entries [begin,
+// min(end, n_small)) of q_small; big: the launch also takes the medium and
+// chunked tiers.
+struct SmallRange {
+  uint32_t begin = 0;
+  uint32_t end = 0xFFFFFFFFu;
+  uint32_t big = 1;
+};
 // Push expansion over the queues of launch_compact (lengths read on the
-// device; at most small_limit small-tier vertices, the rest being the host
-// team's share).
+// device), restricted to `range`.
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
-                       uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
+                       uint64_t item_bound, SmallRange range, const Bins& bins, DevStats* st,
                        uint32_t* stripes, bool exact, hipStream_t s, bool track_min = false);
 // Bottom-up BFS step: unreached vertices look for a parent in `in`.
 // cond not NULL: runs only when cond->choice (left by k_stats) asks for a

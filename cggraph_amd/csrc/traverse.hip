@@ -607,15 +607,20 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
                                                    const uint32_t* __restrict__ q_small,
                                                    const uint32_t* __restrict__ q_med,
                                                    const uint64_t* __restrict__ q_chunk,
-                                                   uint32_t small_limit, uint32_t chunk_edges,
+                                                   SmallRange range, uint32_t chunk_edges,
                                                    const DevStats* __restrict__ st,
                                                    uint32_t* __restrict__ stripes) {
   // The queue lengths come from the compaction; the wavefronts of a grid
   // capped near one chip's worth of resident workgroups stride over the items
   // (chunks first, the heaviest), so no wavefront is launched only to find
   // nothing to do.
-  const uint32_t n_chunk = st->n_chunk, n_med = st->n_med;
-  const uint32_t n_small = st->n_small < small_limit ? st->n_small : small_limit;
+  // range: the part of the small tier this launch expands (the rest is the
+  // host team's, or another launch's), and whether it also takes the chunked
+  // and medium tiers
+  const uint32_t n_chunk = range.big ? st->n_chunk : 0u, n_med = range.big ? st->n_med : 0u;
+  const uint32_t s_end = st->n_small < range.end ? st->n_small : range.end;
+  const uint32_t s_beg = range.begin < s_end ? range.begin : s_end;
+  const uint32_t n_small = s_end - s_beg;
   const uint64_t n_items = (uint64_t)n_chunk + n_med + (n_small + kWave - 1) / kWave;
   const uint32_t lane = lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
@@ -638,8 +643,8 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
       // the lanes: lane j of round r takes edge r*64+j of the concatenated
       // adjacency lists and finds its owner by binary search over the
       // exclusive degree prefix held one entry per lane.
-      const uint64_t base = (item - n_chunk - n_med) * kWave;
-      const uint32_t cnt = (n_small - base) < (uint64_t)kWave ? (uint32_t)(n_small - base) : kWave;
+      const uint64_t base = s_beg + (item - n_chunk - n_med) * kWave;
+      const uint32_t cnt = (s_end - base) < (uint64_t)kWave ? (uint32_t)(s_end - base) : kWave;
       uint32_t deg = 0, sv = 0;
       uint64_t s = 0;
       if (lane < cnt) {
@@ -998,7 +1003,7 @@ uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bin
 
 int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
                        const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
-                       uint64_t item_bound, uint32_t small_limit, const Bins& bins, DevStats* st,
+                       uint64_t item_bound, SmallRange range, const Bins& bins, DevStats* st,
                        uint32_t* stripes, bool exact, hipStream_t s, bool track_min) {
   if (!item_bound) return CG_OK;
   if (track_min && alg != CG_SSSP) return set_error(CG_ERR_INTERNAL, "minimum tracking is for SSSP");
@@ -1008,15 +1013,15 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
   if (blocks > cap) blocks = cap;
   dim3 grid((unsigned)blocks), block(kBlock);
 #define CG_EXPAND(A, X)                                                                          \
-  hipLaunchKernelGGL((k_expand<A, X, false>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk, small_limit, \
+  hipLaunchKernelGGL((k_expand<A, X, false>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk, range, \
                      bins.chunk_edges, st, stripes)
   if (track_min) {
     if (exact)
       hipLaunchKernelGGL((k_expand<ALG_SSSP, true, true>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st, stripes);
+                         range, bins.chunk_edges, st, stripes);
     else
       hipLaunchKernelGGL((k_expand<ALG_SSSP, false, true>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         small_limit, bins.chunk_edges, st, stripes);
+                         range, bins.chunk_edges, st, stripes);
     CG_LAUNCH_CHECK();
     return CG_OK;
   }

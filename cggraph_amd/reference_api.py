@@ -219,7 +219,7 @@ class TwoLevelPartition:
     charged against this GPU's HBM; the graph then lives in HBM."""
 
     def __init__(self, csrResult, zeroOutDegreeNum, deviceNum, algorithm):
-        if deviceNum != 1:
+        if deviceNum > 1:
             raise CgError(_lib.CG_ERR_UNSUPPORTED, "libcgmi drives one GPU per engine")
         self.csr = csrResult
         self.algorithm = Algorithm_type(algorithm)
@@ -281,8 +281,12 @@ def main_partition(graphName="toy_0", orderMethod=OrderMethod.NATIVE, root=0, ru
             if orderMethod != OrderMethod.NATIVE and rootTranfer:
                 old2new = load_binFile(gf.old2newFile, gf.vertices)
             nbrSort_taskSteal(csr)
-    if engine_type not in (Engine_type.COORDINATION, Engine_type.SINGLE_GPU):
+    if engine_type not in (Engine_type.COORDINATION, Engine_type.SINGLE_GPU, Engine_type.MULTI_CORE,
+                           Engine_type.SINGLE_CORE):
         raise CgError(_lib.CG_ERR_UNSUPPORTED, f"engine type {Engine_type(engine_type).name} is not provided")
+    if engine_type in (Engine_type.MULTI_CORE, Engine_type.SINGLE_CORE):
+        # Graph_Numa (main_partition.hpp:248-317): no partition, no GPU
+        useDeviceNum = 0
     part = TwoLevelPartition(csr, zeroOutDegreeNum, useDeviceNum, algorithm).getPartitionResult()
     eng = CGgraphEngine(csr, part, algorithm, useDeviceNum, engine_type, **options)
     if root < 0 or root >= csr.vertexNum:

@@ -11,8 +11,10 @@
  *   cg_graph_generate_*    (synthetic workloads; no reference counterpart)
  *   cg_partition           TwoLevelPartition         my_project/CGgraph/Two-level_partition.cuh:106-748
  *   cg_engine_create       CGgraphEngine ctor        my_project/CGgraph/CGgraph_engine.cuh:154-253
- *                          (engine_type CG_MULTI_CORE / CG_SINGLE_GPU select the
- *                          other engines of main_partition.hpp:162-557)
+ *                          (engine_type and use_device_num as main_partition.hpp:20-31,
+ *                          162-557: CG_SINGLE_GPU runs every iteration on the GPU;
+ *                          use_device_num 0, CG_MULTI_CORE and CG_SINGLE_CORE run every
+ *                          iteration on the host team, Graph_Numa GraphNuma.cuh:166-314)
  *   cg_engine_run          CGgraphEngine::CG_co_execution(root) -> ms, result in
  *                          vertexValue               CGgraph_engine.cuh:265-974, :69
  *   cg_push_step_device    BFS_SPACE::bfs_device / SSSP_SPACE::sssp_device
@@ -50,7 +52,8 @@ extern "C" {
 enum {
   CG_OK = 0,
   CG_ERR_INVALID = 1,      /* bad argument */
-  CG_ERR_NO_DEVICE = 2,    /* a GPU engine was requested but no GPU is usable */
+  CG_ERR_NO_DEVICE = 2,    /* a GPU engine was requested but no GPU is usable (never a silent
+                              switch to the host team: ask for use_device_num 0 for that) */
   CG_ERR_HIP = 3,          /* a HIP runtime call failed */
   CG_ERR_OOM = 4,          /* host or device allocation failed */
   CG_ERR_IO = 5,           /* file missing or of the wrong size */
@@ -62,8 +65,14 @@ enum {
 /* Algorithm_type (Basic/Graph/basic_struct.hpp:144-151); CC is WCC. */
 enum { CG_BFS = 0, CG_SSSP = 1, CG_CC = 2, CG_PR = 3 };
 
-/* Engine_type (Basic/Graph/basic_struct.hpp:165-178), the values this library runs. */
-enum { CG_COORDINATION = 0, CG_MULTI_CORE = 3, CG_SINGLE_GPU = 4 };
+/* Engine_type (Basic/Graph/basic_struct.hpp:165-178), the values this library runs.
+ * COORDINATION with use_device_num 1: GPU and host team per the scheduler; with
+ * use_device_num 0: every iteration on the host team (the reference's LIGHT path,
+ * CGgraph_engine.cuh:907-912).  SINGLE_CORE / MULTI_CORE: the host team with one /
+ * cg_options.cpu_threads threads, whatever use_device_num says.  SINGLE_GPU: every
+ * iteration on the GPU.  PageRank needs a GPU.  COOPERATION and the tool engines
+ * (BLOCK_2D, COMPUTE_INDEGREE, REORDER, ACTIVEBLOCK) return CG_ERR_UNSUPPORTED. */
+enum { CG_COORDINATION = 0, CG_SINGLE_CORE = 2, CG_MULTI_CORE = 3, CG_SINGLE_GPU = 4 };
 
 /* Per-iteration scheduler of the COORDINATION engine. */
 enum {
@@ -264,6 +273,26 @@ int cg_batch_run(cg_graph* const* graphs, const int* algorithms, const uint32_t*
  * (Many CG_co_execution calls, CGgraph_engine.cuh:265, issued as one.) */
 int cg_engine_run_batch(cg_engine* const* engines, const uint32_t* roots, int nq, void* const* outs,
                         cg_run_stats* stats);
+
+/* ---- allocation-free run ---- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the GPU schedule of cg_engine_run (statistics, degree-tier
+ * compaction with the default tiers, push expansion) for BFS, SSSP or CC
+ * (label propagation) over a CSR the caller holds in device memory (d_offsets
+ * uint64[V+1], d_dst / d_w uint32[E]; d_w may be NULL except for SSSP) from
+ * `root`, into d_val (device uint32[V]).  Frontiers, queues and counters live
+ * in one static device array of CG_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; a graph whose buffers do not fit (about 8.3 bytes per
+ * vertex plus 3 bytes per 512 edges) returns CG_ERR_INVALID.  Nothing is
+ * allocated: no hipMalloc, hipFree or pinned host memory.  Every call on one
+ * GPU uses that GPU's array, so calls that overlap on one GPU race.  stream is
+ * a hipStream_t (NULL = the null stream); the call returns when the run has
+ * finished on it.  *iterations (may be NULL) receives the iteration count. */
+#define CG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cg_traverse_device_noalloc(int algorithm, const uint64_t* d_offsets, const uint32_t* d_dst,
+                               const uint32_t* d_w, uint32_t V, uint64_t E, uint32_t root, uint32_t* d_val,
+                               void* stream, uint32_t* iterations);
 
 /* ---- learned scheduler ---- */
 /* Trains the MLP (12 -> 64 -> 64 -> 16, ReLU, softmax) on the GPU from a

@@ -208,3 +208,7 @@ double or_multicore_run(int alg, uint32_t V, const uint64_t* off, const uint32_t
   free(g.in); free(g.out); free(ts);
   return ms;
 }
+
+/* omp_get_max_threads() of this library's OpenMP runtime (the thread count a
+ * run with threads <= 0 uses), for the baseline's report. */
+int or_omp_max_threads(void) { return omp_get_max_threads(); }

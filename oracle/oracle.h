@@ -97,6 +97,7 @@ uint32_t or_max_outdeg_vertex(uint32_t V, const uint64_t* off);
 /* ---- restated reference CPU engine (MULTI_CORE) ----
  * alg: 0 BFS, 1 SSSP, 2 CC.  Returns the wall time of the iterations in ms
  * (load and setup excluded), writes the values to out, iterations to *iters. */
+int or_omp_max_threads(void);
 double or_multicore_run(int alg, uint32_t V, const uint64_t* off, const uint32_t* dst,
                         const uint32_t* w, uint32_t root, int threads, uint32_t* out,
                         uint32_t* iters);

@@ -49,6 +49,7 @@ def lib():
             "or_max_outdeg_vertex": (u32, [u32, _u64p]),
             "or_multicore_run": (dbl, [i32, u32, _u64p, _u32p, _vp, u32, i32, _u32p,
                                        ctypes.POINTER(u32)]),
+            "or_omp_max_threads": (i32, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -187,3 +188,8 @@ def multicore_run(alg, g, root=0, threads=0):
     ms = lib().or_multicore_run(alg, g.V, g.offsets, g.dest, _ptr(g.weight), root, threads,
                                 out, ctypes.byref(it))
     return out, ms, it.value
+
+
+def omp_max_threads():
+    """omp_get_max_threads() of the restated engine's OpenMP runtime."""
+    return int(lib().or_omp_max_threads())

@@ -112,6 +112,30 @@ def test_engine_needs_a_gpu_here():
     with pytest.raises(cg.CgError) as ei:
         cg.Engine(g, 0)
     assert ei.value.code == _lib.CG_ERR_NO_DEVICE
-    with pytest.raises(cg.CgError) as ei:
-        cg.Engine(g, 0, use_device_num=0)
-    assert ei.value.code == _lib.CG_ERR_NO_DEVICE
+    # use_device_num = 0 asks for the engine without a GPU: it runs on the host team
+    e = cg.Engine(g, 0, use_device_num=0)
+    vals, st = e.run(0)
+    assert vals.tolist() == [0, 1] and st["heavy_iterations"] == 0
+    e.close()
+
+
+def test_noalloc_rejects_before_touching_a_device():
+    """The allocation-free run checks its arguments and the scratch size
+    before any HIP call (fake device pointers are never dereferenced)."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    it = ctypes.c_uint32(7)
+    # 2^24 vertices need more than the 64 MiB scratch
+    assert L.cg_traverse_device_noalloc(1, fake, fake, fake, 1 << 24, 0, 0, fake, None, ctypes.byref(it)) == \
+        _lib.CG_ERR_INVALID
+    assert b"scratch" in L.cg_last_error()
+    assert it.value == 0
+    # an edge count whose chunk queue does not fit
+    assert L.cg_traverse_device_noalloc(0, fake, fake, None, 1000, 1 << 36, 0, fake, None, None) == \
+        _lib.CG_ERR_INVALID
+    assert L.cg_traverse_device_noalloc(3, fake, fake, fake, 10, 10, 0, fake, None, None) == _lib.CG_ERR_UNSUPPORTED
+    assert L.cg_traverse_device_noalloc(0, fake, fake, None, 10, 10, 10, fake, None, None) == _lib.CG_ERR_INVALID
+    assert L.cg_traverse_device_noalloc(1, fake, fake, None, 10, 10, 0, fake, None, None) == _lib.CG_ERR_INVALID
+    assert L.cg_traverse_device_noalloc(0, None, fake, None, 10, 10, 0, fake, None, None) == _lib.CG_ERR_INVALID
+    assert L.cg_traverse_device_noalloc(0, fake, fake, None, 0, 0, 0, fake, None, None) == _lib.CG_ERR_INVALID
+    assert _lib.CG_NOALLOC_SCRATCH_BYTES == 64 << 20

@@ -222,3 +222,27 @@ def test_pagerank_step_consistent(oracle):
     step = oracle.pagerank_step(g, p19.astype(np.float32), 0.85)
     assert np.abs(step - p20).sum() < 1e-6
     assert np.abs(step - p20).sum() > 0  # the float32 rounding is visible, so inputs were used
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_reached_edges_and_root_helpers(oracle, seed):
+    """The TEPS numerator (sum of out-degrees of reached vertices) and the
+    root rule (largest out-degree, lowest id on ties) against numpy
+    one-liners, on graphs with unreached vertices and tied maximum degrees."""
+    rng = np.random.default_rng(seed)
+    V = 3000
+    src = rng.integers(0, V // 2, 9000).astype(np.uint32)  # the upper half sends nothing
+    dst = rng.integers(0, V, 9000).astype(np.uint32)
+    g = oracle.csr_build(V, src, dst)
+    g.weight = rng.integers(1, 101, g.E).astype(np.uint32)
+    deg = np.diff(g.offsets.astype(np.int64))
+    for root in (0, int(rng.integers(0, V)), V - 1):
+        val = oracle.sssp(g, root)
+        assert oracle.reached_edges(g, val) == int(deg[val != U].sum())
+    val = np.full(V, U, np.uint32)
+    assert oracle.reached_edges(g, val) == 0
+    assert oracle.max_outdeg_vertex(g) == int(np.argmax(deg))  # argmax takes the first maximum
+    # ties: two vertices share the maximum; the lower id wins
+    off = np.array([0, 1, 3, 3, 5], np.uint64)
+    t = oracle.CSR(off, np.array([1, 0, 2, 0, 1], np.uint32))
+    assert oracle.max_outdeg_vertex(t) == 1

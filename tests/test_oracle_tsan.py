@@ -9,8 +9,8 @@ reporting OpenMP's synchronisation) must run BFS, SSSP and CC with no report
 and the same values as the sequential oracle.  A planted plain-int race in the
 driver shows that the same build does report races.  The host team (the
 engine's LIGHT-iteration relaxation over OpenMP, with chunked hub edges) is
-driven the same way from tests/host_team_tsan.cpp on a host-only graph.  CPU
-only.
+driven in its three modes from tests/host_team_tsan.cpp on a host-only graph.
+CPU only.
 """
 import os
 import subprocess
@@ -80,8 +80,15 @@ def test_planted_race_is_reported(tsan_binary):
 
 
 def test_host_team_is_race_free(host_team_binary):
+    """List (three-stage stealing with donation), bitmap (the engine without a
+    GPU) and tail (team against a head-claiming agent thread) modes."""
     r = run(host_team_binary)
     assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
     assert r.returncode == 0, (r.stdout, r.stderr[-2000:])
     lines = [ln for ln in r.stdout.splitlines() if ln.startswith("alg ")]
-    assert len(lines) == 3 and all(ln.endswith("mismatches 0") for ln in lines), r.stdout
+    assert len(lines) == 9 and all(ln.endswith("mismatches 0") for ln in lines), r.stdout
+    f = {(ln.split()[1], ln.split()[3]): dict(zip(ln.split()[4::2], map(int, ln.split()[5::2]))) for ln in lines}
+    # the hub (16 000 edges) was donated in the list and bitmap modes
+    assert all(f[(a, m)]["donated"] >= 1 for a in "012" for m in ("list", "bitmap"))
+    # CC's first frontier is every vertex: both ends of the queue took entries
+    assert f[("2", "tail")]["agent"] > 0 and f[("2", "tail")]["team"] > 0

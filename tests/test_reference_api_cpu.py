@@ -47,3 +47,22 @@ def test_toy_graph_and_neighbour_sort():
                              csr_dest=np.array([0, 0], np.uint32), csr_weight=None)
     with pytest.raises(cg.CgError):  # strictly ascending, as sortNbr.hpp:84-91 asserts
         ra.nbrSort_taskSteal(dup)
+
+
+@pytest.mark.parametrize("engine_type", [ra.Engine_type.MULTI_CORE, ra.Engine_type.SINGLE_CORE,
+                                         ra.Engine_type.COORDINATION])
+def test_main_partition_without_gpu(golden, engine_type):
+    """main_partition with a CPU engine type (main_partition.hpp:248-317), or
+    COORDINATION with useDeviceNum = 0: the host team runs every iteration;
+    runs = 4 gives the trimmed mean of the two middle times."""
+    t = golden["graphs"]["toy_0"]
+    for alg, key in ((ra.Algorithm_type.BFS, "bfs"), (ra.Algorithm_type.SSSP, "sssp")):
+        for r, exp in t[key].items():
+            res = ra.main_partition("toy_0", root=int(r), runs=4, algorithm=alg, engine_type=engine_type,
+                                    useDeviceNum=0, cpu_threads=3)
+            assert res["vertexValue"].tolist() == exp
+            assert len(res["times_ms"]) == 4
+            assert res["stats"]["light_iterations"] == res["stats"]["iterations"]
+    res = ra.main_partition("toy_0", root=0, algorithm=ra.Algorithm_type.CC, engine_type=engine_type,
+                            useDeviceNum=0)
+    assert res["vertexValue"].tolist() == t["cc_directed"]
