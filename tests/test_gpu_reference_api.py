@@ -43,8 +43,12 @@ def test_main_partition_trimmed_mean_and_result_check(golden, tmp_path):
     bad[5] += 1
     with pytest.raises(cg.CgError):
         checkBinResult(str(tmp_path / "toy_1_SSSP_3.bin"), bad, 12)
+    # MULTI_CORE runs on the host team (no GPU), bit-exact; BLOCK_2D is refused
+    res = main_partition("toy_1", OrderMethod.NATIVE, root=3, runs=3, algorithm=Algorithm_type.SSSP,
+                         engine_type=Engine_type.MULTI_CORE)
+    assert res["vertexValue"].tolist() == golden["graphs"]["toy_1"]["sssp"]["3"]
     with pytest.raises(cg.CgError):
-        main_partition("toy_0", engine_type=Engine_type.MULTI_CORE)
+        main_partition("toy_0", engine_type=Engine_type.BLOCK_2D)
 
 
 def _hip():

@@ -284,7 +284,7 @@ def test_error_paths(oracle):
         cg.Engine(g, SSSP)
     assert ei.value.code == _lib.CG_ERR_INVALID
     with pytest.raises(cg.CgError) as ei:
-        cg.Engine(g, BFS, engine_type=3)
+        cg.Engine(g, BFS, engine_type=5)  # BLOCK_2D
     assert ei.value.code == _lib.CG_ERR_UNSUPPORTED
     gw = gpu_graph(og)
     e = cg.Engine(gw, BFS, sched_mode=_lib.CG_SCHED_GPU, fault_flags=_lib.CG_FAULT_HIP_ERROR)
