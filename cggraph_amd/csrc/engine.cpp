@@ -238,7 +238,10 @@ int co_execute(cg_engine* e, const DevGraph& dg, const Bins& bins, const DevStat
   std::string agent_error;
   uint32_t pieces = 0;
   double gpu_sec = 0, host_sec = 0;
-#pragma omp parallel num_threads(T + 1)
+  // one team of T threads: thread 0 is the agent, threads 1 .. T - 1 the
+  // host's workers (a team larger than the host team would leave a thread
+  // more than the cores spinning in the OpenMP pool afterwards)
+#pragma omp parallel num_threads(T)
   {
     const int tid = omp_get_thread_num();
     if (tid == 0) {
@@ -516,6 +519,15 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // The iteration after the last one finds an empty frontier and does nothing;
   // a delta-stepping threshold moves one iteration later.
   const bool run_ahead = early && !early_learned;
+  // SSSP / CC under the GPU schedule without delta-stepping: the statistics
+  // are counted by the compaction itself and reduced behind the expansion
+  // (three kernels per iteration, one pass over the frontier)
+  const bool fused = run_ahead && alg != CG_BFS && !delta;
+  if (fused && hub_on) {
+    // minima stripes start empty; slot 1 (the first iteration's) holds the
+    // root's value 0 as the frontier's smallest
+    CG_TRY(launch_fill_u32(e->stripes.p + kStripeMinBase, 0xFFFFFFFFu, kStripes * kStripeStride, s));
+  }
   bool early_timed[4] = {false, false, false, false};
   auto take_timing = [&](int k) -> int {
     if (!early_timed[k]) return CG_OK;
@@ -530,6 +542,23 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   auto enqueue = [&](uint32_t k) -> int {
     const int sl = k & 1, c = (k - 1) & 1;
     DevStats* dk = e->st.p + sl;
+    if (fused) {
+      uint32_t nparts = 0;
+      CG_TRY(launch_compact_fused(e->bm[c].p, e->bm[c ^ 1].p, words, dg, fbins, e->q_small.p, e->q_med.p,
+                                  e->q_chunk.p, dk, e->val.p, hub, e->parts.p, &nparts, s));
+      const int tp = k & 3;
+      if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
+      CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
+                                ~0ull >> 8, SmallRange{}, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s,
+                                hub_on));
+      if (e->opt.profile) {
+        CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));
+        early_timed[tp] = true;
+      }
+      ++local.expand_launches;
+      return launch_fused_finish(e->parts.p, nparts, e->stripes.p, e->run_state.p, dk, e->st.p + (sl ^ 1),
+                                 e->hst_map_dev + sl, hub_on, s);
+    }
     CG_TRY(launch_stats(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p, e->run_state.p,
                         mlp_dev, tval, threshold, dk, e->hst_map_dev + sl, s,
                         early_bfs ? 1 : 0, hub_on ? e->st.p + (sl ^ 1) : nullptr, hub_on));
@@ -575,10 +604,13 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     }
     tit = clk::now();
     if (hub_on) {
-      // the previous iteration's waiting hubs were counted as expanded
-      local.edges_relaxed -= sst.m_def_prev;
-      local.gpu_vertices -= sst.n_def_prev;
-      local.expand_alg_bytes -= 20.0 * sst.n_def_prev + 12.0 * (double)sst.m_def_prev;
+      // waiting hubs are counted as expanded below (the fused record carries
+      // its own iteration's, the others the previous iteration's)
+      const uint32_t n_def = fused ? sst.n_def : sst.n_def_prev;
+      const unsigned long long m_def = fused ? sst.m_def : sst.m_def_prev;
+      local.edges_relaxed -= m_def;
+      local.gpu_vertices -= n_def;
+      local.expand_alg_bytes -= 20.0 * n_def + 12.0 * (double)m_def;
     }
     if (sst.n_active == 0) break;
     if (delta && sst.n_near == 0) {

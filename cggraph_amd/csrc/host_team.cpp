@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 #include "common.hpp"
 
@@ -197,12 +198,16 @@ uint64_t HostTeam::three_stage(uint64_t units, Src& src, Sink& sink, const cg_gr
     }
     // 1.3 EDGE_WORKING: help donors until every thread has got here
     working.fetch_sub(1, std::memory_order_acq_rel);
-    while (working.load(std::memory_order_acquire) > 0) {
+    // (a waiting thread yields now and then: when the process has more
+    // runnable threads than cores, the thread everyone waits for may be the
+    // one without a core)
+    for (uint32_t spin = 1; working.load(std::memory_order_acquire) > 0; ++spin) {
       for (int o = 1; o < T; ++o) {
         Slot& h = slots[(tid + o) % T];
         if (h.status.load(std::memory_order_acquire) == kEdgeDonate) drain(h);
       }
-      __builtin_ia32_pause();
+      if (spin % 64 == 0) std::this_thread::yield();
+      else __builtin_ia32_pause();
     }
   }
   last_stolen_units = stolen;

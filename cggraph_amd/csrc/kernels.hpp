@@ -51,12 +51,15 @@ struct RunState {
   uint32_t pad;
 };
 
-// Per-workgroup partial of k_stats, reduced by the last workgroup.
+// Per-workgroup partial of k_stats (and of the fused compaction, which also
+// counts the hubs it keeps waiting), reduced by one workgroup.
 struct StatsPart {
   uint32_t n_active, n_zero;
   unsigned long long m_active;
   uint32_t hist[8];
   uint32_t min_val, n_near;
+  uint32_t n_def, pad;
+  unsigned long long m_def;
 };
 
 struct DevGraph {
@@ -132,6 +135,23 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
                    const uint32_t* val, uint32_t threshold, uint64_t* carry, hipStream_t s,
                    const TierTable* choice_tiers = nullptr, uint32_t skip_bits = 8,
                    HubDefer hub = HubDefer{0, 0});
+// Fused compaction and statistics (GPU schedule, SSSP / CC without
+// delta-stepping): one pass over frontier `in` writes the three work queues
+// (lengths into st, whose counters start at zero), every word of `out` (zero,
+// or the hubs kept waiting under `hub`) and one statistics partial per
+// workgroup into `parts` (*nparts set by the launcher, at most
+// stats_parts_needed(words) - 1).  The hub threshold is st->min_val +
+// hub.delta.  launch_fused_finish reduces the partials after the expansion
+// has run: it publishes the iteration's record (as k_stats_reduce does, plus
+// n_def / m_def of this iteration), folds the expansion's first-visit stripes
+// into the run state, and prepares `next` (the following iteration's slot):
+// queue counters zeroed and, with `track`, min_val taken from the minima
+// stripes, which it resets.
+int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
+                         uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
+                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s);
+int launch_fused_finish(const StatsPart* parts, uint32_t nparts, uint32_t* stripes, RunState* rs, DevStats* st,
+                        DevStats* next, DevStats* host_out, bool track, hipStream_t s);
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);

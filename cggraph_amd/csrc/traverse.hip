@@ -240,6 +240,38 @@ __device__ __forceinline__ void for_each_in_slice(uint32_t bits, uint64_t base, 
   }
 }
 
+// Dense frontiers: a wavefront's 64 slices cover 1024 consecutive vertices,
+// whose 1025 offsets it loads coalesced (lane l takes entries l, l + 64, ...)
+// into LDS, one pad entry after every 16 so that lane l's 17 offsets sit at
+// 17l .. 17l + 15 and 17l + 17 (two-way bank conflicts at most).  Per-lane
+// loads of those offsets touch a different line per lane and are bound by
+// the L1's line rate instead of by HBM.
+constexpr int kWaveVerts = kWave * (int)kSliceBits;                 // 1024
+constexpr int kStageEntries = kWaveVerts + kWaveVerts / 16 + 2;      // 1090
+__device__ __forceinline__ uint32_t stage_index(uint32_t i) { return i + (i >> 4); }
+
+__device__ __forceinline__ void stage_offsets(const uint64_t* __restrict__ off, uint32_t V, uint64_t vb,
+                                              uint64_t* __restrict__ lds) {
+  for (uint32_t i = lane_id(); i <= (uint32_t)kWaveVerts; i += kWave) {
+    const uint64_t v = vb + i;
+    lds[stage_index(i)] = v <= V ? off[v] : 0ull;  // entries past V are never used (no bits there)
+  }
+}
+
+// for_each_in_slice over offsets staged by stage_offsets; `row` is the
+// wavefront's staging area, the lane's slice is its lane_id()-th.
+template <class F>
+__device__ __forceinline__ void for_each_in_slice_staged(uint32_t bits, uint64_t base,
+                                                         const uint64_t* __restrict__ row, F&& f) {
+  if (!bits) return;
+  const uint32_t l0 = lane_id() * (uint32_t)kSliceBits;
+  for (; bits; bits &= bits - 1) {
+    const uint32_t j = (uint32_t)__builtin_ctz(bits);
+    const uint64_t a = row[stage_index(l0 + j)], b = row[stage_index(l0 + j + 1)];
+    f((uint32_t)(base + j), b - a);
+  }
+}
+
 // Slice si of a bitmap: its 16 bits, and its first vertex.
 __device__ __forceinline__ uint32_t slice_bits(uint64_t word, uint64_t si) {
   return (uint32_t)(word >> (kSliceBits * (si % kSlicesPerWord))) & 0xFFFFu;
@@ -276,6 +308,8 @@ __device__ __forceinline__ void stats_block_part(const StatsPart& p, StatsPart* 
       for (int k = 0; k < 8; ++k) b.hist[k] += wpart[w].hist[k];
       b.n_near += wpart[w].n_near;
       b.min_val = wpart[w].min_val < b.min_val ? wpart[w].min_val : b.min_val;
+      b.n_def += wpart[w].n_def;
+      b.m_def += wpart[w].m_def;
     }
     parts[blockIdx.x] = b;
   }
@@ -599,6 +633,240 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
       for (uint32_t c = 0; c < nch; ++c) q_chunk[b_chunk++] = ((uint64_t)v << 32) | c;
     }
   });
+}
+
+constexpr uint32_t kDenseBlock = 256;  // frontier vertices per 4096 that pay for staging offsets
+constexpr uint64_t kFusedMaxBlocks = 1024;  // LDS staging: four workgroups per CU
+
+// Compaction with the frontier statistics folded in (the GPU schedule's
+// SSSP / CC iterations): k_compact's two passes over each 16-vertex slice,
+// plus the counts k_stats would take from the same offsets, so the frontier
+// and its offsets are read once per iteration instead of twice.  A grid
+// capped at kFusedMaxBlocks strides over the slices (one queue-base atomic
+// per queue per workgroup and stride step); every word of `out` is written
+// here -- zero, or the hubs kept waiting -- so no other kernel clears it.
+__global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __restrict__ in,
+                                                          uint64_t* __restrict__ out, uint64_t words, DevGraph g,
+                                                          Bins bins, uint32_t* __restrict__ q_small,
+                                                          uint32_t* __restrict__ q_med,
+                                                          uint64_t* __restrict__ q_chunk,
+                                                          DevStats* __restrict__ st,
+                                                          const uint32_t* __restrict__ val, HubDefer hub,
+                                                          StatsPart* __restrict__ parts) {
+  __shared__ uint32_t s_tot[kWavesPerBlock][3];
+  __shared__ uint32_t s_base[kWavesPerBlock][3];
+  __shared__ uint32_t s_cnt[kWavesPerBlock];
+  __shared__ uint64_t s_off[kWavesPerBlock][kStageEntries];
+  uint64_t* const row = s_off[wave_id()];
+  uint32_t c_act = 0, c_zero = 0, n_def = 0;
+  uint64_t m = 0, m_def = 0;
+  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // hub threshold: the frontier's smallest value (left in st by the previous
+  // iteration's finish) + delta
+  uint32_t thr = 0xFFFFFFFFu;
+  if (hub.degree) {
+    const uint64_t t = (uint64_t)st->min_val + hub.delta;
+    thr = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+  }
+  const uint64_t slices = words * kSlicesPerWord;
+  // block-uniform loop: every thread runs the same number of steps
+  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t si = base + threadIdx.x;
+    const uint64_t wi = si / kSlicesPerWord;
+    const uint64_t vbase = si * kSliceBits;
+    uint32_t word = si < slices ? slice_bits(in[wi], si) : 0u;
+    c_act += (uint32_t)__popc(word);
+    const bool any = __ballot(word != 0) != 0;  // wave-uniform
+    // a workgroup whose 4096 vertices hold at least kDenseBlock frontier
+    // vertices stages its offsets in LDS (block-uniform choice)
+    const uint32_t wcnt = wave_sum_u32((uint32_t)__popc(word));
+    if (lane_id() == 0) s_cnt[wave_id()] = wcnt;
+    __syncthreads();
+    uint32_t bcnt = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) bcnt += s_cnt[w];
+    const bool dense = bcnt >= kDenseBlock;
+    if (dense) {
+      stage_offsets(g.off, g.V, (base + (uint64_t)wave_id() * kWave) * kSliceBits, row);
+      __syncthreads();
+    }
+    // sparse workgroups: one offset pair per frontier vertex
+    auto slice_each = [&](uint32_t bits, auto&& f) {
+      if (dense) {
+        for_each_in_slice_staged(bits, vbase, row, f);
+      } else {
+        for (; bits; bits &= bits - 1) {
+          const uint32_t v = (uint32_t)(vbase + (uint32_t)__builtin_ctz(bits));
+          f(v, g.off[v + 1] - g.off[v]);
+        }
+      }
+    };
+    uint32_t o_small = 0, o_med = 0, o_chunk = 0;
+    uint32_t t_small = 0, t_med = 0, t_chunk = 0;
+    uint64_t wait_word = 0;
+    if (any) {
+      uint32_t c_small = 0, c_med = 0, c_chunk = 0;
+      uint32_t wait = 0, wait_min = 0xFFFFFFFFu;
+      slice_each(word, [&](uint32_t v, uint64_t deg) {
+        m += deg;
+        h[hist_bin(deg)]++;
+        if (deg == 0) {
+          ++c_zero;
+          return;
+        }
+        if (hub.degree && deg >= hub.degree) {
+          const uint32_t x = val[v];
+          if (x >= thr) {
+            wait |= 1u << (v - vbase);
+            wait_min = x < wait_min ? x : wait_min;
+            ++n_def;
+            m_def += deg;
+            return;
+          }
+        }
+        if (deg < bins.bin_wave) c_small++;
+        else if (deg < bins.bin_chunk) c_med++;
+        else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+      });
+      if (hub.degree) {
+        wait_word = join_slices(wait, si);
+        word &= ~wait;
+        // the waiting vertices are part of the next frontier's smallest value
+        if (hub.track) stripe_min(hub.track, wave_min_u32(wait_min));
+      }
+      o_small = wave_excl_scan(c_small, &t_small);
+      o_med = wave_excl_scan(c_med, &t_med);
+      o_chunk = wave_excl_scan(c_chunk, &t_chunk);
+    }
+    if (si < slices && si % kSlicesPerWord == 0) out[wi] = wait_word;
+    if (lane_id() == 0) {
+      s_tot[wave_id()][0] = t_small;
+      s_tot[wave_id()][1] = t_med;
+      s_tot[wave_id()][2] = t_chunk;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      const int q = threadIdx.x;
+      uint32_t tot = 0;
+      for (int w = 0; w < kWavesPerBlock; ++w) { s_base[w][q] = tot; tot += s_tot[w][q]; }
+      uint32_t b = 0;
+      if (tot) b = atomicAdd(q == 0 ? &st->n_small : (q == 1 ? &st->n_med : &st->n_chunk), tot);
+      for (int w = 0; w < kWavesPerBlock; ++w) s_base[w][q] += b;
+    }
+    __syncthreads();
+    if (any) {
+      uint32_t b_small = s_base[wave_id()][0] + o_small;
+      uint32_t b_med = s_base[wave_id()][1] + o_med;
+      uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
+      slice_each(word, [&](uint32_t v, uint64_t deg) {
+        if (deg == 0) return;
+        if (deg < bins.bin_wave) q_small[b_small++] = v;
+        else if (deg < bins.bin_chunk) q_med[b_med++] = v;
+        else {
+          const uint32_t nch = (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+          for (uint32_t c = 0; c < nch; ++c) q_chunk[b_chunk++] = ((uint64_t)v << 32) | c;
+        }
+      });
+    }
+    __syncthreads();  // s_tot / s_base / s_cnt / s_off are reused by the next step
+  }
+  StatsPart p{};
+  p.n_active = wave_sum_u32(c_act);
+  p.m_active = wave_sum_u64(m);
+  p.n_zero = wave_sum_u32(c_zero);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h[k]);
+  p.min_val = 0xFFFFFFFFu;
+  p.n_def = wave_sum_u32(n_def);
+  p.m_def = wave_sum_u64(m_def);
+  stats_block_part(p, parts);
+}
+
+// One workgroup behind the expansion of a fused iteration: fixed-order
+// reduction of the compaction's partials, the record of the iteration
+// (published to mapped host memory with `iter` last), the first-visit
+// stripes of the expansion folded into the run state, and the next slot
+// prepared for the next compaction.
+__global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __restrict__ parts, uint32_t nparts,
+                                                         uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
+                                                         DevStats* __restrict__ st, DevStats* __restrict__ next,
+                                                         DevStats* __restrict__ host_out, int track) {
+  constexpr int kC = 12;  // n_active, n_zero, hist[8], n_def, spare
+  __shared__ unsigned long long s_m[kBlock], s_md[kBlock];
+  __shared__ uint32_t s_c[kBlock][kC];
+  unsigned long long m = 0, md = 0;
+  uint32_t c[kC] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < nparts; b += kBlock) {
+    const StatsPart q = parts[b];
+    m += q.m_active;
+    md += q.m_def;
+    c[0] += q.n_active;
+    c[1] += q.n_zero;
+    for (int k = 0; k < 8; ++k) c[2 + k] += q.hist[k];
+    c[10] += q.n_def;
+  }
+  s_m[threadIdx.x] = m;
+  s_md[threadIdx.x] = md;
+  for (int k = 0; k < kC; ++k) s_c[threadIdx.x][k] = c[k];
+  __syncthreads();
+  for (int w = kBlock / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_m[threadIdx.x] += s_m[threadIdx.x + w];
+      s_md[threadIdx.x] += s_md[threadIdx.x + w];
+      for (int k = 0; k < kC; ++k) s_c[threadIdx.x][k] += s_c[threadIdx.x + w][k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  uint32_t nn = 0;
+#pragma unroll
+  for (int k = 0; k < kStripes; ++k) nn += stripes[k * kStripeStride];
+#pragma unroll
+  for (int k = 0; k < kStripes; ++k) stripes[k * kStripeStride] = 0;
+  DevStats d{};
+  d.n_small = st->n_small;
+  d.n_med = st->n_med;
+  d.n_chunk = st->n_chunk;
+  d.min_val = st->min_val;
+  d.n_active = s_c[0][0];
+  d.n_zero_deg = s_c[0][1];
+  d.m_active = s_m[0];
+  for (int k = 0; k < 8; ++k) d.hist[k] = s_c[0][2 + k];
+  d.n_def = s_c[0][10];
+  d.m_def = s_md[0];
+  // the record describes the frontier this iteration expanded: vertices
+  // reached before it, and the first visits of the previous expansion
+  d.visited = rs->visited;
+  d.n_new = rs->pad;
+  d.iter = rs->iter + 1;
+  d.choice = -1;
+  rs->visited = d.visited + nn;
+  rs->pad = nn;
+  rs->iter = d.iter;
+  uint32_t mv = 0;
+  if (track) {
+    // the next frontier's smallest value: smallest activating candidate of
+    // this expansion and smallest waiting hub of this compaction
+    mv = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < kStripes; ++k) {
+      const uint32_t x = stripes[kStripeMinBase + k * kStripeStride];
+      mv = x < mv ? x : mv;
+      stripes[kStripeMinBase + k * kStripeStride] = 0xFFFFFFFFu;
+    }
+  }
+  next->n_small = 0;
+  next->n_med = 0;
+  next->n_chunk = 0;
+  next->min_val = mv;
+  *st = d;
+  if (host_out) {
+    const uint32_t itv = d.iter;
+    d.iter = 0xFFFFFFFFu;
+    *host_out = d;
+    // the release store orders the record before `iter` at system scope
+    __hip_atomic_store(&host_out->iter, itv, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 template <int ALG, bool EXACT, bool TRACK>
@@ -992,6 +1260,28 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
   hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, words, g, bins,
                      q_small, q_med, q_chunk, st, val, threshold, carry, choice_tiers ? 1 : 0, tiers, skip_bits,
                      hub);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
+                         uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
+                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s) {
+  if (hub.degree && (!val || !hub.delta)) return set_error(CG_ERR_INVALID, "hub deferral needs values and delta >= 1");
+  const uint64_t slices = words * kSlicesPerWord;
+  uint64_t blocks = slices ? (slices + kBlock - 1) / kBlock : 1;
+  if (blocks > kFusedMaxBlocks) blocks = kFusedMaxBlocks;
+  hipLaunchKernelGGL(k_compact_fused, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, out, words, g, bins, q_small,
+                     q_med, q_chunk, st, val, hub, parts);
+  CG_LAUNCH_CHECK();
+  *nparts = (uint32_t)blocks;
+  return CG_OK;
+}
+
+int launch_fused_finish(const StatsPart* parts, uint32_t nparts, uint32_t* stripes, RunState* rs, DevStats* st,
+                        DevStats* next, DevStats* host_out, bool track, hipStream_t s) {
+  hipLaunchKernelGGL(k_fused_finish, dim3(1), dim3(kBlock), 0, s, parts, nparts, stripes, rs, st, next, host_out,
+                     track ? 1 : 0);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
