@@ -60,6 +60,7 @@ class Options(ctypes.Structure):
         ("batch_group", ctypes.c_uint32),
         ("hub_degree", ctypes.c_uint32),
         ("hub_delta", ctypes.c_uint32),
+        ("probe_stride", ctypes.c_uint32),
     ]
 
 

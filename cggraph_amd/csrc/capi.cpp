@@ -200,6 +200,7 @@ int cg_options_default(cg_options* o) {
   o->batch_group = 0;
   o->hub_degree = 0;
   o->hub_delta = 0;
+  o->probe_stride = 0;
   return CG_OK;
 }
 

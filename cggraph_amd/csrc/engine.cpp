@@ -491,10 +491,12 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // and nothing waits for the statistics, so compaction and expansion are
   // queued right behind them and the host reads the statistics while the GPU
   // goes on (BFS decides push or pull from the statistics first).
-  // The learned schedule joins when its network runs on the device: a GPU
-  // push choice is applied there (tiers from the choice), a host choice leaves
-  // the queues empty and the host team runs the iteration afterwards.
-  const bool early_learned = learned && mlp_dev && alg != CG_BFS && !min_pull;
+  // The learned schedule joins when its network runs on the device: the
+  // network runs behind each expansion on the features of the frontier just
+  // expanded and picks the next iteration's schedule; a GPU push choice is
+  // applied there (tiers from the choice), a host choice leaves the queues
+  // empty and the host team runs the iteration after reading its record.
+  const bool early_learned = learned && mlp_dev && alg != CG_BFS && !min_pull && !delta;
   // BFS under the GPU schedule: the direction rule runs on the device too, and
   // both the push pair and the bottom-up kernel are queued; each runs only
   // when the rule picked it
@@ -517,16 +519,23 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // Pure GPU schedule: iteration it + 1 is queued before the host reads the
   // statistics of iteration it, so the host's launch work overlaps the GPU.
   // The iteration after the last one finds an empty frontier and does nothing;
-  // a delta-stepping threshold moves one iteration later.
-  const bool run_ahead = early && !early_learned;
-  // SSSP / CC under the GPU schedule without delta-stepping: the statistics
-  // are counted by the compaction itself and reduced behind the expansion
-  // (three kernels per iteration, one pass over the frontier)
+  // a delta-stepping threshold moves one iteration later.  Under the learned
+  // schedule a host-team choice is known one record ahead: the iteration it
+  // names is already queued (and queues nothing on the GPU), the one after it
+  // is queued once the host team has relaxed the frontier.
+  const bool run_ahead = early;
+  // SSSP / CC under the GPU and learned schedules without delta-stepping: the
+  // statistics are counted by the compaction itself and reduced behind the
+  // expansion (three kernels per iteration, one pass over the frontier)
   const bool fused = run_ahead && alg != CG_BFS && !delta;
   if (fused && hub_on) {
     // minima stripes start empty; slot 1 (the first iteration's) holds the
     // root's value 0 as the frontier's smallest
     CG_TRY(launch_fill_u32(e->stripes.p + kStripeMinBase, 0xFFFFFFFFu, kStripes * kStripeStride, s));
+  }
+  if (early_learned) {
+    // the first iteration has no network choice: the GPU rule (-1)
+    CG_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(&e->st.p[1].choice), 0xFFFFFFFFu, 1, s));
   }
   bool early_timed[4] = {false, false, false, false};
   auto take_timing = [&](int k) -> int {
@@ -545,7 +554,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     if (fused) {
       uint32_t nparts = 0;
       CG_TRY(launch_compact_fused(e->bm[c].p, e->bm[c ^ 1].p, words, dg, fbins, e->q_small.p, e->q_med.p,
-                                  e->q_chunk.p, dk, e->val.p, hub, e->parts.p, &nparts, s));
+                                  e->q_chunk.p, dk, e->val.p, hub, e->parts.p, &nparts, s,
+                                  early_learned ? &choice_tiers : nullptr));
       const int tp = k & 3;
       if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
       CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
@@ -557,14 +567,14 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       }
       ++local.expand_launches;
       return launch_fused_finish(e->parts.p, nparts, e->stripes.p, e->run_state.p, dk, e->st.p + (sl ^ 1),
-                                 e->hst_map_dev + sl, hub_on, s);
+                                 e->hst_map_dev + sl, hub_on, early_learned ? mlp_dev : nullptr, V, g.E, s);
     }
     CG_TRY(launch_stats(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p, e->run_state.p,
                         mlp_dev, tval, threshold, dk, e->hst_map_dev + sl, s,
                         early_bfs ? 1 : 0, hub_on ? e->st.p + (sl ^ 1) : nullptr, hub_on));
     if (!early) return CG_OK;
     CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, cval,
-                          threshold, e->bm[c ^ 1].p, s, (early_learned || early_bfs) ? &choice_tiers : nullptr,
+                          threshold, e->bm[c ^ 1].p, s, early_bfs ? &choice_tiers : nullptr,
                           early_bfs ? 1u : 8u, hub));
     const int tp = k & 3;
     if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
@@ -579,13 +589,18 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     ++local.expand_launches;
     return CG_OK;
   };
+  // learned schedule: the choice the previous record made for this
+  // iteration (-1: the GPU rule)
+  int pending = -1;
   if (run_ahead) CG_TRY(enqueue(1));
   auto tit = clk::now();
   for (;;) {
     ++it;
     const int slot = it & 1;
     DevStats* dst = e->st.p + slot;
-    CG_TRY(enqueue(run_ahead ? it + 1 : it));
+    const bool host_now = early_learned && pending >= 0 && decode_choice(pending).host;
+    if (!run_ahead) CG_TRY(enqueue(it));
+    else if (!host_now) CG_TRY(enqueue(it + 1));
     CG_TRY(wait_stats(e, slot, it));
     if (e->opt.fault_flags & CG_FAULT_HIP_ERROR)
       return set_error(CG_ERR_HIP, "injected HIP failure at iteration %u", it);
@@ -633,8 +648,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       else if (alg == CG_BFS) pull = beamer_pull(pull, sst, m_unexplored, V, prev_active);
       step.ch.pull = pull;
       if (learned) {
-        if (mlp_dev && probs_usable(sst)) {
-          step.ch = decode_choice(sst.choice);
+        if (pending >= 0) {
+          step.ch = decode_choice(pending);
           step.host_share = step.ch.host ? 1 : 0;
           took_learned = 1;
         } else {
@@ -659,15 +674,18 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
 
     uint64_t* in = e->bm[cur].p;
     uint64_t* out = e->bm[cur ^ 1].p;
-    if (probe) {
+    const uint32_t pstride = e->opt.probe_stride > 1 ? e->opt.probe_stride : 1;
+    if (probe && (pstride == 1 || it % pstride <= 1)) {
       double us[CG_NUM_CHOICES];
       CG_TRY(probe_all(e, dg, sst, dst, in, out, threshold, us));
       write_probe(e, rec, root, us);
     }
     if (early && step.host_share) {
       // the network gave this iteration to the host team; the queued GPU
-      // expansion found empty queues
+      // expansion found empty queues.  The next iteration is queued behind
+      // the host team's candidates.
       CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));
+      if (host_now) CG_TRY(enqueue(it + 1));
     } else if (early && step.ch.pull) {
       // the queued bottom-up kernel ran (same accounting as exec_step's)
       ++local.pull_iterations;
@@ -685,6 +703,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     } else {
       CG_TRY(exec_step(e, dg, step, sst, dst, in, out, &timing_pending, &local));
     }
+    // the network's choice for the next iteration, made from this frontier
+    pending = (learned && mlp_dev && probs_usable(sst)) ? sst.choice : -1;
     cur ^= 1;
   }
   CG_HIP(hipStreamSynchronize(s));

@@ -147,11 +147,18 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
 // into the run state, and prepares `next` (the following iteration's slot):
 // queue counters zeroed and, with `track`, min_val taken from the minima
 // stripes, which it resets.
+// Learned schedule (choice_tiers / mlp not NULL): the finish runs the
+// network on the features of the frontier it reduced and leaves its choice
+// for the next iteration in the record and in next->choice; the compaction
+// takes the tiers of the choice in st->choice (entry 0 of the table when it
+// is -1), and for a host-team choice queues nothing and keeps no hub waiting.
 int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
                          uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s);
+                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s,
+                         const TierTable* choice_tiers = nullptr);
 int launch_fused_finish(const StatsPart* parts, uint32_t nparts, uint32_t* stripes, RunState* rs, DevStats* st,
-                        DevStats* next, DevStats* host_out, bool track, hipStream_t s);
+                        DevStats* next, DevStats* host_out, bool track, const float* mlp, uint32_t V, uint64_t E,
+                        hipStream_t s);
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);

@@ -10,6 +10,7 @@
 // trace and seed always give the same weights.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -162,6 +163,21 @@ bool json_floats(const std::string& line, const char* key, std::vector<double>& 
   return true;
 }
 
+// Relative margin within which two probe-timed choices count as equally fast.
+constexpr double kLabelTolerance = 0.03;
+
+// The number that follows `key` (e.g. "\"iter\":") in `line`; false when absent.
+bool json_number(const std::string& line, const char* key, double* out) {
+  const size_t k = line.find(key);
+  if (k == std::string::npos) return false;
+  const char* c = line.c_str() + k + std::strlen(key);
+  char* nx = nullptr;
+  const double v = std::strtod(c, &nx);
+  if (nx == c) return false;
+  *out = v;
+  return true;
+}
+
 struct Gpu {
   DevBuf<float> P, X, gpart, lpart, grad, m, v, loss;
   DevBuf<int> Y;
@@ -285,22 +301,50 @@ int cg_mlp_train(const char* trace_path, const char* out_path, int epochs, float
   if (!trace_path || !out_path || epochs <= 0 || !(lr > 0.f)) return set_error(CG_ERR_INVALID, "bad arguments");
   FILE* f = std::fopen(trace_path, "r");
   if (!f) return set_error(CG_ERR_IO, "cannot open trace %s", trace_path);
+  // The network picks the schedule of the next iteration from the features
+  // of the frontier it has just seen: a sample is the features of probe
+  // record k and, as label, the fastest choice of record k + 1 of the same
+  // run (same algorithm, graph and root, next iteration).
   std::vector<float> X;
   std::vector<int> Y;
   std::string line;
-  std::vector<double> feat, us;
+  std::vector<double> feat, us, prev_feat;
+  double prev_key[5] = {-1, -1, -1, -1, -1};
+  bool have_prev = false;
   char buf[8192];
   while (std::fgets(buf, sizeof(buf), f)) {
     line = buf;
     if (line.find("\"probe\"") == std::string::npos) continue;
+    double key[5];
+    if (!json_number(line, "\"alg\":", &key[0]) || !json_number(line, "\"V\":", &key[1]) ||
+        !json_number(line, "\"E\":", &key[2]) || !json_number(line, "\"root\":", &key[3]) ||
+        !json_number(line, "\"iter\":", &key[4]))
+      continue;
     if (!json_floats(line, "\"features\"", feat) || feat.size() != kMlpIn) continue;
     if (!json_floats(line, "\"us_all\"", us) || us.size() != kMlpOut) continue;
     int best = -1;
     for (int k = 0; k < kMlpOut; ++k)
       if (us[k] > 0 && (best < 0 || us[k] < us[best])) best = k;
-    if (best < 0) continue;
-    for (double v : feat) X.push_back((float)v);
-    Y.push_back(best);
+    // choices within kLabelTolerance of the fastest are the same within the
+    // timing noise; the lowest-numbered one (the GPU rule's default tiers
+    // come first) is the label, so that indifferent iterations agree
+    if (best >= 0) {
+      const double lim = us[best] * (1.0 + kLabelTolerance);
+      for (int k = 0; k < kMlpOut; ++k)
+        if (us[k] > 0 && us[k] <= lim) {
+          best = k;
+          break;
+        }
+    }
+    const bool follows = have_prev && key[0] == prev_key[0] && key[1] == prev_key[1] && key[2] == prev_key[2] &&
+                         key[3] == prev_key[3] && key[4] == prev_key[4] + 1;
+    if (follows && best >= 0) {
+      for (double v : prev_feat) X.push_back((float)v);
+      Y.push_back(best);
+    }
+    prev_feat = feat;
+    for (int k = 0; k < 5; ++k) prev_key[k] = key[k];
+    have_prev = true;
   }
   std::fclose(f);
   const int n = (int)Y.size();

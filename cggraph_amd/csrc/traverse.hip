@@ -652,12 +652,30 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
                                                           uint64_t* __restrict__ q_chunk,
                                                           DevStats* __restrict__ st,
                                                           const uint32_t* __restrict__ val, HubDefer hub,
-                                                          StatsPart* __restrict__ parts) {
+                                                          StatsPart* __restrict__ parts, int use_choice,
+                                                          TierTable tiers) {
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   __shared__ uint32_t s_cnt[kWavesPerBlock];
   __shared__ uint64_t s_off[kWavesPerBlock][kStageEntries];
   uint64_t* const row = s_off[wave_id()];
+  // learned schedule: the choice the previous iteration's finish left in st
+  // (-1: the network deferred; the tier table's entry 0 applies).  A
+  // host-team choice queues nothing and keeps no hub waiting: the statistics
+  // are still counted and the next bitmap cleared, and the host relaxes the
+  // frontier after reading the record.
+  bool queue = true;
+  if (use_choice) {
+    const int c = st->choice;
+    const int tier = c > 0 ? (c >> 1) & 3 : 0;
+    bins.bin_wave = tiers.wave[tier];
+    bins.bin_chunk = tiers.chunk[tier];
+    if (c >= 0 && (c & 8)) {
+      queue = false;
+      hub.degree = 0;
+      hub.track = nullptr;
+    }
+  }
   uint32_t c_act = 0, c_zero = 0, n_def = 0;
   uint64_t m = 0, m_def = 0;
   uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -734,6 +752,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
         // the waiting vertices are part of the next frontier's smallest value
         if (hub.track) stripe_min(hub.track, wave_min_u32(wait_min));
       }
+      if (!queue) {
+        word = 0;
+        c_small = c_med = c_chunk = 0;
+      }
       o_small = wave_excl_scan(c_small, &t_small);
       o_med = wave_excl_scan(c_med, &t_med);
       o_chunk = wave_excl_scan(c_chunk, &t_chunk);
@@ -790,7 +812,8 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
 __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __restrict__ parts, uint32_t nparts,
                                                          uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
                                                          DevStats* __restrict__ st, DevStats* __restrict__ next,
-                                                         DevStats* __restrict__ host_out, int track) {
+                                                         DevStats* __restrict__ host_out, int track,
+                                                         const float* __restrict__ mlp, uint32_t V, uint64_t E) {
   constexpr int kC = 12;  // n_active, n_zero, hist[8], n_def, spare
   __shared__ unsigned long long s_m[kBlock], s_md[kBlock];
   __shared__ uint32_t s_c[kBlock][kC];
@@ -817,13 +840,17 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
     }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
+  __shared__ float feat[CG_NUM_FEATURES];
+  __shared__ float mlp_sh[2 * kMlpH + kMlpOut];
+  __shared__ float probs[kMlpOut];
+  __shared__ int best;
+  DevStats d{};
+  if (threadIdx.x == 0) {
   uint32_t nn = 0;
 #pragma unroll
   for (int k = 0; k < kStripes; ++k) nn += stripes[k * kStripeStride];
 #pragma unroll
   for (int k = 0; k < kStripes; ++k) stripes[k * kStripeStride] = 0;
-  DevStats d{};
   d.n_small = st->n_small;
   d.n_med = st->n_med;
   d.n_chunk = st->n_chunk;
@@ -859,6 +886,29 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
   next->n_med = 0;
   next->n_chunk = 0;
   next->min_val = mv;
+  make_features(d, V, E, d.visited, d.iter, feat);
+  }
+  __syncthreads();
+  if (mlp) {
+    // the learned schedule of the next iteration, from this frontier's
+    // features; the network defers (-1) unless its output is a finite
+    // distribution whose top choice has probability >= kLearnedMinProb
+    mlp_forward_block(mlp, feat, mlp_sh, probs, &best);
+  }
+  if (threadIdx.x != 0) return;
+  int choice = -1;
+  if (mlp) {
+    float sum = 0.f;
+    bool finite = true;
+    for (int k = 0; k < kMlpOut; ++k) {
+      finite = finite && isfinite(probs[k]);
+      sum += probs[k];
+      d.probs[k] = probs[k];
+    }
+    if (finite && fabsf(sum - 1.f) < 1e-3f && probs[best] >= kLearnedMinProb) choice = best;
+  }
+  d.choice = choice;
+  next->choice = choice;
   *st = d;
   if (host_out) {
     const uint32_t itv = d.iter;
@@ -1266,22 +1316,25 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
 
 int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
                          uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s) {
+                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s,
+                         const TierTable* choice_tiers) {
   if (hub.degree && (!val || !hub.delta)) return set_error(CG_ERR_INVALID, "hub deferral needs values and delta >= 1");
   const uint64_t slices = words * kSlicesPerWord;
   uint64_t blocks = slices ? (slices + kBlock - 1) / kBlock : 1;
   if (blocks > kFusedMaxBlocks) blocks = kFusedMaxBlocks;
+  const TierTable tiers = choice_tiers ? *choice_tiers : TierTable{};
   hipLaunchKernelGGL(k_compact_fused, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, out, words, g, bins, q_small,
-                     q_med, q_chunk, st, val, hub, parts);
+                     q_med, q_chunk, st, val, hub, parts, choice_tiers ? 1 : 0, tiers);
   CG_LAUNCH_CHECK();
   *nparts = (uint32_t)blocks;
   return CG_OK;
 }
 
 int launch_fused_finish(const StatsPart* parts, uint32_t nparts, uint32_t* stripes, RunState* rs, DevStats* st,
-                        DevStats* next, DevStats* host_out, bool track, hipStream_t s) {
+                        DevStats* next, DevStats* host_out, bool track, const float* mlp, uint32_t V, uint64_t E,
+                        hipStream_t s) {
   hipLaunchKernelGGL(k_fused_finish, dim3(1), dim3(kBlock), 0, s, parts, nparts, stripes, rs, st, next, host_out,
-                     track ? 1 : 0);
+                     track ? 1 : 0, mlp, V, E);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

@@ -53,11 +53,38 @@ def read_probe_records(trace_path):
     return recs
 
 
+def _run_key(r):
+    return (r["alg"], r["V"], r["E"], r["root"])
+
+
 def dataset(records):
-    X = np.array([r["features"] for r in records], np.float32)
-    T = np.array([r["us_all"] for r in records], np.float64)
-    Y = np.argmin(np.where(T > 0, T, np.inf), axis=1).astype(np.int32)
-    return X, Y, T
+    """Training / evaluation pairs of the lagged scheduler: the network sees
+    the features of iteration k's frontier and picks iteration k + 1's
+    schedule.  X = features of record k, T = probe times of record k + 1 of the
+    same run (same algorithm, graph, root; next iteration), Y = the fastest
+    choice in T, R = the records k + 1 (for rules that look at the iteration
+    itself)."""
+    X, T, R = [], [], []
+    for a, b in zip(records, records[1:]):
+        if _run_key(a) == _run_key(b) and b["iter"] == a["iter"] + 1:
+            X.append(a["features"])
+            T.append(b["us_all"])
+            R.append(b)
+    X = np.array(X, np.float32).reshape(-1, 12)
+    T = np.array(T, np.float64).reshape(-1, 16)
+    Y = labels(T)
+    return X, Y, T, R
+
+
+LABEL_TOLERANCE = 0.03  # cg_mlp_train's kLabelTolerance
+
+
+def labels(T):
+    """The lowest-numbered choice within LABEL_TOLERANCE of the fastest
+    (choices that close are equal within the timing noise)."""
+    Tp = np.where(T > 0, T, np.inf)
+    lim = Tp.min(axis=1, keepdims=True) * (1.0 + LABEL_TOLERANCE)
+    return np.argmax(Tp <= lim, axis=1).astype(np.int32)
 
 
 def train(trace_path, out_path, epochs=400, lr=3e-3, seed=1):

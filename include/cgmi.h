@@ -159,6 +159,10 @@ typedef struct cg_options {
                                      with sssp_delta */
   uint32_t hub_delta;             /* window of hub_degree above the frontier's smallest distance;
                                      0 = 5 */
+  uint32_t probe_stride;          /* CG_SCHED_PROBE: time every choice only at iterations k with
+                                     k mod probe_stride in {0, 1} (pairs of consecutive
+                                     iterations, what the lagged scheduler trains on); the others
+                                     follow the GPU rule.  0 or 1 = every iteration */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

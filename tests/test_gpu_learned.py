@@ -56,7 +56,7 @@ def test_probe_train_and_gain(oracle, tmp_path):
             scheduler.collect_probe_traces(g, alg, [r], trace, cpu_threads=4)
     recs = scheduler.read_probe_records(trace)
     assert len(recs) > 20
-    X, Y, T = scheduler.dataset(recs)
+    X, Y, T, _ = scheduler.dataset(recs)
     loss = scheduler.train(trace, weights, epochs=600, lr=3e-3, seed=7)
     P = scheduler.load_weights(weights)
     l0, _ = mlp_ref.loss_grad(np.zeros(6032), X.astype(np.float64), Y)  # uniform predictor
@@ -69,9 +69,9 @@ def test_probe_train_and_gain(oracle, tmp_path):
     for alg in (0, 1):
         scheduler.collect_probe_traces(g, alg, [root], held, cpu_threads=4)
     hr = scheduler.read_probe_records(held)
-    Xh, Yh, Th = scheduler.dataset(hr)
+    Xh, Yh, Th, Rh = scheduler.dataset(hr)
     learned = np.argmax(scheduler.mlp_forward(P, Xh), axis=1)
-    heur = [scheduler.heuristic_choice(r) for r in hr]
+    heur = [scheduler.heuristic_choice(r) for r in Rh]
     t_learn, t_heur, t_best = (scheduler.chosen_time(Th, learned), scheduler.chosen_time(Th, heur),
                                scheduler.chosen_time(Th, Yh))
     print(f"held-out probe cost: learned {t_learn:.0f} us, heuristic {t_heur:.0f} us, oracle-best {t_best:.0f} us")
