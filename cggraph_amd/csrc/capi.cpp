@@ -544,7 +544,8 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->acc.ensure(kAccSlots));
     CG_TRY(e->parts.ensure(stats_parts_needed(words)));
     CG_TRY(e->stripes.ensure(kStripeWords));
-    CG_TRY(e->run_state.ensure(1));
+    CG_TRY(e->run_state.ensure(2));
+    if (algorithm == CG_BFS) CG_TRY(e->bfs_cnt.ensure(2 * kBfsCntWords));
   }
   if (e->opt.trace_path) {
     e->trace = std::fopen(e->opt.trace_path, "a");

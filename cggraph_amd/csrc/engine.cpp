@@ -460,21 +460,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
 
   cg_run_stats local{};
   const auto t0 = clk::now();
-  // ---- initAlgorithm (CGgraph_engine.cuh:1259-1316) ----
-  CG_HIP(hipMemsetAsync(e->bm[0].p, 0, words * sizeof(uint64_t), s));
-  CG_HIP(hipMemsetAsync(e->bm[1].p, 0, words * sizeof(uint64_t), s));
-  CG_HIP(hipMemsetAsync(e->stripes.p, 0, kStripeWords * sizeof(uint32_t), s));
-  CG_HIP(hipMemsetAsync(e->st.p, 0, 2 * sizeof(DevStats), s));
+  // ---- initAlgorithm (CGgraph_engine.cuh:1259-1316): one launch, below ----
   for (int k = 0; k < 2; ++k) __atomic_store_n(&e->hst_map.p[k].iter, 0xFFFFFFFFu, __ATOMIC_RELEASE);
-  if (alg == CG_CC) {
-    CG_TRY(launch_iota_u32(e->val.p, V, s));
-    CG_TRY(launch_bitmap_prefix(e->bm[0].p, words, (uint64_t)V - g.zero_tail, s));
-    CG_TRY(launch_init_run_state(e->run_state.p, V, s));
-  } else {
-    CG_TRY(launch_fill_u32(e->val.p, kUnreached, V, s));
-    CG_TRY(launch_set_root(e->val.p, e->bm[0].p, root, s));
-    CG_TRY(launch_init_run_state(e->run_state.p, 1, s));
-  }
   if (host_possible) e->team.reset_values(alg, root);
 
   int cur = 0;
@@ -528,14 +515,32 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // statistics are counted by the compaction itself and reduced behind the
   // expansion (three kernels per iteration, one pass over the frontier)
   const bool fused = run_ahead && alg != CG_BFS && !delta;
-  if (fused && hub_on) {
-    // minima stripes start empty; slot 1 (the first iteration's) holds the
-    // root's value 0 as the frontier's smallest
-    CG_TRY(launch_fill_u32(e->stripes.p + kStripeMinBase, 0xFFFFFFFFu, kStripes * kStripeStride, s));
-  }
-  if (early_learned) {
-    // the first iteration has no network choice: the GPU rule (-1)
-    CG_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(&e->st.p[1].choice), 0xFFFFFFFFu, 1, s));
+  {
+    TraverseInit ti{};
+    ti.val = e->val.p;
+    ti.V = V;
+    ti.alg = alg;
+    ti.root = root;
+    ti.cc_active = (uint64_t)V - g.zero_tail;  // CC starts from V' (CGgraph_engine.cuh:1303-1311)
+    ti.bm0 = e->bm[0].p;
+    ti.bm1 = e->bm[1].p;
+    ti.bm_words = words;
+    ti.stripes = e->stripes.p;
+    // fused SSSP with hub deferral: the minima stripes start empty, and slot
+    // 1 (the first iteration's) holds the root's value 0 as the frontier's
+    // smallest (the cleared slot's min_val)
+    ti.minima_empty = fused && hub_on;
+    ti.st = e->st.p;
+    // learned: the first iteration has no network choice (the GPU rule, -1)
+    ti.choice1_none = early_learned;
+    ti.rs = e->run_state.p;
+    // BFS on the GPU schedule counts the root into its first frontier;
+    // the other paths start with the root (or, for CC, every vertex) visited
+    ti.visited0 = alg == CG_CC ? V : (early_bfs ? 0 : 1);
+    // BFS: the direction of iteration 1 is taken from the root's counts
+    ti.bfs_cnt = early_bfs ? e->bfs_cnt.p : nullptr;
+    ti.off = dg.off;
+    CG_TRY(launch_init_traverse(ti, s));
   }
   bool early_timed[4] = {false, false, false, false};
   auto take_timing = [&](int k) -> int {
@@ -551,6 +556,23 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   auto enqueue = [&](uint32_t k) -> int {
     const int sl = k & 1, c = (k - 1) & 1;
     DevStats* dk = e->st.p + sl;
+    if (early_bfs) {
+      // BFS: the front kernel decides the direction from the counts the
+      // previous step left and compacts a push frontier; the step kernel
+      // expands or searches bottom-up and counts the next frontier
+      const int tp = k & 3;
+      CG_TRY(launch_bfs_iteration(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, fbins, e->val.p, e->q_small.p,
+                                  e->q_med.p, e->q_chunk.p, dk, e->st.p + (sl ^ 1),
+                                  e->bfs_cnt.p + (size_t)c * kBfsCntWords, e->bfs_cnt.p + (size_t)sl * kBfsCntWords,
+                                  e->run_state.p + c, e->run_state.p + sl, e->hst_map_dev + sl,
+                                  e->opt.atomic_return != 0, s, e->opt.profile ? e->ev_pair[tp][0] : nullptr));
+      if (e->opt.profile) {
+        CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));
+        early_timed[tp] = true;
+      }
+      ++local.expand_launches;
+      return CG_OK;
+    }
     if (fused) {
       uint32_t nparts = 0;
       CG_TRY(launch_compact_fused(e->bm[c].p, e->bm[c ^ 1].p, words, dg, fbins, e->q_small.p, e->q_med.p,
@@ -714,8 +736,12 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   local.ms = ms_since(t0);
   e->cur = cur;
   if (e->trace) std::fflush(e->trace);
-  // TEPS numerator, outside the timed region
-  if (alg != CG_CC) {
+  // TEPS numerator, outside the timed region.  BFS on the GPU schedule has it
+  // already: every reached vertex is in exactly one frontier, and the records
+  // carry each frontier's exact out-edge count.
+  if (early_bfs) {
+    local.edges_traversed = m_explored;
+  } else if (alg != CG_CC) {
     CG_HIP(hipMemsetAsync(e->acc.p, 0, kAccSlots * sizeof(unsigned long long), s));
     CG_TRY(launch_reached_edges(dg, e->val.p, e->acc.p, s));
     unsigned long long m[kAccSlots];

@@ -38,7 +38,8 @@ struct cg_engine {
   cg::DevBuf<unsigned long long> acc;  // kAccSlots striped accumulators
   cg::DevBuf<cg::StatsPart> parts;     // k_stats workgroup partials
   cg::DevBuf<uint32_t> stripes;        // striped new-vertex counters
-  cg::DevBuf<cg::RunState> run_state;
+  cg::DevBuf<cg::RunState> run_state;  // two: BFS's front kernel reads one, writes the other
+  cg::DevBuf<unsigned long long> bfs_cnt;  // BFS first-visit counters, two parities
   cg::DevBuf<uint32_t> val_snap;       // probe mode: state restored between choices
   cg::DevBuf<uint64_t> bm_snap;
 

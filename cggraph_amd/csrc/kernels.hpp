@@ -159,6 +159,50 @@ int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, cons
 int launch_fused_finish(const StatsPart* parts, uint32_t nparts, uint32_t* stripes, RunState* rs, DevStats* st,
                         DevStats* next, DevStats* host_out, bool track, const float* mlp, uint32_t V, uint64_t E,
                         hipStream_t s);
+// Everything a traversal run starts from, in one launch: val (BFS/SSSP:
+// unreached, root 0; CC: identity), bm0 = the first frontier (the root, or
+// vertices [0, cc_active) for CC), bm1 cleared (bm_words words each), the
+// striped counters cleared (the minima set empty when minima_empty), both
+// statistics slots cleared (slot 1's choice -1 when choice1_none), both run
+// states with `visited0`, and -- bfs_cnt not NULL -- the BFS counters with
+// the root's counts in parity 0 (off: the graph's offsets).
+struct TraverseInit {
+  uint32_t* val;
+  uint64_t V;
+  int alg;
+  uint32_t root;
+  uint64_t cc_active;
+  uint64_t* bm0;
+  uint64_t* bm1;
+  uint64_t bm_words;
+  uint32_t* stripes;
+  int minima_empty;
+  DevStats* st;
+  int choice1_none;
+  RunState* rs;
+  unsigned long long visited0;
+  unsigned long long* bfs_cnt;
+  const uint64_t* off;
+};
+int launch_init_traverse(const TraverseInit& a, hipStream_t s);
+// BFS under the GPU schedule, two kernels per iteration (traverse.hip): the// BFS under the GPU schedule, two kernels per iteration (traverse.hip): the
+// front kernel takes the direction decision from the counts the previous
+// step left in cnt_prev (first visits, their out-edges, those without) and
+// rs_prev, stores it with the next run state rs_cur and publishes the record
+// (host_out), clears cnt_cur and st_next's queue counters, and for a push
+// step compacts `in` into the queues (st's counters, zero on entry) and
+// clears `out`; the step kernel expands (push) or searches parents bottom-up
+// and counts its first visits into cnt_cur.  Counters: kStripes stripes of
+// kBfsCntStride uint64 per parity.  launch_bfs_seed writes the root's counts
+// (frontier one) into parity 0 and clears parity 1.
+constexpr int kBfsCntStride = 16;                    // 128 bytes per stripe
+constexpr int kBfsCntWords = kStripes * kBfsCntStride;  // per parity
+int launch_bfs_seed(const uint64_t* off, uint32_t root, unsigned long long* cnt, hipStream_t s);
+int launch_bfs_iteration(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, uint64_t E,
+                         const Bins& bins, uint32_t* val, uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk,
+                         DevStats* st, DevStats* st_next, const unsigned long long* cnt_prev,
+                         unsigned long long* cnt_cur, const RunState* rs_prev, RunState* rs_cur, DevStats* host_out,
+                         bool exact, hipStream_t s, hipEvent_t step_begin = nullptr);
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);

@@ -22,6 +22,8 @@
 // :253-257 (SSSP: value+w), host_algorithm.hpp:424-429 (CC: value).  The
 // degree tiers replace WarpDonate::schedule (device_warpDonate.cuh:43-388),
 // whose CTA / warp / half-warp tiers assume 32-lane warps.
+#include <cstddef>
+
 #include "kernels.hpp"
 #include "mlp_dev.hpp"
 #include "sched.hpp"
@@ -103,50 +105,81 @@ __device__ __forceinline__ uint32_t candidate(uint32_t sv, uint64_t e, const uin
   else return sv;
 }
 
+// What the activation of a destination is folded into.  MinTrack (with
+// TRACK): the smallest activating candidate, so the next frontier's smallest
+// value is known without reading it back (every activated vertex ends at or
+// below a tracked candidate, and the smallest tracked candidate is some
+// vertex's final value).  FirstVisits: the vertices whose bit an activation
+// set first in the next bitmap (the atomicOr returns the old word), with their
+// out-degrees -- the next frontier's size and edges, counted where it is made.
+struct MinTrack {
+  static constexpr bool counts() { return false; }
+  uint32_t mn = 0xFFFFFFFFu;
+  __device__ __forceinline__ void track(uint32_t c) { mn = c < mn ? c : mn; }
+  __device__ __forceinline__ void first(uint32_t) {}
+};
+struct FirstVisits {
+  static constexpr bool counts() { return true; }
+  const uint64_t* __restrict__ off;
+  uint32_t n = 0, z = 0;
+  unsigned long long m = 0;
+  __device__ __forceinline__ void track(uint32_t) {}
+  __device__ __forceinline__ void first(uint32_t d) {
+    const uint64_t deg = off[d + 1] - off[d];
+    ++n;
+    m += deg;
+    z += deg == 0 ? 1u : 0u;
+  }
+};
+
 // The update of destination d with candidate cand, given a value `seen` read
 // earlier (values only decrease, so a stale read is never below the current
 // value and can only let an update through that the atomic then rejects).
-// TRACK: mn takes the smallest candidate that activated a destination, so
-// the next frontier's smallest value is known without reading it back (every
-// activated vertex ends at or below a tracked candidate, and the smallest
-// tracked candidate is some vertex's final value).
-template <int ALG, bool EXACT, bool TRACK = false>
+template <int ALG, bool EXACT, bool TRACK = false, class A = MinTrack>
 __device__ __forceinline__ uint32_t commit(uint32_t cand, uint32_t seen, uint32_t d, uint32_t* __restrict__ val,
-                                           uint64_t* __restrict__ out, uint32_t& mn) {
+                                           uint64_t* __restrict__ out, A& acc) {
   if (cand < seen) {
+    bool act;
+    uint32_t first;
     if constexpr (EXACT) {
       const uint32_t old = atomicMin(val + d, cand);
-      if (cand < old) {
-        atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
-        if constexpr (TRACK) mn = cand < mn ? cand : mn;
-        return old == kUnreached ? 1u : 0u;
-      }
+      act = cand < old;
+      first = old == kUnreached ? 1u : 0u;
     } else {
       (void)atomicMin(val + d, cand);
-      (void)atomicOr(reinterpret_cast<unsigned long long*>(out) + (d >> 6), 1ull << (d & 63));
-      if constexpr (TRACK) mn = cand < mn ? cand : mn;
-      return seen == kUnreached ? 1u : 0u;
+      act = true;
+      first = seen == kUnreached ? 1u : 0u;
+    }
+    if (act) {
+      unsigned long long* const wp = reinterpret_cast<unsigned long long*>(out) + (d >> 6);
+      const unsigned long long bit = 1ull << (d & 63);
+      if constexpr (A::counts()) {
+        if (!(atomicOr(wp, bit) & bit)) acc.first(d);
+      } else {
+        (void)atomicOr(wp, bit);
+      }
+      if constexpr (TRACK) acc.track(cand);
+      return first;
     }
   }
   return 0u;
 }
 
-template <int ALG, bool EXACT, bool TRACK = false>
+template <int ALG, bool EXACT, bool TRACK = false, class A = MinTrack>
 __device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_t* __restrict__ dst,
                                           const uint32_t* __restrict__ w, uint32_t* __restrict__ val,
-                                          uint64_t* __restrict__ out, uint32_t& mn) {
+                                          uint64_t* __restrict__ out, A& acc) {
   const uint32_t d = dst[e];
-  return commit<ALG, EXACT, TRACK>(candidate<ALG>(sv, e, w), val[d], d, val, out, mn);
+  return commit<ALG, EXACT, TRACK>(candidate<ALG>(sv, e, w), val[d], d, val, out, acc);
 }
 
 // Edges [s, e) of one source with value sv, lane-strided over a wavefront.
 // Four edges per lane are loaded (destination, weight, destination value)
 // before any of them is committed, so four gathers are in flight per lane.
-template <int ALG, bool EXACT, bool TRACK>
+template <int ALG, bool EXACT, bool TRACK, class A>
 __device__ __forceinline__ uint32_t relax_strided(uint32_t sv, uint64_t s, uint64_t e, uint32_t lane,
                                                   const uint32_t* __restrict__ dst, const uint32_t* __restrict__ w,
-                                                  uint32_t* __restrict__ val, uint64_t* __restrict__ out,
-                                                  uint32_t& mn) {
+                                                  uint32_t* __restrict__ val, uint64_t* __restrict__ out, A& acc) {
   uint32_t n = 0;
   uint64_t k = s + lane;
   for (; k + 3 * kWave < e; k += 4 * kWave) {
@@ -154,12 +187,12 @@ __device__ __forceinline__ uint32_t relax_strided(uint32_t sv, uint64_t s, uint6
     const uint32_t c0 = candidate<ALG>(sv, k, w), c1 = candidate<ALG>(sv, k + kWave, w);
     const uint32_t c2 = candidate<ALG>(sv, k + 2 * kWave, w), c3 = candidate<ALG>(sv, k + 3 * kWave, w);
     const uint32_t s0 = val[d0], s1 = val[d1], s2 = val[d2], s3 = val[d3];
-    n += commit<ALG, EXACT, TRACK>(c0, s0, d0, val, out, mn);
-    n += commit<ALG, EXACT, TRACK>(c1, s1, d1, val, out, mn);
-    n += commit<ALG, EXACT, TRACK>(c2, s2, d2, val, out, mn);
-    n += commit<ALG, EXACT, TRACK>(c3, s3, d3, val, out, mn);
+    n += commit<ALG, EXACT, TRACK>(c0, s0, d0, val, out, acc);
+    n += commit<ALG, EXACT, TRACK>(c1, s1, d1, val, out, acc);
+    n += commit<ALG, EXACT, TRACK>(c2, s2, d2, val, out, acc);
+    n += commit<ALG, EXACT, TRACK>(c3, s3, d3, val, out, acc);
   }
-  for (; k < e; k += kWave) n += relax<ALG, EXACT, TRACK>(sv, k, dst, w, val, out, mn);
+  for (; k < e; k += kWave) n += relax<ALG, EXACT, TRACK>(sv, k, dst, w, val, out, acc);
   return n;
 }
 
@@ -210,7 +243,52 @@ __global__ void k_init_run_state(RunState* rs, unsigned long long visited) {
   }
 }
 
-// The statistics and the compaction give each lane one 16-vertex slice of
+// One launch for everything a traversal run starts from (the reference's
+// initAlgorithm, CGgraph_engine.cuh:1259-1316, plus the engine's own state):
+// values and the first frontier, the other frontier bitmap, the striped
+// counters, both statistics slots, both run states and, for BFS, the root's
+// counts.  One kernel instead of a chain of small launches and copies.
+__global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
+  const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
+  const bool cc = a.alg == CG_CC;
+  for (uint64_t i = t0; i < a.V; i += nt) a.val[i] = cc ? (uint32_t)i : (i == a.root ? 0u : kUnreached);
+  for (uint64_t i = t0; i < a.bm_words; i += nt) {
+    uint64_t w = 0;
+    if (cc) {
+      const uint64_t lo = i * 64;
+      if (lo + 64 <= a.cc_active) w = ~0ull;
+      else if (lo < a.cc_active) w = (1ull << (a.cc_active - lo)) - 1;
+    } else if (i == (a.root >> 6)) {
+      w = 1ull << (a.root & 63);
+    }
+    a.bm0[i] = w;
+    a.bm1[i] = 0;
+  }
+  for (uint64_t i = t0; i < (uint64_t)kStripeWords; i += nt)
+    a.stripes[i] = (a.minima_empty && i >= (uint64_t)kStripeMinBase) ? 0xFFFFFFFFu : 0u;
+  constexpr uint64_t kStWords = 2 * sizeof(DevStats) / 4;
+  const uint64_t choice1 = (sizeof(DevStats) + offsetof(DevStats, choice)) / 4;
+  for (uint64_t i = t0; i < kStWords; i += nt)
+    reinterpret_cast<uint32_t*>(a.st)[i] = (a.choice1_none && i == choice1) ? 0xFFFFFFFFu : 0u;
+  if (a.bfs_cnt) {
+    const uint64_t deg = a.off[a.root + 1] - a.off[a.root];
+    for (uint64_t i = t0; i < 2ull * kBfsCntWords; i += nt)
+      a.bfs_cnt[i] = i == 0 ? 1ull : (i == 1 ? deg : (i == 2 ? (deg == 0 ? 1ull : 0ull) : 0ull));
+  }
+  if (t0 == 0) {
+    for (int k = 0; k < 2; ++k) {
+      a.rs[k].visited = a.visited0;
+      a.rs[k].iter = 0;
+      a.rs[k].pull = 0;
+      a.rs[k].m_explored = 0;
+      a.rs[k].prev_active = 0;
+      a.rs[k].pad = 0;
+    }
+  }
+}
+
+// The statistics and the compaction give each lane one 16-vertex slice of// The statistics and the compaction give each lane one 16-vertex slice of
 // the frontier bitmap (four lanes per 64-bit word, all in one wavefront), so
 // a small graph still spreads over many wavefronts and no lane walks more
 // than 16 vertices.
@@ -638,47 +716,33 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
 constexpr uint32_t kDenseBlock = 256;  // frontier vertices per 4096 that pay for staging offsets
 constexpr uint64_t kFusedMaxBlocks = 1024;  // LDS staging: four workgroups per CU
 
-// Compaction with the frontier statistics folded in (the GPU schedule's
-// SSSP / CC iterations): k_compact's two passes over each 16-vertex slice,
-// plus the counts k_stats would take from the same offsets, so the frontier
-// and its offsets are read once per iteration instead of twice.  A grid
-// capped at kFusedMaxBlocks strides over the slices (one queue-base atomic
-// per queue per workgroup and stride step); every word of `out` is written
-// here -- zero, or the hubs kept waiting -- so no other kernel clears it.
-__global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __restrict__ in,
-                                                          uint64_t* __restrict__ out, uint64_t words, DevGraph g,
-                                                          Bins bins, uint32_t* __restrict__ q_small,
-                                                          uint32_t* __restrict__ q_med,
-                                                          uint64_t* __restrict__ q_chunk,
-                                                          DevStats* __restrict__ st,
-                                                          const uint32_t* __restrict__ val, HubDefer hub,
-                                                          StatsPart* __restrict__ parts, int use_choice,
-                                                          TierTable tiers) {
+// Statistics the compaction counts while it reads the frontier's offsets.
+struct CompactAcc {
+  uint32_t c_act = 0, c_zero = 0, n_def = 0;
+  uint64_t m = 0, m_def = 0;
+  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// One compaction pass over frontier `in`, by every thread of the workgroup:
+// k_compact's two passes over each 16-vertex slice (count, then place), plus
+// the counts k_stats would take from the same offsets.  A grid capped at
+// kFusedMaxBlocks strides over the slices (block-uniform loop; one
+// queue-base atomic per queue per workgroup and stride step); every word of
+// `out` is written -- zero, or the hubs kept waiting -- so no other kernel
+// clears it.  A workgroup whose 4096 vertices hold at least kDenseBlock
+// frontier vertices stages its offsets in LDS.  queue = false: nothing is
+// queued and no hub waits (statistics only).
+__device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                 uint64_t words, const DevGraph& g, const Bins& bins,
+                                                 uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
+                                                 uint64_t* __restrict__ q_chunk, DevStats* __restrict__ st,
+                                                 const uint32_t* __restrict__ val, const HubDefer& hub, bool queue,
+                                                 CompactAcc& a) {
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   __shared__ uint32_t s_cnt[kWavesPerBlock];
   __shared__ uint64_t s_off[kWavesPerBlock][kStageEntries];
   uint64_t* const row = s_off[wave_id()];
-  // learned schedule: the choice the previous iteration's finish left in st
-  // (-1: the network deferred; the tier table's entry 0 applies).  A
-  // host-team choice queues nothing and keeps no hub waiting: the statistics
-  // are still counted and the next bitmap cleared, and the host relaxes the
-  // frontier after reading the record.
-  bool queue = true;
-  if (use_choice) {
-    const int c = st->choice;
-    const int tier = c > 0 ? (c >> 1) & 3 : 0;
-    bins.bin_wave = tiers.wave[tier];
-    bins.bin_chunk = tiers.chunk[tier];
-    if (c >= 0 && (c & 8)) {
-      queue = false;
-      hub.degree = 0;
-      hub.track = nullptr;
-    }
-  }
-  uint32_t c_act = 0, c_zero = 0, n_def = 0;
-  uint64_t m = 0, m_def = 0;
-  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // hub threshold: the frontier's smallest value (left in st by the previous
   // iteration's finish) + delta
   uint32_t thr = 0xFFFFFFFFu;
@@ -693,10 +757,9 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
     const uint64_t wi = si / kSlicesPerWord;
     const uint64_t vbase = si * kSliceBits;
     uint32_t word = si < slices ? slice_bits(in[wi], si) : 0u;
-    c_act += (uint32_t)__popc(word);
+    a.c_act += (uint32_t)__popc(word);
     const bool any = __ballot(word != 0) != 0;  // wave-uniform
-    // a workgroup whose 4096 vertices hold at least kDenseBlock frontier
-    // vertices stages its offsets in LDS (block-uniform choice)
+    // block-uniform choice of the offsets' source
     const uint32_t wcnt = wave_sum_u32((uint32_t)__popc(word));
     if (lane_id() == 0) s_cnt[wave_id()] = wcnt;
     __syncthreads();
@@ -726,10 +789,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
       uint32_t c_small = 0, c_med = 0, c_chunk = 0;
       uint32_t wait = 0, wait_min = 0xFFFFFFFFu;
       slice_each(word, [&](uint32_t v, uint64_t deg) {
-        m += deg;
-        h[hist_bin(deg)]++;
+        a.m += deg;
+        a.h[hist_bin(deg)]++;
         if (deg == 0) {
-          ++c_zero;
+          ++a.c_zero;
           return;
         }
         if (hub.degree && deg >= hub.degree) {
@@ -737,8 +800,8 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
           if (x >= thr) {
             wait |= 1u << (v - vbase);
             wait_min = x < wait_min ? x : wait_min;
-            ++n_def;
-            m_def += deg;
+            ++a.n_def;
+            a.m_def += deg;
             return;
           }
         }
@@ -792,15 +855,48 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
     }
     __syncthreads();  // s_tot / s_base / s_cnt / s_off are reused by the next step
   }
+}
+
+// Compaction with the frontier statistics folded in (the GPU and learned
+// schedules' SSSP / CC iterations), so the frontier and its offsets are read
+// once per iteration instead of twice.
+__global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __restrict__ in,
+                                                          uint64_t* __restrict__ out, uint64_t words, DevGraph g,
+                                                          Bins bins, uint32_t* __restrict__ q_small,
+                                                          uint32_t* __restrict__ q_med,
+                                                          uint64_t* __restrict__ q_chunk,
+                                                          DevStats* __restrict__ st,
+                                                          const uint32_t* __restrict__ val, HubDefer hub,
+                                                          StatsPart* __restrict__ parts, int use_choice,
+                                                          TierTable tiers) {
+  // learned schedule: the choice the previous iteration's finish left in st
+  // (-1: the network deferred; the tier table's entry 0 applies).  A
+  // host-team choice queues nothing and keeps no hub waiting: the statistics
+  // are still counted and the next bitmap cleared, and the host relaxes the
+  // frontier after reading the record.
+  bool queue = true;
+  if (use_choice) {
+    const int c = st->choice;
+    const int tier = c > 0 ? (c >> 1) & 3 : 0;
+    bins.bin_wave = tiers.wave[tier];
+    bins.bin_chunk = tiers.chunk[tier];
+    if (c >= 0 && (c & 8)) {
+      queue = false;
+      hub.degree = 0;
+      hub.track = nullptr;
+    }
+  }
+  CompactAcc a;
+  compact_frontier(in, out, words, g, bins, q_small, q_med, q_chunk, st, val, hub, queue, a);
   StatsPart p{};
-  p.n_active = wave_sum_u32(c_act);
-  p.m_active = wave_sum_u64(m);
-  p.n_zero = wave_sum_u32(c_zero);
+  p.n_active = wave_sum_u32(a.c_act);
+  p.m_active = wave_sum_u64(a.m);
+  p.n_zero = wave_sum_u32(a.c_zero);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h[k]);
+  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(a.h[k]);
   p.min_val = 0xFFFFFFFFu;
-  p.n_def = wave_sum_u32(n_def);
-  p.m_def = wave_sum_u64(m_def);
+  p.n_def = wave_sum_u32(a.n_def);
+  p.m_def = wave_sum_u64(a.m_def);
   stats_block_part(p, parts);
 }
 
@@ -919,22 +1015,21 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
   }
 }
 
-template <int ALG, bool EXACT, bool TRACK>
-__global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restrict__ val,
-                                                   uint64_t* __restrict__ out,
-                                                   const uint32_t* __restrict__ q_small,
-                                                   const uint32_t* __restrict__ q_med,
-                                                   const uint64_t* __restrict__ q_chunk,
-                                                   SmallRange range, uint32_t chunk_edges,
-                                                   const DevStats* __restrict__ st,
-                                                   uint32_t* __restrict__ stripes) {
+// The push expansion's work items (the queues of a compaction), strided over
+// the wavefronts of the grid: chunks first (the heaviest), then medium
+// vertices, then groups of 64 small ones.  `range` is the part of the small
+// tier these items cover (the rest is the host team's, or another launch's)
+// and whether the chunked and medium tiers come with it.  Returns the
+// wavefront-lane's first-visit count; activations are folded into acc.
+template <int ALG, bool EXACT, bool TRACK, class A>
+__device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __restrict__ val,
+                                                 uint64_t* __restrict__ out, const uint32_t* __restrict__ q_small,
+                                                 const uint32_t* __restrict__ q_med,
+                                                 const uint64_t* __restrict__ q_chunk, SmallRange range,
+                                                 uint32_t chunk_edges, const DevStats* __restrict__ st, A& acc) {
   // The queue lengths come from the compaction; the wavefronts of a grid
-  // capped near one chip's worth of resident workgroups stride over the items
-  // (chunks first, the heaviest), so no wavefront is launched only to find
-  // nothing to do.
-  // range: the part of the small tier this launch expands (the rest is the
-  // host team's, or another launch's), and whether it also takes the chunked
-  // and medium tiers
+  // capped near one chip's worth of resident workgroups stride over the items,
+  // so no wavefront is launched only to find nothing to do.
   const uint32_t n_chunk = range.big ? st->n_chunk : 0u, n_med = range.big ? st->n_med : 0u;
   const uint32_t s_end = st->n_small < range.end ? st->n_small : range.end;
   const uint32_t s_beg = range.begin < s_end ? range.begin : s_end;
@@ -943,7 +1038,6 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
   const uint32_t lane = lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
   uint32_t n_new = 0;
-  uint32_t mn = 0xFFFFFFFFu;  // smallest activating candidate (TRACK)
   for (uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); item < n_items; item += stride) {
     if (item < n_chunk) {
       // a slice of a high-degree vertex: lanes stride its edges
@@ -952,10 +1046,10 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
       const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
       const uint64_t end = g.off[v + 1];
       const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
-      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], s, e, lane, g.dst, g.w, val, out, mn);
+      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], s, e, lane, g.dst, g.w, val, out, acc);
     } else if (item < (uint64_t)n_chunk + n_med) {
       const uint32_t v = q_med[item - n_chunk];
-      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out, mn);
+      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out, acc);
     } else {
       // 64 low-degree vertices per wavefront, their edges spread evenly over
       // the lanes: lane j of round r takes edge r*64+j of the concatenated
@@ -984,15 +1078,30 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
         const uint64_t os = __shfl(s, (int)lo, kWave);
         const uint32_t osv = __shfl(sv, (int)lo, kWave);
         const uint32_t oex = __shfl(excl, (int)lo, kWave);
-        if (j < total) n_new += relax<ALG, EXACT, TRACK>(osv, os + (j - oex), g.dst, g.w, val, out, mn);
+        if (j < total) n_new += relax<ALG, EXACT, TRACK>(osv, os + (j - oex), g.dst, g.w, val, out, acc);
       }
     }
   }
+  return n_new;
+}
+
+template <int ALG, bool EXACT, bool TRACK>
+__global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restrict__ val,
+                                                   uint64_t* __restrict__ out,
+                                                   const uint32_t* __restrict__ q_small,
+                                                   const uint32_t* __restrict__ q_med,
+                                                   const uint64_t* __restrict__ q_chunk,
+                                                   SmallRange range, uint32_t chunk_edges,
+                                                   const DevStats* __restrict__ st,
+                                                   uint32_t* __restrict__ stripes) {
+  MinTrack mt;  // smallest activating candidate (TRACK)
+  const uint32_t n_new =
+      expand_items<ALG, EXACT, TRACK>(g, val, out, q_small, q_med, q_chunk, range, chunk_edges, st, mt);
   if constexpr (ALG != ALG_CC) {
     const uint32_t tot = wave_sum_u32(n_new);
-    if (lane == 0) stripe_add(stripes, tot);
+    if (lane_id() == 0) stripe_add(stripes, tot);
   }
-  if constexpr (TRACK) stripe_min(stripes, wave_min_u32(mn));
+  if constexpr (TRACK) stripe_min(stripes, wave_min_u32(mt.mn));
 }
 
 // Bottom-up BFS (Beamer, Asanovic, Patterson, SC'12): every unreached vertex
@@ -1030,6 +1139,137 @@ __global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __res
     n_new += (uint32_t)__popcll(mask);
   }
   if (lane_id() == 0) stripe_add(stripes, n_new);
+}
+
+// BFS under the GPU schedule in two kernels per iteration.  The direction of
+// iteration k (Beamer's rule) needs the size and out-edges of frontier k; the
+// step kernel of iteration k - 1 counts exactly those where it makes the
+// frontier (first visits: the bits it set first in the next bitmap, with
+// their out-degrees), so no statistics pass over the frontier is needed.
+//
+// k_bfs_front: every workgroup takes the decision from the previous step's
+// counts and run state (the same inputs give the same decision); workgroup 0
+// stores it, the next run state and the iteration's record (published to
+// mapped host memory, `iter` last) and clears the counters this step fills
+// and the next slot's queue counters.  A push iteration then compacts the
+// frontier into the degree-tier queues, writing every word of the next bitmap
+// (zero); a bottom-up iteration leaves the next bitmap to the step, which
+// writes it whole.
+// k_bfs_step: push: the tiers of k_expand with first visits counted;
+// bottom-up: lane per vertex, an unreached vertex looks for a parent in the
+// frontier among its in-edges, a wavefront writes its word of the next
+// bitmap and counts the vertices found and their out-degrees.
+__global__ __launch_bounds__(kBlock) void k_bfs_seed(const uint64_t* __restrict__ off, uint32_t root,
+                                                     unsigned long long* __restrict__ cnt) {
+  // the root is the first frontier: what "iteration 0" reached
+  const uint64_t deg = off[root + 1] - off[root];
+  for (uint32_t i = threadIdx.x; i < 2u * kBfsCntWords; i += blockDim.x) {
+    unsigned long long x = 0;
+    if (i == 0) x = 1;
+    else if (i == 1) x = deg;
+    else if (i == 2) x = deg == 0 ? 1 : 0;
+    cnt[i] = x;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                      uint64_t words, DevGraph g, uint64_t E, Bins bins,
+                                                      uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
+                                                      uint64_t* __restrict__ q_chunk, DevStats* __restrict__ st,
+                                                      DevStats* __restrict__ st_next,
+                                                      const unsigned long long* __restrict__ cnt_prev,
+                                                      unsigned long long* __restrict__ cnt_cur,
+                                                      const RunState* __restrict__ rs_prev,
+                                                      RunState* __restrict__ rs_cur, DevStats* __restrict__ host_out) {
+  unsigned long long n = 0, m = 0, z = 0;
+#pragma unroll
+  for (int k = 0; k < kStripes; ++k) {
+    n += cnt_prev[k * kBfsCntStride];
+    m += cnt_prev[k * kBfsCntStride + 1];
+    z += cnt_prev[k * kBfsCntStride + 2];
+  }
+  const RunState r = *rs_prev;
+  // Beamer, Asanovic, Patterson (SC'12): bottom-up when the frontier's edges
+  // exceed the unexplored edges / 15, back to push when the frontier falls
+  // below V / 18 and shrinks
+  const unsigned long long m_unexplored = E > r.m_explored ? E - r.m_explored : 0ull;
+  bool pull;
+  if (!r.pull) pull = 15ull * m > m_unexplored;
+  else pull = !(18ull * n < (unsigned long long)g.V && n < (unsigned long long)r.prev_active);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    RunState w = r;
+    w.pull = pull ? 1u : 0u;
+    w.m_explored = r.m_explored + m;
+    w.prev_active = (uint32_t)n;
+    w.iter = r.iter + 1;
+    w.visited = r.visited + n;
+    *rs_cur = w;
+    for (int k = 0; k < kStripes; ++k)
+      for (int j = 0; j < 3; ++j) cnt_cur[k * kBfsCntStride + j] = 0;
+    st->choice = pull ? 1 : 0;
+    st_next->n_small = 0;
+    st_next->n_med = 0;
+    st_next->n_chunk = 0;
+    if (host_out) {
+      DevStats d{};
+      d.n_active = (uint32_t)n;
+      d.m_active = m;
+      d.n_zero_deg = (uint32_t)z;
+      d.visited = w.visited;
+      d.choice = pull ? 1 : 0;
+      d.iter = 0xFFFFFFFFu;
+      *host_out = d;
+      // the release store orders the record before `iter` at system scope
+      __hip_atomic_store(&host_out->iter, w.iter, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+  if (pull) return;  // the same in every workgroup
+  CompactAcc a;
+  compact_frontier(in, out, words, g, bins, q_small, q_med, q_chunk, st, nullptr, HubDefer{0, 1}, true, a);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(kBlock) void k_bfs_step(DevGraph g, uint32_t* __restrict__ val,
+                                                     const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                     const uint32_t* __restrict__ q_small,
+                                                     const uint32_t* __restrict__ q_med,
+                                                     const uint64_t* __restrict__ q_chunk, uint32_t chunk_edges,
+                                                     const DevStats* __restrict__ st,
+                                                     unsigned long long* __restrict__ cnt) {
+  FirstVisits fv{g.off};
+  if (st->choice == 1) {
+    const uint64_t words = ((uint64_t)g.V + 63) / 64;
+    const uint64_t nwaves = (uint64_t)gridDim.x * kWavesPerBlock;
+    for (uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); w < words; w += nwaves) {
+      const uint64_t v = w * 64 + lane_id();
+      bool found = false;
+      if (v < g.V && val[v] == kUnreached) {
+        const uint64_t s = g.in_off[v], e = g.in_off[v + 1];
+        for (uint64_t k = s; k < e; ++k) {
+          const uint32_t u = g.in_src[k];
+          if ((in[u >> 6] >> (u & 63)) & 1ull) {
+            val[v] = val[u] + 1;
+            found = true;
+            break;
+          }
+        }
+      }
+      const uint64_t mask = __ballot(found);
+      if (lane_id() == 0) out[w] = mask;
+      if (found) fv.first((uint32_t)v);
+    }
+  } else {
+    (void)expand_items<ALG_BFS, EXACT, false>(g, val, out, q_small, q_med, q_chunk, SmallRange{}, chunk_edges, st,
+                                              fv);
+  }
+  const uint32_t n = wave_sum_u32(fv.n), z = wave_sum_u32(fv.z);
+  const unsigned long long m = wave_sum_u64(fv.m);
+  if (lane_id() == 0 && n) {
+    unsigned long long* c = cnt + (blockIdx.x % kStripes) * kBfsCntStride;
+    atomicAdd(c, (unsigned long long)n);
+    atomicAdd(c + 1, m);
+    if (z) atomicAdd(c + 2, (unsigned long long)z);
+  }
 }
 
 // Bottom-up SSSP and label propagation: thread j of a workgroup takes kPullK
@@ -1216,8 +1456,8 @@ __global__ __launch_bounds__(kBlock) void k_push_bitmap(DevGraph g, uint32_t* __
     const uint64_t os = __shfl(s, (int)lo, kWave);
     const uint32_t osv = __shfl(sv, (int)lo, kWave);
     const uint32_t oex = __shfl(excl, (int)lo, kWave);
-    uint32_t mn = 0;  // not tracked here
-    if (j < total) (void)relax<ALG, true>(osv, os + (j - oex), g.dst, g.w, val, out, mn);
+    MinTrack mt;  // not tracked here
+    if (j < total) (void)relax<ALG, true>(osv, os + (j - oex), g.dst, g.w, val, out, mt);
   }
 }
 
@@ -1391,6 +1631,50 @@ int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64
   if (blocks > 4096) blocks = 4096;  // wavefronts stride over the words
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_pull_bfs, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, stripes, cond);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_init_traverse(const TraverseInit& a, hipStream_t s) {
+  uint64_t n = a.V > a.bm_words ? a.V : a.bm_words;
+  if (n < 2ull * kBfsCntWords) n = 2ull * kBfsCntWords;
+  hipLaunchKernelGGL(k_init_traverse, dim3(grid_for(n, kBlock, 65536)), dim3(kBlock), 0, s, a);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_bfs_seed(const uint64_t* off, uint32_t root, unsigned long long* cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_bfs_seed, dim3(1), dim3(kBlock), 0, s, off, root, cnt);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_bfs_iteration(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, uint64_t E,
+                         const Bins& bins, uint32_t* val, uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk,
+                         DevStats* st, DevStats* st_next, const unsigned long long* cnt_prev,
+                         unsigned long long* cnt_cur, const RunState* rs_prev, RunState* rs_cur, DevStats* host_out,
+                         bool exact, hipStream_t s, hipEvent_t step_begin) {
+  if (!g.in_off || !g.in_src) return set_error(CG_ERR_INTERNAL, "bottom-up steps without in-edges");
+  const uint64_t slices = words * kSlicesPerWord;
+  uint64_t fblocks = slices ? (slices + kBlock - 1) / kBlock : 1;
+  if (fblocks > kFusedMaxBlocks) fblocks = kFusedMaxBlocks;
+  hipLaunchKernelGGL(k_bfs_front, dim3((unsigned)fblocks), dim3(kBlock), 0, s, in, out, words, g, E, bins, q_small,
+                     q_med, q_chunk, st, st_next, cnt_prev, cnt_cur, rs_prev, rs_cur, host_out);
+  CG_LAUNCH_CHECK();
+  if (step_begin) CG_HIP(hipEventRecord(step_begin, s));
+  // one grid for both directions: the push tiers' cap, at least the
+  // bottom-up words' wavefronts up to 4096 workgroups
+  uint64_t blocks = bins.grid_cap ? bins.grid_cap : 2048;
+  uint64_t pblocks = (words + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (pblocks > 4096) pblocks = 4096;
+  if (blocks < pblocks) blocks = pblocks;
+  if (blocks < 1) blocks = 1;
+  if (exact)
+    hipLaunchKernelGGL((k_bfs_step<true>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, q_small, q_med,
+                       q_chunk, bins.chunk_edges, st, cnt_cur);
+  else
+    hipLaunchKernelGGL((k_bfs_step<false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, val, in, out, q_small,
+                       q_med, q_chunk, bins.chunk_edges, st, cnt_cur);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
