@@ -825,6 +825,9 @@ int engine_run_host(cg_engine* e, uint32_t root, cg_run_stats* rs) {
 }  // namespace cg
 
 cg_engine::~cg_engine() {
+  // speculative iterations of the last run may still be queued: the buffers
+  // they use are freed after this body
+  if (stream) (void)hipStreamSynchronize(stream);
   if (plan) cg::pr_plan_destroy(plan);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
