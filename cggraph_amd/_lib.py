@@ -61,6 +61,7 @@ class Options(ctypes.Structure):
         ("hub_degree", ctypes.c_uint32),
         ("hub_delta", ctypes.c_uint32),
         ("probe_stride", ctypes.c_uint32),
+        ("hbm_budget", ctypes.c_uint64),
     ]
 
 
@@ -81,6 +82,7 @@ class RunStats(ctypes.Structure):
         ("expand_kernel_ms", ctypes.c_double),
         ("expand_alg_bytes", ctypes.c_double),
         ("learned_fallbacks", ctypes.c_uint32),
+        ("resident_edges", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -166,6 +166,140 @@ int validate_and_adopt(cg_graph* g) {
 
 }  // namespace
 
+namespace cg {
+
+int64_t device_memory_bytes() {
+  if (!device_available()) return 0;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return (int64_t)tot;
+}
+
+// TwoLevelPartition (my_project/CGgraph/Two-level_partition.cuh:213-476) on
+// one GPU: segments of 1024 vertices over V', sorted by average out-degree
+// (ties by id) and cut into buckets at 17 / 33 / 257 by lower bound; the GPU
+// takes segments in that order while the budget lasts, charging the fixed
+// arrays (values, offsets, 1 GiB of working space) first and then 8 bytes per
+// edge for SSSP (destination + weight), 4 otherwise.  resident (may be NULL)
+// receives one flag per segment.
+int plan_partition(const cg_graph* g, int algorithm, int64_t budget, cg_partition_info* info,
+                   std::vector<uint8_t>* resident) {
+  const uint64_t seg = 1024;
+  const uint64_t Vp = (uint64_t)g->V - g->zero_tail;
+  const uint64_t nseg = (Vp + seg - 1) / seg;
+  struct S { double deg; uint32_t id; uint64_t edges; };
+  std::vector<S> segs(nseg);
+  for (uint64_t s = 0; s < nseg; ++s) {
+    const uint64_t a = s * seg, b = std::min<uint64_t>(Vp, a + seg);
+    const uint64_t edges = g->h_off[b] - g->h_off[a];
+    segs[s] = S{(double)edges / (double)(b - a), (uint32_t)s, edges};
+  }
+  std::sort(segs.begin(), segs.end(), [](const S& x, const S& y) {
+    return x.deg < y.deg || (x.deg == y.deg && x.id < y.id);
+  });
+  auto lb = [&](double t) {
+    return (uint64_t)(std::lower_bound(segs.begin(), segs.end(), t,
+                                       [](const S& x, double v) { return x.deg < v; }) - segs.begin());
+  };
+  info->segment_num = nseg;
+  info->bucket_start[0] = 0;
+  info->bucket_start[1] = lb(17.0);
+  info->bucket_start[2] = lb(33.0);
+  info->bucket_start[3] = lb(257.0);
+  info->bucket_start[4] = nseg;
+  const uint64_t fixed = 4ull * g->V + 8ull * (Vp + 1) + (1ull << 30);
+  uint64_t used = fixed;
+  bool all = budget > 0 && (int64_t)fixed <= budget;
+  if (resident) resident->assign(nseg, 0);
+  for (const S& s : segs) {
+    const uint64_t b = s.edges * (algorithm == CG_SSSP ? 8 : 4);
+    if (!all || (int64_t)(used + b) > budget) {
+      all = false;
+      break;
+    }
+    used += b;
+    if (resident) (*resident)[s.id] = 1;
+  }
+  info->can_hold_all = all ? 1 : 0;
+  info->device_bytes = used;
+  return CG_OK;
+}
+
+namespace {
+
+// The device view of a partly resident graph: the same vertices, the
+// offsets and edges of the resident segments only (a non-resident vertex has
+// no edges on the device), the bitmap of the non-resident vertices that have
+// edges, and the full offsets for degree-based counts.
+int build_resident_view(cg_engine* e, const std::vector<uint8_t>& resident) {
+  const cg_graph& g = *e->caller_g;
+  const uint64_t V = g.V, seg = 1024;
+  std::vector<uint64_t> roff((size_t)V + 1, 0);
+  std::vector<uint64_t> nonres(((size_t)V + 63) / 64 + 1, 0);
+  uint64_t er = 0, n_nonres = 0;
+  for (uint64_t v = 0; v < V; ++v) {
+    roff[v] = er;
+    const uint64_t deg = g.h_off[v + 1] - g.h_off[v];
+    const uint64_t s = v / seg;
+    const bool res = s >= resident.size() || resident[s];  // the zero-out-degree tail has no edges
+    if (res) er += deg;
+    else if (deg) {
+      nonres[v >> 6] |= 1ull << (v & 63);
+      ++n_nonres;
+    }
+  }
+  roff[V] = er;
+  std::vector<uint32_t> rdst, rw;
+  try {
+    rdst.resize(er);
+    if (g.has_w) rw.resize(er);
+  } catch (const std::bad_alloc&) {
+    return set_error(CG_ERR_OOM, "resident edges");
+  }
+  for (uint64_t v = 0; v < V; ++v) {
+    const uint64_t deg = roff[v + 1] - roff[v];
+    if (!deg) continue;
+    std::memcpy(rdst.data() + roff[v], g.h_dst.data() + g.h_off[v], deg * 4);
+    if (g.has_w) std::memcpy(rw.data() + roff[v], g.h_w.data() + g.h_off[v], deg * 4);
+  }
+  std::unique_ptr<cg_graph> rg(new (std::nothrow) cg_graph());
+  if (!rg) return set_error(CG_ERR_OOM, "resident graph object");
+  rg->V = g.V;
+  rg->E = er;
+  rg->zero_tail = g.zero_tail;
+  rg->has_w = g.has_w;
+  rg->symmetric = false;
+  rg->max_outdeg_vertex = g.max_outdeg_vertex;
+  rg->max_outdeg = g.max_outdeg;
+  CG_TRY(rg->d_off.ensure(V + 1));
+  CG_TRY(rg->d_dst.ensure(er ? er : 1));
+  if (g.has_w) CG_TRY(rg->d_w.ensure(er ? er : 1));
+  CG_HIP(hipMemcpy(rg->d_off.p, roff.data(), (V + 1) * 8, hipMemcpyHostToDevice));
+  if (er) CG_HIP(hipMemcpy(rg->d_dst.p, rdst.data(), er * 4, hipMemcpyHostToDevice));
+  if (er && g.has_w) CG_HIP(hipMemcpy(rg->d_w.p, rw.data(), er * 4, hipMemcpyHostToDevice));
+  rg->dev_valid = true;
+  CG_TRY(e->nonres_bm.ensure(nonres.size()));
+  CG_HIP(hipMemcpy(e->nonres_bm.p, nonres.data(), nonres.size() * 8, hipMemcpyHostToDevice));
+  CG_TRY(e->full_off.ensure(V + 1));
+  CG_HIP(hipMemcpy(e->full_off.p, g.h_off.data(), (V + 1) * 8, hipMemcpyHostToDevice));
+  e->n_nonres = n_nonres;
+  CG_TRY(e->d_nr_ids.ensure(n_nonres ? n_nonres : 1));
+  CG_TRY(e->d_nr_vals.ensure(n_nonres ? n_nonres : 1));
+  CG_TRY(e->d_nr_cnt.ensure(1));
+  CG_TRY(e->p_nr_cnt.ensure(1));
+  e->own_g = std::move(rg);
+  e->g = e->own_g.get();
+  e->partial = true;
+  return CG_OK;
+}
+
+}  // namespace
+
+}  // namespace cg
+
 extern "C" {
 
 int cg_options_default(cg_options* o) {
@@ -370,46 +504,7 @@ int cg_partition(const cg_graph* cg, int algorithm, cg_partition_info* info) {
   if (!cg || !info) return set_error(CG_ERR_INVALID, "NULL argument");
   cg_graph* g = const_cast<cg_graph*>(cg);
   CG_TRY(g->ensure_host());
-  const uint64_t seg = 1024;
-  const uint64_t Vp = (uint64_t)g->V - g->zero_tail;
-  const uint64_t nseg = (Vp + seg - 1) / seg;
-  struct S { double deg; uint32_t id; uint64_t edges; };
-  std::vector<S> segs(nseg);
-  for (uint64_t s = 0; s < nseg; ++s) {
-    const uint64_t a = s * seg, b = std::min<uint64_t>(Vp, a + seg);
-    const uint64_t edges = g->h_off[b] - g->h_off[a];
-    segs[s] = S{(double)edges / (double)(b - a), (uint32_t)s, edges};
-  }
-  std::sort(segs.begin(), segs.end(), [](const S& x, const S& y) {
-    return x.deg < y.deg || (x.deg == y.deg && x.id < y.id);
-  });
-  auto lb = [&](double t) {
-    return (uint64_t)(std::lower_bound(segs.begin(), segs.end(), t,
-                                       [](const S& x, double v) { return x.deg < v; }) - segs.begin());
-  };
-  info->segment_num = nseg;
-  info->bucket_start[0] = 0;
-  info->bucket_start[1] = lb(17.0);
-  info->bucket_start[2] = lb(33.0);
-  info->bucket_start[3] = lb(257.0);
-  info->bucket_start[4] = nseg;
-  int64_t budget = 0;
-  if (device_available()) {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = (int64_t)tot;
-    else (void)hipGetLastError();
-  }
-  const uint64_t fixed = 4ull * g->V + 8ull * (Vp + 1) + (1ull << 30);
-  uint64_t used = fixed;
-  bool all = budget > 0;
-  for (const S& s : segs) {
-    const uint64_t b = s.edges * (algorithm == CG_SSSP ? 8 : 4);
-    if ((int64_t)(used + b) > budget) { all = false; break; }
-    used += b;
-  }
-  info->can_hold_all = all ? 1 : 0;
-  info->device_bytes = used;
-  return CG_OK;
+  return cg::plan_partition(g, algorithm, cg::device_memory_bytes(), info, nullptr);
 }
 
 int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use_device_num, const cg_options* opt,
@@ -461,7 +556,6 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   CG_TRY(require_device());
   cg_graph* g = const_cast<cg_graph*>(cg);
   if (algorithm == CG_SSSP && !g->has_w) return set_error(CG_ERR_INVALID, "SSSP needs edge weights");
-  CG_TRY(g->ensure_device());
   std::unique_ptr<cg_engine> e(new (std::nothrow) cg_engine());
   if (!e) return set_error(CG_ERR_OOM, "engine object");
   e->g = g;
@@ -471,6 +565,28 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   e->use_device = use_device_num;
   if (opt) e->opt = *opt;
   else cg_options_default(&e->opt);
+  // TwoLevelPartition's placement under the HBM budget: a graph that does
+  // not fit runs partly resident (BFS / SSSP / CC)
+  {
+    const int64_t budget = e->opt.hbm_budget ? (int64_t)e->opt.hbm_budget : device_memory_bytes();
+    if (!g->dev_valid || e->opt.hbm_budget) {
+      CG_TRY(g->ensure_host());
+      cg_partition_info pinfo{};
+      std::vector<uint8_t> resident;
+      CG_TRY(plan_partition(g, algorithm, budget, &pinfo, &resident));
+      e->segments = pinfo.segment_num;
+      if (!pinfo.can_hold_all) {
+        if (algorithm == CG_PR)
+          return set_error(CG_ERR_UNSUPPORTED, "PageRank needs the whole graph in HBM (%llu bytes over the budget)",
+                           (unsigned long long)pinfo.device_bytes);
+        if (e->opt.sched_mode == CG_SCHED_PROBE)
+          return set_error(CG_ERR_UNSUPPORTED, "probe traces need the whole graph in HBM");
+        CG_TRY(build_resident_view(e.get(), resident));
+        for (uint8_t r : resident) e->resident_segments += r;
+      }
+    }
+    if (!e->partial) CG_TRY(g->ensure_device());
+  }
   // keep private copies of the caller's strings
   if (e->opt.trace_path) { e->trace_path = e->opt.trace_path; e->opt.trace_path = e->trace_path.c_str(); }
   if (e->opt.mlp_path) { e->mlp_path = e->opt.mlp_path; e->opt.mlp_path = e->mlp_path.c_str(); }
@@ -498,7 +614,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   // (measured on R-MAT: SSSP from 2^21 vertices, BFS from 2^23; below that
   // the renumbered run is up to 15 % slower).
   const uint32_t relabel_min_v = algorithm == CG_BFS ? (1u << 23) : (1u << 21);
-  if ((algorithm == CG_BFS || algorithm == CG_SSSP) && g->V > 0 &&
+  if ((algorithm == CG_BFS || algorithm == CG_SSSP) && g->V > 0 && !e->partial &&
       (e->opt.relabel == 2 || (e->opt.relabel == 0 && g->V >= relabel_min_v))) {
     // Renumber by descending out-degree when asked, or (automatic) when the
     // top 1/64 of vertices send at least half of the edges: the values those
@@ -627,6 +743,7 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
   if (st != CG_OK) return st;
   e->result_valid = true;
   e->last_root = root;
+  rs.resident_edges = e->g->E;
   if (stats) *stats = rs;
   if (out) return cg_engine_result(e, out);
   return CG_OK;

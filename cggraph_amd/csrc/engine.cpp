@@ -44,8 +44,8 @@ double ms_since(clk::time_point t0) {
 
 int ensure_team(cg_engine* e) {
   if (e->team_ready) return CG_OK;
-  CG_TRY(e->g->ensure_host());
-  e->team.init(e->g->V, e->opt.cpu_threads);
+  CG_TRY(e->host_g()->ensure_host());
+  e->team.init(e->host_g()->V, e->opt.cpu_threads);
   e->team_ready = true;
   return CG_OK;
 }
@@ -112,7 +112,7 @@ int apply_host_candidates(cg_engine* e, uint64_t nc, uint64_t* out, cg_run_stats
 int host_relax_and_apply(cg_engine* e, uint32_t n, uint64_t* out, cg_run_stats* rs) {
   uint64_t edges = 0;
   const auto th = clk::now();
-  const uint64_t nc = e->team.relax(e->alg, e->g, e->p_ids.p, e->p_vals.p, n, e->cand_id, e->cand_val, &edges);
+  const uint64_t nc = e->team.relax(e->alg, e->host_g(), e->p_ids.p, e->p_vals.p, n, e->cand_id, e->cand_val, &edges);
   const double sec = ms_since(th) * 1e-3;
   if (edges && sec > 0) e->rate_cpu = 0.7 * e->rate_cpu + 0.3 * ((double)edges / sec);
   rs->edges_relaxed += edges;
@@ -281,7 +281,7 @@ int co_execute(cg_engine* e, const DevGraph& dg, const Bins& bins, const DevStat
       const auto t0 = clk::now();
       if (hipEventSynchronize(e->ev_copy) == hipSuccess) {
         uint64_t ed = 0, en = 0;
-        e->team.tail_worker(tid - 1, alg, e->g, e->p_ids.p, e->p_vals.p, &ends, &ed, &en);
+        e->team.tail_worker(tid - 1, alg, e->host_g(), e->p_ids.p, e->p_vals.p, &ends, &ed, &en);
 #pragma omp atomic
         host_entries += en;
 #pragma omp atomic
@@ -366,13 +366,36 @@ int exec_step(cg_engine* e, const DevGraph& dg, const Step& step, const DevStats
                              (alg == CG_SSSP ? 12.0 : 8.0) * (double)sst.m_active;
   if (step.host_share == 2) return co_execute(e, dg, bins, sst, dst, out, local);
   const uint64_t bound = expand_item_bound(sst.n_active, sst.m_active, bins);
+  // partly resident graph: the frontier's non-resident vertices (no edges on
+  // the device) are gathered with their values before the expansion and
+  // relaxed by the host team while it runs
+  uint32_t n_nr = 0;
+  if (e->partial) {
+    CG_TRY(launch_gather_masked(in, e->nonres_bm.p, words, e->val.p, e->d_nr_ids.p, e->d_nr_vals.p,
+                                e->d_nr_cnt.p, s));
+    CG_HIP(hipMemcpyAsync(e->p_nr_cnt.p, e->d_nr_cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    CG_HIP(hipStreamSynchronize(s));
+    n_nr = e->p_nr_cnt.p[0];
+    if (n_nr > e->n_nonres) return set_error(CG_ERR_INTERNAL, "non-resident frontier larger than the non-resident set");
+    if (n_nr) {
+      CG_TRY(e->p_ids.ensure(n_nr));
+      CG_TRY(e->p_vals.ensure(n_nr));
+      CG_HIP(hipMemcpyAsync(e->p_ids.p, e->d_nr_ids.p, (size_t)n_nr * 4, hipMemcpyDeviceToHost, s));
+      CG_HIP(hipMemcpyAsync(e->p_vals.p, e->d_nr_vals.p, (size_t)n_nr * 4, hipMemcpyDeviceToHost, s));
+      CG_HIP(hipEventRecord(e->ev_copy, s));
+    }
+  }
   if (e->opt.profile) CG_HIP(hipEventRecord(e->ev0, s));
   CG_TRY(launch_expand_push(alg, dg, e->val.p, out, e->q_small.p, e->q_med.p, e->q_chunk.p, bound,
                             SmallRange{}, bins, dst, e->stripes.p, e->opt.atomic_return != 0, s));
   if (e->opt.profile) { CG_HIP(hipEventRecord(e->ev1, s)); *timing_pending = true; }
   ++local->expand_launches;
-  local->gpu_vertices += sst.n_active;
+  local->gpu_vertices += sst.n_active - n_nr;
   local->edges_relaxed += sst.m_active;
+  if (n_nr) {
+    CG_HIP(hipEventSynchronize(e->ev_copy));
+    CG_TRY(host_relax_and_apply(e, n_nr, out, local));
+  }
   return CG_OK;
 }
 
@@ -432,10 +455,13 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // drain them before the host resets the statistics records they publish
   // into (their error, if any, was reported by that run).
   if (hipStreamSynchronize(s) != hipSuccess) (void)hipGetLastError();
-  if (alg == CG_BFS && mode != CG_SCHED_HEURISTIC) CG_TRY(build_transpose(&g));
+  // a partly resident graph has only its resident edges on the device: no
+  // transposed graph (bottom-up steps would miss the others)
+  const bool partial = e->partial;
+  if (alg == CG_BFS && mode != CG_SCHED_HEURISTIC && !partial) CG_TRY(build_transpose(&g));
   // bottom-up min steps (SSSP, label propagation): transposed graph + edge tiles
   const bool forced_pull = mode == CG_SCHED_FORCED && decode_choice(e->opt.forced_choice).pull;
-  const bool min_pull = (alg == CG_SSSP || alg == CG_CC) && g.E > 0 &&
+  const bool min_pull = (alg == CG_SSSP || alg == CG_CC) && g.E > 0 && !partial &&
                         ((e->opt.min_pull && (mode == CG_SCHED_GPU || mode == CG_SCHED_LEARNED)) || forced_pull);
   if (min_pull) {
     CG_TRY(build_transpose(&g));
@@ -449,7 +475,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   const bool learned = mode == CG_SCHED_LEARNED;
   const bool probe = mode == CG_SCHED_PROBE;
   if (learned) CG_TRY(mlp_load(e));
-  const bool host_possible = mode == CG_SCHED_HEURISTIC || learned || probe ||
+  const bool host_possible = partial || mode == CG_SCHED_HEURISTIC || learned || probe ||
                              (mode == CG_SCHED_FORCED && decode_choice(e->opt.forced_choice).host);
   if (host_possible) CG_TRY(ensure_team(e));
   const float* mlp_dev = (learned && e->mlp_valid) ? e->d_mlp.p : nullptr;
@@ -483,11 +509,11 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // expanded and picks the next iteration's schedule; a GPU push choice is
   // applied there (tiers from the choice), a host choice leaves the queues
   // empty and the host team runs the iteration after reading its record.
-  const bool early_learned = learned && mlp_dev && alg != CG_BFS && !min_pull && !delta;
+  const bool early_learned = learned && mlp_dev && alg != CG_BFS && !min_pull && !delta && !partial;
   // BFS under the GPU schedule: the direction rule runs on the device too, and
   // both the push pair and the bottom-up kernel are queued; each runs only
   // when the rule picked it
-  const bool early_bfs = mode == CG_SCHED_GPU && !host_possible && alg == CG_BFS && g.dev_t_valid;
+  const bool early_bfs = mode == CG_SCHED_GPU && !host_possible && alg == CG_BFS && g.dev_t_valid && !partial;
   const bool early = (mode == CG_SCHED_GPU && !host_possible && alg != CG_BFS && !min_pull) || early_learned ||
                      early_bfs;
   // SSSP hub deferral (GPU-queued iterations only): the statistics read the
@@ -683,6 +709,12 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     }
     if (alg != CG_BFS && !(min_pull && (mode != CG_SCHED_HEURISTIC))) step.ch.pull = 0;
     if (step.ch.pull) step.host_share = 0;  // bottom-up steps run on the GPU
+    if (partial) {
+      // partly resident: the GPU pushes the resident vertices, the host team
+      // the others (exec_step), every iteration
+      step.ch = Choice{0, 0, 0};
+      step.host_share = 0;
+    }
     pull = step.ch.pull;
 
     rec.iter = it;
@@ -742,8 +774,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   if (early_bfs) {
     local.edges_traversed = m_explored;
   } else if (alg != CG_CC) {
+    DevGraph tg = dg;
+    if (partial) tg.off = e->full_off.p;  // every edge of a reached vertex, resident or not
     CG_HIP(hipMemsetAsync(e->acc.p, 0, kAccSlots * sizeof(unsigned long long), s));
-    CG_TRY(launch_reached_edges(dg, e->val.p, e->acc.p, s));
+    CG_TRY(launch_reached_edges(tg, e->val.p, e->acc.p, s));
     unsigned long long m[kAccSlots];
     CG_HIP(hipMemcpyAsync(m, e->acc.p, sizeof(m), hipMemcpyDeviceToHost, s));
     CG_HIP(hipStreamSynchronize(s));
@@ -751,7 +785,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     for (int k = 0; k < kAccSlots; ++k) tot += m[k];
     local.edges_traversed = tot;
   } else {
-    local.edges_traversed = g.E;
+    local.edges_traversed = e->caller_g->E;
   }
   if (rs) *rs = local;
   return CG_OK;

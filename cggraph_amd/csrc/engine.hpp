@@ -63,6 +63,19 @@ struct cg_engine {
   cg::DevBuf<uint32_t> pull_tiles;
   bool pull_tiles_ready = false;
 
+  // partial residency (TwoLevelPartition under hbm_budget): g is a device
+  // view holding the resident segments' edges only; the host team relaxes
+  // the frontier's vertices in nonres_bm from caller_g's host CSR
+  bool partial = false;
+  cg::DevBuf<uint64_t> nonres_bm;   // non-resident vertices with edges
+  cg::DevBuf<uint64_t> full_off;    // the graph's own offsets (degrees, TEPS numerator)
+  uint64_t n_nonres = 0, segments = 0, resident_segments = 0;
+  cg::DevBuf<uint32_t> d_nr_cnt;    // non-resident frontier vertices gathered this iteration
+  cg::DevBuf<uint32_t> d_nr_ids, d_nr_vals;  // and their (vertex, value), n_nonres at most
+  cg::PinnedBuf<uint32_t> p_nr_cnt;
+  // the CSR the host team reads
+  cg_graph* host_g() const { return partial ? caller_g : g; }
+
   // PageRank
   cg::DevBuf<float> rank, rank2, contrib, contrib2;
   cg::PrPlan* plan = nullptr;

@@ -238,6 +238,10 @@ int launch_pull_min(int alg, const DevGraph& g, uint64_t E, const uint32_t* tile
 // improved vertices in `out`.
 int launch_apply_candidates(const uint32_t* ids, const uint32_t* vals, uint32_t n, uint32_t* val,
                             uint64_t* out, uint32_t* stripes, hipStream_t s);
+// (vertex, value) of the vertices set in both `in` and `mask` (words words
+// each) into ids / vals, their number into *count (cleared first).
+int launch_gather_masked(const uint64_t* in, const uint64_t* mask, uint64_t words, const uint32_t* val, uint32_t* ids,
+                         uint32_t* vals, uint32_t* count, hipStream_t s);
 // Gathers (id, value) of the frontier vertices listed in `ids`.
 int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, uint32_t* out,
                          hipStream_t s);

@@ -1401,6 +1401,31 @@ __global__ void k_apply_candidates(const uint32_t* __restrict__ ids, const uint3
   if (lane_id() == 0) stripe_add(stripes, tot);
 }
 
+// (vertex, value) of every vertex set in both `in` and `mask`, appended at
+// *count (one atomic per wavefront that found any).  Lane per word.
+__global__ __launch_bounds__(kBlock) void k_gather_masked(const uint64_t* __restrict__ in,
+                                                          const uint64_t* __restrict__ mask, uint64_t words,
+                                                          const uint32_t* __restrict__ val,
+                                                          uint32_t* __restrict__ ids, uint32_t* __restrict__ vals,
+                                                          uint32_t* __restrict__ count) {
+  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < words; base += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t w = base + threadIdx.x;
+    const uint64_t bits = w < words ? in[w] & mask[w] : 0ull;
+    const uint32_t c = (uint32_t)__popcll(bits);
+    uint32_t total;
+    const uint32_t off = wave_excl_scan(c, &total);
+    uint32_t pos = 0;
+    if (lane_id() == 0 && total) pos = atomicAdd(count, total);
+    pos = __shfl(pos, 0, kWave) + off;
+    for (uint64_t b = bits; b; b &= b - 1) {
+      const uint32_t v = (uint32_t)(w * 64 + (uint64_t)__builtin_ctzll(b));
+      ids[pos] = v;
+      vals[pos] = val[v];
+      ++pos;
+    }
+  }
+}
+
 __global__ void k_gather_values(const uint32_t* __restrict__ ids, uint32_t n,
                                 const uint32_t* __restrict__ val, uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1715,6 +1740,17 @@ int launch_apply_candidates(const uint32_t* ids, const uint32_t* vals, uint32_t 
   if (!n) return CG_OK;
   hipLaunchKernelGGL(k_apply_candidates, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ids,
                      vals, n, val, out, stripes);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+int launch_gather_masked(const uint64_t* in, const uint64_t* mask, uint64_t words, const uint32_t* val, uint32_t* ids,
+                         uint32_t* vals, uint32_t* count, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, 0u, (uint64_t)1);
+  CG_LAUNCH_CHECK();
+  if (!words) return CG_OK;
+  hipLaunchKernelGGL(k_gather_masked, dim3(grid_for(words, kBlock, 4096)), dim3(kBlock), 0, s, in, mask, words, val,
+                     ids, vals, count);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }

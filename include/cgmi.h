@@ -163,6 +163,14 @@ typedef struct cg_options {
                                      k mod probe_stride in {0, 1} (pairs of consecutive
                                      iterations, what the lagged scheduler trains on); the others
                                      follow the GPU rule.  0 or 1 = every iteration */
+  uint64_t hbm_budget;            /* bytes of HBM the partition may charge (TwoLevelPartition's
+                                     budget, Two-level_partition.cuh:392-445); 0 = the device's
+                                     memory.  A BFS / SSSP / CC graph over it runs partly
+                                     resident: the 1024-vertex segments in ascending average-degree
+                                     order keep their edges in HBM while the budget lasts, and the
+                                     host team relaxes the other segments' frontier vertices every
+                                     iteration (GPU push for the resident part, no bottom-up steps,
+                                     no renumbering).  PageRank needs the whole graph in HBM */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */
@@ -186,6 +194,7 @@ typedef struct cg_run_stats {
   double expand_kernel_ms;        /* summed HIP-event time of those launches (profile=1) */
   double expand_alg_bytes;        /* algorithmic bytes those launches moved */
   uint32_t learned_fallbacks;     /* iterations where the learned scheduler deferred */
+  uint64_t resident_edges;        /* edges held in HBM (E unless the graph ran partly resident) */
 } cg_run_stats;
 
 typedef struct cg_partition_info {
