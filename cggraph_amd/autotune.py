@@ -34,6 +34,8 @@ ENGINE_SPACE = {
 }
 BATCH_SPACE = {
     "sssp_delta": [1000, 3000, 10000, 30000, 100000],
+    "hub_degree": [0, 32, 64, 128, 512],
+    "hub_delta": [5, 20, 50, 200, 1000],
     "batch_grid": [256, 512, 1024, 2048, 4096],
     "batch_block": [256, 512, 1024],
     "batch_levels": [8, 32, 128],

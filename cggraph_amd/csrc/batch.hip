@@ -49,6 +49,9 @@ struct QueryDesc {
   int alg;
   uint32_t root;
   uint32_t delta;  // 0: no deferral
+  // SSSP hub deferral: a frontier vertex of out-degree >= hub_degree waits
+  // while its distance is >= the query's frontier minimum + hub_delta (0: off)
+  uint32_t hub_degree, hub_delta;
 };
 
 struct BatchState {
@@ -176,7 +179,7 @@ __device__ __forceinline__ void wave_append(bool push, int q, uint32_t v, uint32
     uint32_t pos = 0;
     if ((int)lane == leader) {
       pos = atomicAdd(nxt_cnt + lq, (uint32_t)__popcll(same));
-      if (qd[lq].delta) atomicMin(minv_nxt + lq, mn);
+      if (qd[lq].delta || qd[lq].hub_degree) atomicMin(minv_nxt + lq, mn);
     }
     pos = __shfl(pos, leader, 64);
     if (mine) qd[lq].front[np][pos + (uint32_t)__popcll(same & ((1ull << lane) - 1))] = v;
@@ -195,6 +198,7 @@ __global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restri
   const int cp = L & 1, np = cp ^ 1;
   uint32_t* nxt_cnt = cnt + np * nq;
   const uint32_t* thr = qs;
+  const uint32_t* minv_cur = qs + nq + cp * nq;
   uint32_t* minv_nxt = qs_mut + nq + np * nq;
   const uint32_t lane = __lane_id();
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
@@ -215,14 +219,22 @@ __global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restri
         else hi = mid - 1;
       }
       q = lo;
-      v = qd[q].front[cp][i - item_off[q]];
-      sv = qd[q].val[v];
-      if (sv >= thr[q]) {
-        // waits for a later bucket: keep it in the frontier, expand nothing
-        carry = atomicMax(qd[q].stamp + v, L + 1) < L + 1;
-      } else {
-        s = qd[q].off[v];
-        deg = (uint32_t)(qd[q].off[v + 1] - s);
+      const QueryDesc& Q = qd[q];
+      v = Q.front[cp][i - item_off[q]];
+      sv = Q.val[v];
+      s = Q.off[v];
+      deg = (uint32_t)(Q.off[v + 1] - s);
+      // a vertex waits for a later bucket (delta-stepping) or, as a hub, for
+      // its distance to come within hub_delta of the frontier's smallest:
+      // it stays in the frontier and expands nothing now
+      bool wait = sv >= thr[q];
+      if (!wait && Q.hub_degree && deg >= Q.hub_degree) {
+        const uint64_t lim = (uint64_t)minv_cur[q] + Q.hub_delta;
+        wait = (uint64_t)sv >= lim;
+      }
+      if (wait) {
+        carry = atomicMax(Q.stamp + v, L + 1) < L + 1;
+        deg = 0;
       }
     }
     wave_append(carry, q, v, sv, qd, nxt_cnt, minv_nxt, np, lane);
@@ -374,8 +386,11 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
     cg_graph* g = qgraph[i];
     uint32_t* b = pool.p + 4 * vpre[i];
     const uint32_t delta = algs[i] == CG_SSSP ? opt.sssp_delta : 0;
+    const uint32_t hub_deg = algs[i] == CG_SSSP ? opt.hub_degree : 0;
+    const uint32_t hub_delta = opt.hub_delta ? opt.hub_delta : kHubDeltaDefault;
     hq[i] = QueryDesc{g->d_off.p, g->d_dst.p, g->has_w ? g->d_w.p : nullptr, b, b + g->V,
-                      {b + 2 * (uint64_t)g->V, b + 3 * (uint64_t)g->V}, g->V, algs[i], qroot[i], delta};
+                      {b + 2 * (uint64_t)g->V, b + 3 * (uint64_t)g->V}, g->V, algs[i], qroot[i], delta, hub_deg,
+                      hub_delta};
   }
   DevBuf<QueryDesc> dq;
   DevBuf<uint64_t> dvpre;

@@ -116,3 +116,28 @@ def test_batch_renumbered_graph(oracle):
             ref = oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)
             assert np.array_equal(got, ref), (rep, a, r)
         assert st["edges_traversed"] == sum(oracle.reached_edges(o, x) for o, x in zip(qo, outs))
+
+
+@pytest.mark.parametrize("hub", [(1, 1), (16, 20), (64, 200), (8, 5)])
+@pytest.mark.parametrize("delta", [0, 3000])
+def test_batch_hub_deferral(oracle, hub, delta):
+    """SSSP hub deferral in the batch (a frontier vertex of out-degree >=
+    hub_degree waits while its distance is >= its query's frontier minimum +
+    hub_delta), alone and with delta-stepping: every output bit-exact, and
+    BFS queries in the same batch unaffected."""
+    ogs = [oracle.rmat_graph(s, 16, seed=s + 3, weights=(1, 1000)) for s in (10, 13)]
+    ogs += [oracle.grid_graph(40, 0.1, seed=2)]
+    gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
+    rng = np.random.default_rng(hub[0] + delta)
+    qg, qo, qa, qr = [], [], [], []
+    for i in range(24):
+        k = i % len(gs)
+        qg.append(gs[k])
+        qo.append(ogs[k])
+        qa.append(1 if i % 3 else 0)
+        qr.append(int(rng.integers(0, ogs[k].V)))
+    outs, st = batch.run_batch(qg, qa, qr, hub_degree=hub[0], hub_delta=hub[1], sssp_delta=delta,
+                               batch_levels=8, batch_grid=256)
+    for o, a, r, got in zip(qo, qa, qr, outs):
+        assert np.array_equal(got, oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)), (a, r, hub, delta)
+    assert st["edges_traversed"] == sum(oracle.reached_edges(o, x) for o, x in zip(qo, outs))
