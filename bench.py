@@ -75,13 +75,29 @@ def trimmed_mean(times):
     return sum(t[1:-1]) / (len(t) - 2) if len(t) > 2 else sum(t) / len(t)
 
 
+_HIP = None
+
+
+def device_sync():
+    """hipDeviceSynchronize: a query may return while iterations that find
+    nothing to do are still queued on its engine's stream."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        _HIP = ctypes.CDLL("libamdhip64.so")
+    if _HIP.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
 def timed_runs(eng, root, steps, warmup):
     for _ in range(warmup):
         eng.run(root, copy_out=False)
+    device_sync()
     t0 = time.perf_counter()
     st = None
     for _ in range(steps):
         _, st = eng.run(root, copy_out=False)
+    device_sync()
     dt = (time.perf_counter() - t0) / steps
     return dt, st
 

@@ -651,7 +651,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->q_small.ensure(V));
     CG_TRY(e->q_med.ensure(V));
     uint32_t min_chunk = e->opt.chunk_edges;
-    CG_TRY(e->q_chunk.ensure(V + g->E / min_chunk + 1));
+    CG_TRY(e->q_chunk.ensure(V + g->E / min_chunk + 2 * kChunkSpread + 1));
     CG_TRY(e->st.ensure(2));
     CG_TRY(e->hst.ensure(3));
     CG_TRY(e->hst_map.ensure(2, hipHostMallocMapped | hipHostMallocCoherent));
@@ -759,6 +759,9 @@ int cg_engine_result(cg_engine* e, void* out) {
   const void* src = e->alg == CG_PR ? (const void*)e->rank.p
                     : e->renumbered   ? (const void*)e->val_out.p
                                       : (const void*)e->val.p;
+  // the engine's stream does not synchronise with the null stream, and a run
+  // may return with no-op iterations still queued on it
+  CG_HIP(hipStreamSynchronize(e->stream));
   CG_HIP(hipMemcpy(out, src, (size_t)e->g->V * 4, hipMemcpyDeviceToHost));
   return CG_OK;
 }
@@ -766,6 +769,7 @@ int cg_engine_result(cg_engine* e, void* out) {
 int cg_engine_result_device(cg_engine* e, void** dev_ptr) {
   if (!e || !dev_ptr) return set_error(CG_ERR_INVALID, "NULL argument");
   if (e->host_only) return set_error(CG_ERR_UNSUPPORTED, "an engine without a GPU keeps its values in host memory");
+  CG_HIP(hipStreamSynchronize(e->stream));  // the values are complete when the pointer is handed out
   *dev_ptr = e->alg == CG_PR ? (void*)e->rank.p : e->renumbered ? (void*)e->val_out.p : (void*)e->val.p;
   return CG_OK;
 }

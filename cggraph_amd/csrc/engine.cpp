@@ -761,7 +761,11 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     pending = (learned && mlp_dev && probs_usable(sst)) ? sst.choice : -1;
     cur ^= 1;
   }
-  CG_HIP(hipStreamSynchronize(s));
+  // BFS on the GPU schedule: the record of the empty frontier is published
+  // after the last step has finished, so the values are final here; the
+  // iterations queued behind it find nothing to do and are left to drain
+  // (the next run, and every read of the values, waits for the stream)
+  if (!early_bfs) CG_HIP(hipStreamSynchronize(s));
   if (early)
     for (int k = 0; k < 4; ++k) CG_TRY(take_timing(k));
   local.iterations = it - 1;

@@ -19,7 +19,7 @@ struct DevStats {
   uint32_t n_zero_deg;         // frontier vertices with out-degree 0
   unsigned long long visited;  // vertices reached so far (BFS/SSSP), V for CC
   uint32_t n_cand_applied;     // host candidates that improved a value
-  uint32_t pad0;
+  uint32_t chunk_edges;         // BFS direction kernel: edges per chunk-tier item this iteration (0: the engine's)
   int32_t choice;              // learned-scheduler argmax, -1 when it did not run
   uint32_t iter;               // iteration index (1-based) seen by k_stats
   float probs[16];             // learned-scheduler softmax
@@ -32,6 +32,12 @@ struct DevStats {
   unsigned long long m_def, m_def_prev;
 };
 static_assert(sizeof(DevStats) % 8 == 0, "DevStats layout");
+
+// A BFS push frontier whose edges would give fewer than kChunkSpread chunk
+// items at the engine's chunk size is cut into smaller chunks (halved down to
+// one wavefront's width), so that a lone hub spreads over the machine; at
+// most 2 * kChunkSpread extra items come of it.
+constexpr uint32_t kChunkSpread = 2048;
 
 // Counters written by many wavefronts are striped over kStripes cache lines
 // so that their atomics do not serialise on one address.

@@ -580,6 +580,43 @@ __global__ __launch_bounds__(kBlock) void k_stats_reduce(const StatsPart* __rest
   stats_finish(parts, nparts, stripes, rs, V, E, mlp, st, host_out, direction, prev, min_from_stripes);
 }
 
+// Chunk-tier items of a frontier vertex: nch entries (v, c).  A lane writes
+// up to kLaneChunks of them itself; a longer run (a hub, or a small frontier
+// cut into small chunks) is listed for its wavefront and written by all 64
+// lanes once the pass is over, so one lane does not store thousands of
+// entries while the others wait.  A full list falls back to the lane.
+constexpr uint32_t kLaneChunks = 8;
+constexpr uint32_t kHubSlots = 32;
+struct WaveHubs {
+  uint32_t n;
+  uint32_t v[kHubSlots], nch[kHubSlots], base[kHubSlots];
+};
+
+__device__ __forceinline__ void put_chunks(uint64_t* __restrict__ q_chunk, uint32_t& b, uint32_t v, uint32_t nch,
+                                           WaveHubs& h) {
+  if (nch > kLaneChunks) {
+    const uint32_t k = atomicAdd(&h.n, 1u);
+    if (k < kHubSlots) {
+      h.v[k] = v;
+      h.nch[k] = nch;
+      h.base[k] = b;
+      b += nch;
+      return;
+    }
+  }
+  for (uint32_t c = 0; c < nch; ++c) q_chunk[b++] = ((uint64_t)v << 32) | c;
+}
+
+// The whole wavefront, converged, after its lanes' put_chunks calls.
+__device__ __forceinline__ void write_hub_chunks(uint64_t* __restrict__ q_chunk, const WaveHubs& h) {
+  const uint32_t n = h.n < kHubSlots ? h.n : kHubSlots;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint64_t v = (uint64_t)h.v[k] << 32;
+    const uint32_t nch = h.nch[k], base = h.base[k];
+    for (uint32_t c = lane_id(); c < nch; c += kWave) q_chunk[base + c] = v | c;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__ in, uint64_t words,
                                                     DevGraph g, Bins bins,
                                                     uint32_t* __restrict__ q_small,
@@ -593,6 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   __shared__ uint32_t s_ndef[kWavesPerBlock];
   __shared__ unsigned long long s_mdef[kWavesPerBlock];
+  __shared__ WaveHubs s_hubs[kWavesPerBlock];
   if (use_choice) {
     // the network's choice, left by k_stats (-1: it deferred to the GPU rule)
     const int c = st->choice;
@@ -687,6 +725,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
     s_tot[wave_id()][0] = t_small;
     s_tot[wave_id()][1] = t_med;
     s_tot[wave_id()][2] = t_chunk;
+    s_hubs[wave_id()].n = 0;
   }
   __syncthreads();
   if (threadIdx.x < 3) {
@@ -698,19 +737,20 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
     for (int w = 0; w < kWavesPerBlock; ++w) s_base[w][q] += base;
   }
   __syncthreads();
-  if (!any) return;
-  uint32_t b_small = s_base[wave_id()][0] + o_small;
-  uint32_t b_med = s_base[wave_id()][1] + o_med;
-  uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
-  for_each_in_slice(word, vbase, g.off, g.V, [&](uint32_t v, uint64_t deg) {
-    if (deg == 0) return;
-    if (deg < bins.bin_wave) q_small[b_small++] = v;
-    else if (deg < bins.bin_chunk) q_med[b_med++] = v;
-    else {
-      const uint32_t nch = (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
-      for (uint32_t c = 0; c < nch; ++c) q_chunk[b_chunk++] = ((uint64_t)v << 32) | c;
-    }
-  });
+  WaveHubs& hubs = s_hubs[wave_id()];
+  if (any) {
+    uint32_t b_small = s_base[wave_id()][0] + o_small;
+    uint32_t b_med = s_base[wave_id()][1] + o_med;
+    uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
+    for_each_in_slice(word, vbase, g.off, g.V, [&](uint32_t v, uint64_t deg) {
+      if (deg == 0) return;
+      if (deg < bins.bin_wave) q_small[b_small++] = v;
+      else if (deg < bins.bin_chunk) q_med[b_med++] = v;
+      else put_chunks(q_chunk, b_chunk, v, (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges), hubs);
+    });
+  }
+  __syncthreads();
+  write_hub_chunks(q_chunk, hubs);
 }
 
 constexpr uint32_t kDenseBlock = 256;  // frontier vertices per 4096 that pay for staging offsets
@@ -742,6 +782,8 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   __shared__ uint32_t s_cnt[kWavesPerBlock];
   __shared__ uint64_t s_off[kWavesPerBlock][kStageEntries];
+  __shared__ WaveHubs s_hubs[kWavesPerBlock];
+  WaveHubs& hubs = s_hubs[wave_id()];
   uint64_t* const row = s_off[wave_id()];
   // hub threshold: the frontier's smallest value (left in st by the previous
   // iteration's finish) + delta
@@ -828,6 +870,7 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
       s_tot[wave_id()][0] = t_small;
       s_tot[wave_id()][1] = t_med;
       s_tot[wave_id()][2] = t_chunk;
+      hubs.n = 0;
     }
     __syncthreads();
     if (threadIdx.x < 3) {
@@ -847,13 +890,11 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
         if (deg == 0) return;
         if (deg < bins.bin_wave) q_small[b_small++] = v;
         else if (deg < bins.bin_chunk) q_med[b_med++] = v;
-        else {
-          const uint32_t nch = (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
-          for (uint32_t c = 0; c < nch; ++c) q_chunk[b_chunk++] = ((uint64_t)v << 32) | c;
-        }
+        else put_chunks(q_chunk, b_chunk, v, (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges), hubs);
       });
     }
     __syncthreads();  // s_tot / s_base / s_cnt / s_off are reused by the next step
+    write_hub_chunks(q_chunk, hubs);  // the list is the wavefront's own until its next reset
   }
 }
 
@@ -1196,6 +1237,10 @@ __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict
   bool pull;
   if (!r.pull) pull = 15ull * m > m_unexplored;
   else pull = !(18ull * n < (unsigned long long)g.V && n < (unsigned long long)r.prev_active);
+  // chunk size of this iteration's push queues (the step kernel reads it back)
+  uint32_t ce = bins.chunk_edges;
+  while (ce > (uint32_t)kWave && m < (unsigned long long)ce * kChunkSpread) ce >>= 1;
+  bins.chunk_edges = ce;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     RunState w = r;
     w.pull = pull ? 1u : 0u;
@@ -1207,6 +1252,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict
     for (int k = 0; k < kStripes; ++k)
       for (int j = 0; j < 3; ++j) cnt_cur[k * kBfsCntStride + j] = 0;
     st->choice = pull ? 1 : 0;
+    st->chunk_edges = ce;
     st_next->n_small = 0;
     st_next->n_med = 0;
     st_next->n_chunk = 0;
@@ -1223,7 +1269,10 @@ __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict
       __hip_atomic_store(&host_out->iter, w.iter, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
-  if (pull) return;  // the same in every workgroup
+  // Both the same in every workgroup.  An empty frontier ends the traversal:
+  // the iterations queued behind this one find empty queues and zero counts
+  // (the next run's initialisation clears both bitmaps).
+  if (pull || n == 0) return;
   CompactAcc a;
   compact_frontier(in, out, words, g, bins, q_small, q_med, q_chunk, st, nullptr, HubDefer{0, 1}, true, a);
 }
@@ -1259,8 +1308,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_step(DevGraph g, uint32_t* __res
       if (found) fv.first((uint32_t)v);
     }
   } else {
-    (void)expand_items<ALG_BFS, EXACT, false>(g, val, out, q_small, q_med, q_chunk, SmallRange{}, chunk_edges, st,
-                                              fv);
+    const uint32_t ce = st->chunk_edges ? st->chunk_edges : chunk_edges;
+    (void)expand_items<ALG_BFS, EXACT, false>(g, val, out, q_small, q_med, q_chunk, SmallRange{}, ce, st, fv);
   }
   const uint32_t n = wave_sum_u32(fv.n), z = wave_sum_u32(fv.z);
   const unsigned long long m = wave_sum_u64(fv.m);
