@@ -654,7 +654,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->q_chunk.ensure(V + g->E / min_chunk + 2 * kChunkSpread + 1));
     CG_TRY(e->st.ensure(2));
     CG_TRY(e->hst.ensure(3));
-    CG_TRY(e->hst_map.ensure(2, hipHostMallocMapped | hipHostMallocCoherent));
+    CG_TRY(e->hst_map.ensure(3, hipHostMallocMapped | hipHostMallocCoherent));  // two iterations, query done
     e->hst_map_dev = e->hst_map.device_ptr();
     if (!e->hst_map_dev) return set_error(CG_ERR_HIP, "mapped statistics buffer has no device address");
     CG_TRY(e->acc.ensure(kAccSlots));
@@ -729,14 +729,8 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
     rs.edges_relaxed = g.E;
     rs.expand_alg_bytes = 8.0 * ((double)g.V + 1) + 4.0 * (double)g.E + 8.0 * (double)g.V;
   } else if (e->renumbered) {
-    // the query's time includes bringing the values back to the caller's ids
+    // the query's last pass brings the values back to the caller's ids
     st = engine_run(e, e->new_of[root], &rs);
-    if (st == CG_OK) {
-      const auto t0 = std::chrono::steady_clock::now();
-      st = launch_renumber_back(e->new_of_d.p, e->val.p, g.V, e->val_out.p, e->stream);
-      if (st == CG_OK) CG_HIP(hipStreamSynchronize(e->stream));
-      rs.ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
   } else {
     st = engine_run(e, root, &rs);
   }
@@ -761,7 +755,7 @@ int cg_engine_result(cg_engine* e, void* out) {
                                       : (const void*)e->val.p;
   // the engine's stream does not synchronise with the null stream, and a run
   // may return with no-op iterations still queued on it
-  CG_HIP(hipStreamSynchronize(e->stream));
+  CG_HIP(stream_wait(e->stream));
   CG_HIP(hipMemcpy(out, src, (size_t)e->g->V * 4, hipMemcpyDeviceToHost));
   return CG_OK;
 }
@@ -769,7 +763,7 @@ int cg_engine_result(cg_engine* e, void* out) {
 int cg_engine_result_device(cg_engine* e, void** dev_ptr) {
   if (!e || !dev_ptr) return set_error(CG_ERR_INVALID, "NULL argument");
   if (e->host_only) return set_error(CG_ERR_UNSUPPORTED, "an engine without a GPU keeps its values in host memory");
-  CG_HIP(hipStreamSynchronize(e->stream));  // the values are complete when the pointer is handed out
+  CG_HIP(stream_wait(e->stream));  // the values are complete when the pointer is handed out
   *dev_ptr = e->alg == CG_PR ? (void*)e->rank.p : e->renumbered ? (void*)e->val_out.p : (void*)e->val.p;
   return CG_OK;
 }

@@ -38,6 +38,10 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
+// BFS on the GPU schedule: a frontier with fewer out-edges than this, not
+// larger than the one before, ends the queueing ahead (see engine_run)
+constexpr unsigned long long kBfsTailEdges = 256;
+
 double ms_since(clk::time_point t0) {
   return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
 }
@@ -172,9 +176,10 @@ bool probs_usable(const DevStats& s) {
 // synchronisation if the kernel takes longer.  Copies the stats to hst_last.
 int wait_stats(cg_engine* e, int slot, uint32_t it) {
   DevStats* h = e->hst_map.p + slot;
+  const uint32_t tag = record_tag(e->epoch, it);
   const auto t0 = clk::now();
   for (uint32_t spin = 1;; ++spin) {
-    if (__atomic_load_n(&h->iter, __ATOMIC_ACQUIRE) == it) {
+    if (__atomic_load_n(&h->iter, __ATOMIC_ACQUIRE) == tag) {
       std::memcpy(&e->hst_last, h, sizeof(DevStats));
       e->hst_last.iter = it;
       return CG_OK;
@@ -183,7 +188,7 @@ int wait_stats(cg_engine* e, int slot, uint32_t it) {
     if ((spin & 1023) == 0 && ms_since(t0) > 50.0) break;  // long iteration: block instead
   }
   CG_HIP(hipStreamSynchronize(e->stream));
-  if (__atomic_load_n(&h->iter, __ATOMIC_ACQUIRE) != it)
+  if (__atomic_load_n(&h->iter, __ATOMIC_ACQUIRE) != tag)
     return set_error(CG_ERR_INTERNAL, "statistics of iteration %u were not published", it);
   std::memcpy(&e->hst_last, h, sizeof(DevStats));
   e->hst_last.iter = it;
@@ -454,7 +459,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // A run that failed part-way can leave queued-ahead kernels on the stream;
   // drain them before the host resets the statistics records they publish
   // into (their error, if any, was reported by that run).
-  if (hipStreamSynchronize(s) != hipSuccess) (void)hipGetLastError();
+  if (!e->clean_end && stream_wait(s) != hipSuccess) (void)hipGetLastError();
+  e->clean_end = false;
+  e->epoch = (e->epoch + 1) & 0xFFu;
+  if (!e->epoch) e->epoch = 1;
   // a partly resident graph has only its resident edges on the device: no
   // transposed graph (bottom-up steps would miss the others)
   const bool partial = e->partial;
@@ -487,7 +495,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   cg_run_stats local{};
   const auto t0 = clk::now();
   // ---- initAlgorithm (CGgraph_engine.cuh:1259-1316): one launch, below ----
-  for (int k = 0; k < 2; ++k) __atomic_store_n(&e->hst_map.p[k].iter, 0xFFFFFFFFu, __ATOMIC_RELEASE);
+  for (int k = 0; k < 3; ++k) __atomic_store_n(&e->hst_map.p[k].iter, 0u, __ATOMIC_RELEASE);
   if (host_possible) e->team.reset_values(alg, root);
 
   int cur = 0;
@@ -563,9 +571,19 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     // BFS on the GPU schedule counts the root into its first frontier;
     // the other paths start with the root (or, for CC, every vertex) visited
     ti.visited0 = alg == CG_CC ? V : (early_bfs ? 0 : 1);
-    // BFS: the direction of iteration 1 is taken from the root's counts
+    // BFS: the direction of iteration 1 is taken from the root's counts,
+    // and the initialisation does iteration 1's front work
     ti.bfs_cnt = early_bfs ? e->bfs_cnt.p : nullptr;
     ti.off = dg.off;
+    ti.bfs_first = early_bfs ? 1 : 0;
+    ti.E = g.E;
+    ti.bins = fbins;
+    ti.q_small = e->q_small.p;
+    ti.q_med = e->q_med.p;
+    ti.q_chunk = e->q_chunk.p;
+    ti.bfs_host_out = e->hst_map_dev + 1;
+    ti.acc = e->acc.p;
+    ti.epoch = e->epoch;
     CG_TRY(launch_init_traverse(ti, s));
   }
   bool early_timed[4] = {false, false, false, false};
@@ -582,27 +600,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   auto enqueue = [&](uint32_t k) -> int {
     const int sl = k & 1, c = (k - 1) & 1;
     DevStats* dk = e->st.p + sl;
-    if (early_bfs) {
-      // BFS: the front kernel decides the direction from the counts the
-      // previous step left and compacts a push frontier; the step kernel
-      // expands or searches bottom-up and counts the next frontier
-      const int tp = k & 3;
-      CG_TRY(launch_bfs_iteration(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, fbins, e->val.p, e->q_small.p,
-                                  e->q_med.p, e->q_chunk.p, dk, e->st.p + (sl ^ 1),
-                                  e->bfs_cnt.p + (size_t)c * kBfsCntWords, e->bfs_cnt.p + (size_t)sl * kBfsCntWords,
-                                  e->run_state.p + c, e->run_state.p + sl, e->hst_map_dev + sl,
-                                  e->opt.atomic_return != 0, s, e->opt.profile ? e->ev_pair[tp][0] : nullptr));
-      if (e->opt.profile) {
-        CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));
-        early_timed[tp] = true;
-      }
-      ++local.expand_launches;
-      return CG_OK;
-    }
     if (fused) {
       uint32_t nparts = 0;
       CG_TRY(launch_compact_fused(e->bm[c].p, e->bm[c ^ 1].p, words, dg, fbins, e->q_small.p, e->q_med.p,
-                                  e->q_chunk.p, dk, e->val.p, hub, e->parts.p, &nparts, s,
+                                  e->q_chunk.p, dk, e->val.p, hub, e->parts.p, &nparts, e->run_state.p, s,
                                   early_learned ? &choice_tiers : nullptr));
       const int tp = k & 3;
       if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
@@ -619,17 +620,15 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     }
     CG_TRY(launch_stats(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, e->parts.p, e->stripes.p, e->run_state.p,
                         mlp_dev, tval, threshold, dk, e->hst_map_dev + sl, s,
-                        early_bfs ? 1 : 0, hub_on ? e->st.p + (sl ^ 1) : nullptr, hub_on));
+                        0, hub_on ? e->st.p + (sl ^ 1) : nullptr, hub_on));
     if (!early) return CG_OK;
     CG_TRY(launch_compact(e->bm[c].p, words, dg, fbins, e->q_small.p, e->q_med.p, e->q_chunk.p, dk, cval,
-                          threshold, e->bm[c ^ 1].p, s, early_bfs ? &choice_tiers : nullptr,
-                          early_bfs ? 1u : 8u, hub));
+                          threshold, e->bm[c ^ 1].p, s, nullptr, 8u, hub));
     const int tp = k & 3;
     if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
     CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
                               ~0ull >> 8, SmallRange{}, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s,
                               hub_on));
-    if (early_bfs) CG_TRY(launch_pull_bfs(dg, e->val.p, e->bm[c].p, e->bm[c ^ 1].p, e->stripes.p, s, dk));
     if (e->opt.profile) {
       CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));
       early_timed[tp] = true;
@@ -637,10 +636,52 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     ++local.expand_launches;
     return CG_OK;
   };
+  // BFS on the GPU schedule: the front kernel of iteration k (direction,
+  // record, compaction) and its step kernel are queued separately; the
+  // initialisation did front 1.  Queued ahead, iteration it + 1 is in the
+  // stream while the host waits for record it.  A frontier of few edges that
+  // did not grow is probably among the last ones: the next loop then queues
+  // step it + 1 and front it + 2 only once record it + 1 has arrived (a front
+  // kernel publishes its record before its work), so the run ends with a
+  // single front kernel queued behind it, which finds the frontier empty.
+  uint32_t bfs_front_q = 1, bfs_step_q = 0;
+  bool bfs_ahead = true;
+  uint32_t bfs_prev_n = 0;
+  auto bfs_front = [&](uint32_t k) -> int {
+    const int sl = k & 1, c = (k - 1) & 1;
+    CG_TRY(launch_bfs_front(e->bm[c].p, e->bm[c ^ 1].p, words, dg, g.E, fbins, e->q_small.p, e->q_med.p,
+                            e->q_chunk.p, e->st.p + sl, e->st.p + (sl ^ 1), e->bfs_cnt.p + (size_t)c * kBfsCntWords,
+                            e->bfs_cnt.p + (size_t)sl * kBfsCntWords, e->run_state.p + c, e->run_state.p + sl,
+                            e->hst_map_dev + sl, s));
+    bfs_front_q = k;
+    return CG_OK;
+  };
+  auto bfs_step = [&](uint32_t k) -> int {
+    const int sl = k & 1, c = (k - 1) & 1, tp = k & 3;
+    if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
+    CG_TRY(launch_bfs_step(e->bm[c].p, e->bm[c ^ 1].p, words, dg, fbins, e->val.p, e->q_small.p, e->q_med.p,
+                           e->q_chunk.p, e->st.p + sl, e->bfs_cnt.p + (size_t)sl * kBfsCntWords,
+                           e->opt.atomic_return != 0, s));
+    if (e->opt.profile) {
+      CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));
+      early_timed[tp] = true;
+    }
+    ++local.expand_launches;
+    bfs_step_q = k;
+    return CG_OK;
+  };
+  // the fronts and steps of the iterations up to k, in stream order
+  auto bfs_queue_upto = [&](uint32_t k) -> int {
+    for (uint32_t j = bfs_step_q + 1; j <= k; ++j) {
+      if (bfs_front_q < j) CG_TRY(bfs_front(j));
+      CG_TRY(bfs_step(j));
+    }
+    return CG_OK;
+  };
   // learned schedule: the choice the previous record made for this
   // iteration (-1: the GPU rule)
   int pending = -1;
-  if (run_ahead) CG_TRY(enqueue(1));
+  if (run_ahead && !early_bfs) CG_TRY(enqueue(1));
   auto tit = clk::now();
   for (;;) {
     ++it;
@@ -648,7 +689,11 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     DevStats* dst = e->st.p + slot;
     const bool host_now = early_learned && pending >= 0 && decode_choice(pending).host;
     if (!run_ahead) CG_TRY(enqueue(it));
-    else if (!host_now) CG_TRY(enqueue(it + 1));
+    else if (early_bfs) {
+      if (bfs_ahead) CG_TRY(bfs_queue_upto(it + 1));
+    } else if (!host_now) {
+      CG_TRY(enqueue(it + 1));
+    }
     CG_TRY(wait_stats(e, slot, it));
     if (e->opt.fault_flags & CG_FAULT_HIP_ERROR)
       return set_error(CG_ERR_HIP, "injected HIP failure at iteration %u", it);
@@ -676,6 +721,14 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       local.expand_alg_bytes -= 20.0 * n_def + 12.0 * (double)m_def;
     }
     if (sst.n_active == 0) break;
+    if (early_bfs) {
+      if (!bfs_ahead) {
+        CG_TRY(bfs_queue_upto(it));
+        if (bfs_front_q < it + 1) CG_TRY(bfs_front(it + 1));
+      }
+      bfs_ahead = !(sst.m_active < kBfsTailEdges && sst.n_active <= bfs_prev_n);
+      bfs_prev_n = sst.n_active;
+    }
     if (delta && sst.n_near == 0) {
       // nothing below the threshold: move it past the smallest waiting value
       const uint64_t t = (uint64_t)sst.min_val + delta;
@@ -761,36 +814,40 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     pending = (learned && mlp_dev && probs_usable(sst)) ? sst.choice : -1;
     cur ^= 1;
   }
-  // BFS on the GPU schedule: the record of the empty frontier is published
-  // after the last step has finished, so the values are final here; the
-  // iterations queued behind it find nothing to do and are left to drain
-  // (the next run, and every read of the values, waits for the stream)
-  if (!early_bfs) CG_HIP(hipStreamSynchronize(s));
+  // The query's last pass, queued behind the iterations (those queued past
+  // the empty frontier find nothing to do): the TEPS numerator -- BFS on the
+  // GPU schedule has it already, every reached vertex being in exactly one
+  // frontier whose out-edges the records count -- and, on a renumbered copy,
+  // the values in the caller's ids.  The host then waits for the query-done
+  // record that a one-lane kernel publishes behind it, with the numerator.
+  // Where the record of the empty frontier came after the last expansion
+  // (BFS and the fused SSSP / CC pipeline) and nothing else is to be done,
+  // the values are final already: the kernels still queued are left to drain
+  // (every read of the values waits for the stream).
+  const bool count_edges = !early_bfs && alg != CG_CC;
+  if (count_edges || e->renumbered) {
+    DevGraph tg = dg;
+    if (partial) tg.off = e->full_off.p;  // every edge of a reached vertex, resident or not
+    CG_TRY(launch_finish_query(tg, e->val.p, count_edges ? e->acc.p : nullptr,
+                               e->renumbered ? e->new_of_d.p : nullptr, e->val_out.p, s));
+  }
+  unsigned long long reached = 0;
+  const bool wait_done = count_edges || e->renumbered || !(early_bfs || fused);
+  if (wait_done) {
+    CG_TRY(launch_query_done(count_edges ? e->acc.p : nullptr, e->hst_map_dev + 2, record_tag(e->epoch, 0), s));
+    CG_TRY(wait_stats(e, 2, 0));
+    reached = e->hst_last.m_active;
+  }
   if (early)
     for (int k = 0; k < 4; ++k) CG_TRY(take_timing(k));
   local.iterations = it - 1;
   local.ms = ms_since(t0);
   e->cur = cur;
   if (e->trace) std::fflush(e->trace);
-  // TEPS numerator, outside the timed region.  BFS on the GPU schedule has it
-  // already: every reached vertex is in exactly one frontier, and the records
-  // carry each frontier's exact out-edge count.
-  if (early_bfs) {
-    local.edges_traversed = m_explored;
-  } else if (alg != CG_CC) {
-    DevGraph tg = dg;
-    if (partial) tg.off = e->full_off.p;  // every edge of a reached vertex, resident or not
-    CG_HIP(hipMemsetAsync(e->acc.p, 0, kAccSlots * sizeof(unsigned long long), s));
-    CG_TRY(launch_reached_edges(tg, e->val.p, e->acc.p, s));
-    unsigned long long m[kAccSlots];
-    CG_HIP(hipMemcpyAsync(m, e->acc.p, sizeof(m), hipMemcpyDeviceToHost, s));
-    CG_HIP(hipStreamSynchronize(s));
-    unsigned long long tot = 0;
-    for (int k = 0; k < kAccSlots; ++k) tot += m[k];
-    local.edges_traversed = tot;
-  } else {
-    local.edges_traversed = e->caller_g->E;
-  }
+  if (early_bfs) local.edges_traversed = m_explored;
+  else if (count_edges) local.edges_traversed = reached;
+  else local.edges_traversed = e->caller_g->E;
+  e->clean_end = true;
   if (rs) *rs = local;
   return CG_OK;
 }

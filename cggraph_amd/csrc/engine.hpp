@@ -36,6 +36,12 @@ struct cg_engine {
   cg::DevStats* hst_map_dev = nullptr;  // its device address
   cg::DevStats hst_last{};
   cg::DevBuf<unsigned long long> acc;  // kAccSlots striped accumulators
+  // run number tagged into the records (record_tag), and whether the last
+  // run returned normally: only no-op iterations can then be left on the
+  // stream, and the next run need not drain it (an error return may leave
+  // queued work and a HIP error, which the next run waits for and clears)
+  uint32_t epoch = 0;
+  bool clean_end = true;
   cg::DevBuf<cg::StatsPart> parts;     // k_stats workgroup partials
   cg::DevBuf<uint32_t> stripes;        // striped new-vertex counters
   cg::DevBuf<cg::RunState> run_state;  // two: BFS's front kernel reads one, writes the other

@@ -54,8 +54,17 @@ struct RunState {
   uint32_t pull;                   // device-side direction rule: last step was bottom-up
   unsigned long long m_explored;   // frontier edges of the iterations so far
   uint32_t prev_active;            // frontier size of the previous iteration
-  uint32_t pad;
+  uint32_t pad;                    // SSSP / CC fused pipeline: first visits of the last expansion
+  uint32_t done;                   // SSSP / CC fused pipeline: an empty frontier was counted
+  uint32_t epoch;                  // the run's number, tagged into the records it publishes
 };
+
+// A record published to mapped host memory carries, as its last store, the
+// tag of the run (low 8 bits, never 0) and the iteration (low 24 bits) it
+// belongs to, so records that iterations still queued from an earlier run
+// publish late never match what the host waits for, and the host does not
+// have to drain the stream before a run.
+constexpr uint32_t record_tag(uint32_t epoch, uint32_t iter) { return (epoch << 24) | (iter & 0xFFFFFFu); }
 
 // Per-workgroup partial of k_stats (and of the fused compaction, which also
 // counts the hubs it keeps waiting), reduced by one workgroup.
@@ -158,10 +167,12 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
 // for the next iteration in the record and in next->choice; the compaction
 // takes the tiers of the choice in st->choice (entry 0 of the table when it
 // is -1), and for a host-team choice queues nothing and keeps no hub waiting.
+// rs: the run state k_fused_finish keeps; once it has counted an empty
+// frontier, the iterations queued behind return at once.
 int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
                          uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s,
-                         const TierTable* choice_tiers = nullptr);
+                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts,
+                         const RunState* rs, hipStream_t s, const TierTable* choice_tiers = nullptr);
 int launch_fused_finish(const StatsPart* parts, uint32_t nparts, uint32_t* stripes, RunState* rs, DevStats* st,
                         DevStats* next, DevStats* host_out, bool track, const float* mlp, uint32_t V, uint64_t E,
                         hipStream_t s);
@@ -189,9 +200,22 @@ struct TraverseInit {
   unsigned long long visited0;
   unsigned long long* bfs_cnt;
   const uint64_t* off;
+  // BFS on the GPU schedule: the direction kernel's work for frontier one
+  // ({root}) is done here as well -- the decision, the root's queue entries
+  // in slot 1 (iteration one's), run state 1 and the record of iteration one
+  // (bfs_host_out) -- so iteration one is its step kernel alone
+  int bfs_first;
+  uint64_t E;
+  Bins bins;
+  uint32_t* q_small;
+  uint32_t* q_med;
+  uint64_t* q_chunk;
+  DevStats* bfs_host_out;
+  unsigned long long* acc;  // the query's striped accumulators (kAccSlots), cleared
+  uint32_t epoch;           // the run's tag (record_tag)
 };
 int launch_init_traverse(const TraverseInit& a, hipStream_t s);
-// BFS under the GPU schedule, two kernels per iteration (traverse.hip): the// BFS under the GPU schedule, two kernels per iteration (traverse.hip): the
+// BFS under the GPU schedule, two kernels per iteration (traverse.hip): the
 // front kernel takes the direction decision from the counts the previous
 // step left in cnt_prev (first visits, their out-edges, those without) and
 // rs_prev, stores it with the next run state rs_cur and publishes the record
@@ -199,16 +223,17 @@ int launch_init_traverse(const TraverseInit& a, hipStream_t s);
 // step compacts `in` into the queues (st's counters, zero on entry) and
 // clears `out`; the step kernel expands (push) or searches parents bottom-up
 // and counts its first visits into cnt_cur.  Counters: kStripes stripes of
-// kBfsCntStride uint64 per parity.  launch_bfs_seed writes the root's counts
-// (frontier one) into parity 0 and clears parity 1.
+// kBfsCntStride uint64 per parity; the traversal's initialisation writes the
+// root's counts (frontier one) into parity 0 and clears parity 1.
 constexpr int kBfsCntStride = 16;                    // 128 bytes per stripe
 constexpr int kBfsCntWords = kStripes * kBfsCntStride;  // per parity
-int launch_bfs_seed(const uint64_t* off, uint32_t root, unsigned long long* cnt, hipStream_t s);
-int launch_bfs_iteration(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, uint64_t E,
-                         const Bins& bins, uint32_t* val, uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk,
-                         DevStats* st, DevStats* st_next, const unsigned long long* cnt_prev,
-                         unsigned long long* cnt_cur, const RunState* rs_prev, RunState* rs_cur, DevStats* host_out,
-                         bool exact, hipStream_t s, hipEvent_t step_begin = nullptr);
+int launch_bfs_front(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, uint64_t E,
+                     const Bins& bins, uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
+                     DevStats* st_next, const unsigned long long* cnt_prev, unsigned long long* cnt_cur,
+                     const RunState* rs_prev, RunState* rs_cur, DevStats* host_out, hipStream_t s);
+int launch_bfs_step(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
+                    uint32_t* val, const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
+                    const DevStats* st, unsigned long long* cnt_cur, bool exact, hipStream_t s);
 // Upper bound on the expansion's work items for a frontier of n_active
 // vertices and m_active edges.
 uint64_t expand_item_bound(uint64_t n_active, uint64_t m_active, const Bins& bins);
@@ -251,11 +276,24 @@ int launch_gather_masked(const uint64_t* in, const uint64_t* mask, uint64_t word
 // Gathers (id, value) of the frontier vertices listed in `ids`.
 int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, uint32_t* out,
                          hipStream_t s);
-// Sum of out-degrees of vertices whose value != CG_UNREACHED (TEPS numerator),
-// accumulated into kStripes striped slots (acc[k * kStripeStride / 2]).
+// The query's last pass over its vertices, one kernel: with acc, the sum of
+// out-degrees of the vertices whose value != CG_UNREACHED (the TEPS
+// numerator), added into kStripes striped slots (acc[k * kStripeStride / 2],
+// zero on entry); with new_of, the values in the caller's vertex ids
+// (out[i] = val[new_of[i]]) for an engine that runs on a renumbered copy.
 constexpr int kAccSlots = kStripes * kStripeStride / 2;
-int launch_reached_edges(const DevGraph& g, const uint32_t* val, unsigned long long* acc,
-                         hipStream_t s);
+int launch_finish_query(const DevGraph& g, const uint32_t* val, unsigned long long* acc, const uint32_t* new_of,
+                        uint32_t* out, hipStream_t s);
+
+// The query's end, published like an iteration's record (m_active: the sum
+// of the striped accumulators when acc is given, iter: tag) once every kernel
+// queued before it has finished.
+int launch_query_done(const unsigned long long* acc, DevStats* host_out, uint32_t tag, hipStream_t s);
+
+// Waits for a stream by polling it: a blocking wait can sleep through the end
+// of a short kernel and take tens of microseconds to wake.  After 2 ms of
+// polling it blocks.
+hipError_t stream_wait(hipStream_t s);
 // One push step straight from a bitmap (cg_push_step_device): compaction and
 // expansion with scratch queues owned by the caller.
 int push_step_from_bitmap(int alg, const DevGraph& g, uint32_t* val, const uint64_t* in,

@@ -22,6 +22,7 @@
 // :253-257 (SSSP: value+w), host_algorithm.hpp:424-429 (CC: value).  The
 // degree tiers replace WarpDonate::schedule (device_warpDonate.cuh:43-388),
 // whose CTA / warp / half-warp tiers assume 32-lane warps.
+#include <chrono>
 #include <cstddef>
 
 #include "kernels.hpp"
@@ -239,6 +240,8 @@ __global__ void k_init_run_state(RunState* rs, unsigned long long visited) {
     rs->pull = 0;
     rs->m_explored = 0;
     rs->prev_active = 0;
+    rs->done = 0;
+    rs->epoch = 0;
     rs->pad = 0;
   }
 }
@@ -267,10 +270,45 @@ __global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
   }
   for (uint64_t i = t0; i < (uint64_t)kStripeWords; i += nt)
     a.stripes[i] = (a.minima_empty && i >= (uint64_t)kStripeMinBase) ? 0xFFFFFFFFu : 0u;
-  constexpr uint64_t kStWords = 2 * sizeof(DevStats) / 4;
-  const uint64_t choice1 = (sizeof(DevStats) + offsetof(DevStats, choice)) / 4;
-  for (uint64_t i = t0; i < kStWords; i += nt)
-    reinterpret_cast<uint32_t*>(a.st)[i] = (a.choice1_none && i == choice1) ? 0xFFFFFFFFu : 0u;
+  if (a.acc)
+    for (uint64_t i = t0; i < (uint64_t)kAccSlots; i += nt) a.acc[i] = 0;
+  // BFS: iteration one's direction and queues for frontier {root}, as
+  // k_bfs_front decides and compacts them (nothing explored yet, so the push
+  // rule reads 15 * deg(root) > E; the chunk size shrinks as for any frontier)
+  uint64_t f_deg = 0;
+  uint32_t f_pull = 0, f_ce = 0, f_small = 0, f_med = 0, f_chunk = 0;
+  if (a.bfs_first) {
+    f_deg = a.off[a.root + 1] - a.off[a.root];
+    f_pull = 15ull * f_deg > a.E ? 1u : 0u;
+    f_ce = a.bins.chunk_edges;
+    while (f_ce > (uint32_t)kWave && f_deg < (uint64_t)f_ce * kChunkSpread) f_ce >>= 1;
+    if (!f_pull && f_deg) {
+      if (f_deg < a.bins.bin_wave) f_small = 1;
+      else if (f_deg < a.bins.bin_chunk) f_med = 1;
+      else f_chunk = (uint32_t)((f_deg + f_ce - 1) / f_ce);
+    }
+    for (uint64_t i = t0; i < f_chunk; i += nt) a.q_chunk[i] = ((uint64_t)a.root << 32) | i;
+    if (t0 == 0 && f_small) a.q_small[0] = a.root;
+    if (t0 == 0 && f_med) a.q_med[0] = a.root;
+  }
+  // both statistics slots cleared; slot 1 (iteration one's) may carry the
+  // learned schedule's "no choice" or the BFS decision and queue counts
+  constexpr uint64_t kStU32 = sizeof(DevStats) / 4;
+  for (uint64_t i = t0; i < 2 * kStU32; i += nt) {
+    const uint64_t j = i - kStU32;  // field word of slot 1 (wraps for slot 0)
+    uint32_t x = 0;
+    if (i >= kStU32) {
+      if (a.choice1_none && j == offsetof(DevStats, choice) / 4) x = 0xFFFFFFFFu;
+      if (a.bfs_first) {
+        if (j == offsetof(DevStats, n_small) / 4) x = f_small;
+        else if (j == offsetof(DevStats, n_med) / 4) x = f_med;
+        else if (j == offsetof(DevStats, n_chunk) / 4) x = f_chunk;
+        else if (j == offsetof(DevStats, choice) / 4) x = f_pull;
+        else if (j == offsetof(DevStats, chunk_edges) / 4) x = f_ce;
+      }
+    }
+    reinterpret_cast<uint32_t*>(a.st)[i] = x;
+  }
   if (a.bfs_cnt) {
     const uint64_t deg = a.off[a.root + 1] - a.off[a.root];
     for (uint64_t i = t0; i < 2ull * kBfsCntWords; i += nt)
@@ -284,11 +322,32 @@ __global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
       a.rs[k].m_explored = 0;
       a.rs[k].prev_active = 0;
       a.rs[k].pad = 0;
+      a.rs[k].done = 0;
+      a.rs[k].epoch = a.epoch;
+    }
+    if (a.bfs_first) {
+      // run state after iteration one's decision, and its record
+      a.rs[1].visited = a.visited0 + 1;
+      a.rs[1].iter = 1;
+      a.rs[1].pull = f_pull;
+      a.rs[1].m_explored = f_deg;
+      a.rs[1].prev_active = 1;
+      DevStats d{};
+      d.n_active = 1;
+      d.m_active = f_deg;
+      d.n_zero_deg = f_deg == 0 ? 1u : 0u;
+      d.visited = a.visited0 + 1;
+      d.choice = (int32_t)f_pull;
+      d.iter = 0xFFFFFFFFu;
+      *a.bfs_host_out = d;
+      // the release store orders the record before `iter` at system scope
+      __hip_atomic_store(&a.bfs_host_out->iter, record_tag(a.epoch, 1u), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
 }
 
-// The statistics and the compaction give each lane one 16-vertex slice of// The statistics and the compaction give each lane one 16-vertex slice of
+// The statistics and the compaction give each lane one 16-vertex slice of
 // the frontier bitmap (four lanes per 64-bit word, all in one wavefront), so
 // a small graph still spreads over many wavefronts and no lane walks more
 // than 16 vertices.
@@ -515,7 +574,7 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
       for (int k = 0; k < kMlpOut; ++k) d.probs[k] = probs[k];
       d.choice = best;
     }
-    const uint32_t itv = d.iter;
+    const uint32_t itv = record_tag(rs->epoch, d.iter);
     d.iter = 0xFFFFFFFFu;
     *host_out = d;
     // the release store orders the record before `iter` at system scope
@@ -909,7 +968,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
                                                           DevStats* __restrict__ st,
                                                           const uint32_t* __restrict__ val, HubDefer hub,
                                                           StatsPart* __restrict__ parts, int use_choice,
-                                                          TierTable tiers) {
+                                                          TierTable tiers, const RunState* __restrict__ rs) {
+  // an empty frontier was counted: this iteration is queued behind the
+  // traversal's end and has nothing to do
+  if (rs->done) return;
   // learned schedule: the choice the previous iteration's finish left in st
   // (-1: the network deferred; the tier table's entry 0 applies).  A
   // host-team choice queues nothing and keeps no hub waiting: the statistics
@@ -951,6 +1013,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
                                                          DevStats* __restrict__ st, DevStats* __restrict__ next,
                                                          DevStats* __restrict__ host_out, int track,
                                                          const float* __restrict__ mlp, uint32_t V, uint64_t E) {
+  if (rs->done) return;  // queued behind the traversal's end (uniform)
   constexpr int kC = 12;  // n_active, n_zero, hist[8], n_def, spare
   __shared__ unsigned long long s_m[kBlock], s_md[kBlock];
   __shared__ uint32_t s_c[kBlock][kC];
@@ -1007,6 +1070,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
   rs->visited = d.visited + nn;
   rs->pad = nn;
   rs->iter = d.iter;
+  rs->done = d.n_active == 0 ? 1u : 0u;
   uint32_t mv = 0;
   if (track) {
     // the next frontier's smallest value: smallest activating candidate of
@@ -1048,7 +1112,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
   next->choice = choice;
   *st = d;
   if (host_out) {
-    const uint32_t itv = d.iter;
+    const uint32_t itv = record_tag(rs->epoch, d.iter);
     d.iter = 0xFFFFFFFFu;
     *host_out = d;
     // the release store orders the record before `iter` at system scope
@@ -1200,19 +1264,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __res
 // bottom-up: lane per vertex, an unreached vertex looks for a parent in the
 // frontier among its in-edges, a wavefront writes its word of the next
 // bitmap and counts the vertices found and their out-degrees.
-__global__ __launch_bounds__(kBlock) void k_bfs_seed(const uint64_t* __restrict__ off, uint32_t root,
-                                                     unsigned long long* __restrict__ cnt) {
-  // the root is the first frontier: what "iteration 0" reached
-  const uint64_t deg = off[root + 1] - off[root];
-  for (uint32_t i = threadIdx.x; i < 2u * kBfsCntWords; i += blockDim.x) {
-    unsigned long long x = 0;
-    if (i == 0) x = 1;
-    else if (i == 1) x = deg;
-    else if (i == 2) x = deg == 0 ? 1 : 0;
-    cnt[i] = x;
-  }
-}
-
+// Iteration one's front work (frontier {root}) is done by k_init_traverse.
 __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                       uint64_t words, DevGraph g, uint64_t E, Bins bins,
                                                       uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
@@ -1266,7 +1318,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict
       d.iter = 0xFFFFFFFFu;
       *host_out = d;
       // the release store orders the record before `iter` at system scope
-      __hip_atomic_store(&host_out->iter, w.iter, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&host_out->iter, record_tag(w.epoch, w.iter), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
   // Both the same in every workgroup.  An empty frontier ends the traversal:
@@ -1481,13 +1533,18 @@ __global__ void k_gather_values(const uint32_t* __restrict__ ids, uint32_t n,
   if (i < n) out[i] = val[ids[i]];
 }
 
-__global__ __launch_bounds__(kBlock) void k_reached_edges(DevGraph g, const uint32_t* __restrict__ val,
-                                                          unsigned long long* __restrict__ acc) {
+__global__ __launch_bounds__(kBlock) void k_finish_query(DevGraph g, const uint32_t* __restrict__ val,
+                                                         unsigned long long* __restrict__ acc,
+                                                         const uint32_t* __restrict__ new_of,
+                                                         uint32_t* __restrict__ out) {
   __shared__ unsigned long long s[kWavesPerBlock];
   uint64_t sum = 0;
   for (uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x; v < g.V;
-       v += (uint64_t)gridDim.x * kBlock)
-    if (val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
+       v += (uint64_t)gridDim.x * kBlock) {
+    if (acc && val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
+    if (new_of) out[v] = val[new_of[v]];
+  }
+  if (!acc) return;  // uniform
   sum = wave_sum_u64(sum);
   if (lane_id() == 0) s[wave_id()] = sum;
   __syncthreads();
@@ -1630,15 +1687,15 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
 
 int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
                          uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
-                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts, hipStream_t s,
-                         const TierTable* choice_tiers) {
+                         const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts,
+                         const RunState* rs, hipStream_t s, const TierTable* choice_tiers) {
   if (hub.degree && (!val || !hub.delta)) return set_error(CG_ERR_INVALID, "hub deferral needs values and delta >= 1");
   const uint64_t slices = words * kSlicesPerWord;
   uint64_t blocks = slices ? (slices + kBlock - 1) / kBlock : 1;
   if (blocks > kFusedMaxBlocks) blocks = kFusedMaxBlocks;
   const TierTable tiers = choice_tiers ? *choice_tiers : TierTable{};
   hipLaunchKernelGGL(k_compact_fused, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, out, words, g, bins, q_small,
-                     q_med, q_chunk, st, val, hub, parts, choice_tiers ? 1 : 0, tiers);
+                     q_med, q_chunk, st, val, hub, parts, choice_tiers ? 1 : 0, tiers, rs);
   CG_LAUNCH_CHECK();
   *nparts = (uint32_t)blocks;
   return CG_OK;
@@ -1717,25 +1774,23 @@ int launch_init_traverse(const TraverseInit& a, hipStream_t s) {
   return CG_OK;
 }
 
-int launch_bfs_seed(const uint64_t* off, uint32_t root, unsigned long long* cnt, hipStream_t s) {
-  hipLaunchKernelGGL(k_bfs_seed, dim3(1), dim3(kBlock), 0, s, off, root, cnt);
-  CG_LAUNCH_CHECK();
-  return CG_OK;
-}
-
-int launch_bfs_iteration(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, uint64_t E,
-                         const Bins& bins, uint32_t* val, uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk,
-                         DevStats* st, DevStats* st_next, const unsigned long long* cnt_prev,
-                         unsigned long long* cnt_cur, const RunState* rs_prev, RunState* rs_cur, DevStats* host_out,
-                         bool exact, hipStream_t s, hipEvent_t step_begin) {
-  if (!g.in_off || !g.in_src) return set_error(CG_ERR_INTERNAL, "bottom-up steps without in-edges");
+int launch_bfs_front(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, uint64_t E,
+                     const Bins& bins, uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
+                     DevStats* st_next, const unsigned long long* cnt_prev, unsigned long long* cnt_cur,
+                     const RunState* rs_prev, RunState* rs_cur, DevStats* host_out, hipStream_t s) {
   const uint64_t slices = words * kSlicesPerWord;
   uint64_t fblocks = slices ? (slices + kBlock - 1) / kBlock : 1;
   if (fblocks > kFusedMaxBlocks) fblocks = kFusedMaxBlocks;
   hipLaunchKernelGGL(k_bfs_front, dim3((unsigned)fblocks), dim3(kBlock), 0, s, in, out, words, g, E, bins, q_small,
                      q_med, q_chunk, st, st_next, cnt_prev, cnt_cur, rs_prev, rs_cur, host_out);
   CG_LAUNCH_CHECK();
-  if (step_begin) CG_HIP(hipEventRecord(step_begin, s));
+  return CG_OK;
+}
+
+int launch_bfs_step(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
+                    uint32_t* val, const uint32_t* q_small, const uint32_t* q_med, const uint64_t* q_chunk,
+                    const DevStats* st, unsigned long long* cnt_cur, bool exact, hipStream_t s) {
+  if (!g.in_off || !g.in_src) return set_error(CG_ERR_INTERNAL, "bottom-up steps without in-edges");
   // one grid for both directions: the push tiers' cap, at least the
   // bottom-up words' wavefronts up to 4096 workgroups
   uint64_t blocks = bins.grid_cap ? bins.grid_cap : 2048;
@@ -1813,12 +1868,43 @@ int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, u
   return CG_OK;
 }
 
-int launch_reached_edges(const DevGraph& g, const uint32_t* val, unsigned long long* acc,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_reached_edges, dim3(grid_for(g.V, kBlock, 8192)), dim3(kBlock), 0, s, g, val,
-                     acc);
+int launch_finish_query(const DevGraph& g, const uint32_t* val, unsigned long long* acc, const uint32_t* new_of,
+                        uint32_t* out, hipStream_t s) {
+  if ((!acc && !new_of) || !g.V) return CG_OK;
+  hipLaunchKernelGGL(k_finish_query, dim3(grid_for(g.V, kBlock, 8192)), dim3(kBlock), 0, s, g, val, acc, new_of,
+                     out);
   CG_LAUNCH_CHECK();
   return CG_OK;
+}
+
+__global__ void k_query_done(const unsigned long long* __restrict__ acc, DevStats* __restrict__ host_out,
+                             uint32_t tag) {
+  if (threadIdx.x != 0) return;
+  DevStats d{};
+  if (acc)
+    for (int k = 0; k < kStripes; ++k) d.m_active += acc[k * (kStripeStride / 2)];
+  d.iter = 0xFFFFFFFFu;
+  *host_out = d;
+  // the release store orders the record before `iter` at system scope
+  __hip_atomic_store(&host_out->iter, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+int launch_query_done(const unsigned long long* acc, DevStats* host_out, uint32_t tag, hipStream_t s) {
+  hipLaunchKernelGGL(k_query_done, dim3(1), dim3(64), 0, s, acc, host_out, tag);
+  CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+hipError_t stream_wait(hipStream_t s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spin = 1;; ++spin) {
+    const hipError_t r = hipStreamQuery(s);
+    if (r != hipErrorNotReady) return r;
+    if ((spin & 255) == 0 &&
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() > 2.0)
+      return hipStreamSynchronize(s);
+    __builtin_ia32_pause();
+  }
 }
 
 int push_step_from_bitmap(int alg, const DevGraph& g, uint32_t* val, const uint64_t* in,
