@@ -470,3 +470,30 @@ def test_gpu_schedule_pipelines(oracle):
     for alg in (BFS, SSSP):
         got, _ = run(gl, alg, 3, sssp_delta=200)
         assert np.array_equal(got, oracle.bfs(lat, 3) if alg == BFS else oracle.sssp(lat, 3))
+
+
+def test_back_to_back_runs(oracle):
+    """A run that leaves only no-op iterations on its stream returns without
+    draining it; the next run is queued behind them, and the records they
+    publish late carry the earlier run's tag.  Successive queries with
+    different roots, their values read or not in between, each give that
+    root's values and counts, on the caller's ids and on a renumbered copy,
+    and device-pointer reads see the finished values."""
+    og = random_graph(oracle, 23, 20000, 160000, hub=20000)
+    g = gpu_graph(og)
+    roots = [oracle.max_outdeg_vertex(og), 0, 17, 19999, 5]
+    for alg in (BFS, SSSP, CC):
+        for relabel in ((1, 2) if alg != CC else (1,)):  # never / always renumbered
+            e = cg.Engine(g, alg, sched_mode=_lib.CG_SCHED_GPU, relabel=relabel)
+            for i, r in enumerate(roots * 2):
+                ref = oracle.bfs(og, r) if alg == BFS else (oracle.sssp(og, r) if alg == SSSP else None)
+                if alg == CC:
+                    ref = oracle.cc_directed(og)
+                got, st = e.run(r, copy_out=(i % 3 == 2))
+                if got is not None:
+                    assert np.array_equal(got, ref), (alg, relabel, r)
+                if alg != CC:
+                    assert st["edges_traversed"] == oracle.reached_edges(og, ref), (alg, relabel, r)
+            assert np.array_equal(e.result(), ref), (alg, relabel)
+            e.close()
+    g.close()
