@@ -111,10 +111,17 @@ void write_probe(cg_engine* e, const IterRecord& r, uint32_t root, const double*
 
 int apply_host_candidates(cg_engine* e, uint64_t nc, uint64_t* out, cg_run_stats* rs);
 
+void shadow_ready(cg_engine* e) {
+  if (!e->shadow_stale) return;
+  e->team.reset_values(e->alg, e->shadow_root);
+  e->shadow_stale = false;
+}
+
 // Relaxes the host team's share (ids/vals already in pinned memory) and
 // applies its candidates on the GPU.
 int host_relax_and_apply(cg_engine* e, uint32_t n, uint64_t* out, cg_run_stats* rs) {
   uint64_t edges = 0;
+  shadow_ready(e);
   const auto th = clk::now();
   const uint64_t nc = e->team.relax(e->alg, e->host_g(), e->p_ids.p, e->p_vals.p, n, e->cand_id, e->cand_val, &edges);
   const double sec = ms_since(th) * 1e-3;
@@ -236,6 +243,7 @@ int co_execute(cg_engine* e, const DevGraph& dg, const Bins& bins, const DevStat
   CG_HIP(hipEventRecord(e->ev_pcie[1], s));
   QueueEnds ends;
   ends.reset(0, n_small, floor_);
+  shadow_ready(e);
   e->team.clear_candidates();
   const uint32_t min_piece = std::max<uint32_t>(4096u, n_small / 64);
   uint64_t host_entries = 0, host_edges = 0;
@@ -415,6 +423,7 @@ int probe_all(cg_engine* e, const DevGraph& dg, const DevStats& sst, DevStats* d
   CG_TRY(e->bm_snap.ensure(words + 1));
   CG_HIP(hipMemcpyAsync(e->val_snap.p, e->val.p, (size_t)V * 4, hipMemcpyDeviceToDevice, s));
   CG_HIP(hipMemcpyAsync(e->bm_snap.p, in, words * 8, hipMemcpyDeviceToDevice, s));
+  shadow_ready(e);
   std::vector<uint32_t> shadow_snap = e->team.shadow;
   CG_HIP(hipStreamSynchronize(s));
   // Choices that execute identically share one timing: a bottom-up step
@@ -496,7 +505,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   const auto t0 = clk::now();
   // ---- initAlgorithm (CGgraph_engine.cuh:1259-1316): one launch, below ----
   for (int k = 0; k < 3; ++k) __atomic_store_n(&e->hst_map.p[k].iter, 0u, __ATOMIC_RELEASE);
-  if (host_possible) e->team.reset_values(alg, root);
+  e->shadow_stale = host_possible;
+  e->shadow_root = root;
 
   int cur = 0;
   uint32_t it = 0, prev_active = 0;

@@ -57,6 +57,11 @@ struct cg_engine {
   std::vector<uint64_t> chunk_lo, chunk_hi;
   cg::HostTeam team;
   bool team_ready = false;
+  // the team's shadow values (an upper bound of the run's values that
+  // filters its candidates) are reset on the run's first host-team work, so
+  // runs that never hand work to the team skip the pass over V
+  bool shadow_stale = false;
+  uint32_t shadow_root = 0;
 
   // renumbering by out-degree (BFS / SSSP): g is an engine-owned copy whose
   // vertex new_of[v] is the caller's v; results are gathered back through it
