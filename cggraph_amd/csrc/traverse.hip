@@ -463,6 +463,9 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
   __shared__ unsigned long long s_m[kBlock];
   __shared__ uint32_t s_c[kBlock][11];
   __shared__ uint32_t s_min[kBlock];
+  __shared__ float w_mlp[kMlpLds];
+  MlpStage<kBlock> wst;  // weights loaded now, stored to LDS after the reduction
+  if (mlp) wst.load(mlp);
   unsigned long long m = 0;
   uint32_t c[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t mn = 0xFFFFFFFFu;
@@ -550,9 +553,10 @@ __device__ void stats_finish(const StatsPart* __restrict__ parts, uint32_t npart
     *st = d;
     make_features(d, V, E, visited, iter, feat);
   }
+  if (mlp) wst.store(w_mlp);
   __syncthreads();
   if (mlp) {
-    mlp_forward_block(mlp, feat, mlp_sh, probs, &best);
+    mlp_forward_lds(w_mlp, feat, mlp_sh, probs, &best);
     if (threadIdx.x < kMlpOut) st->probs[threadIdx.x] = probs[threadIdx.x];
     if (threadIdx.x == 0) {
       // the network defers (choice -1) unless its output is a finite
@@ -1017,6 +1021,11 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
   constexpr int kC = 12;  // n_active, n_zero, hist[8], n_def, spare
   __shared__ unsigned long long s_m[kBlock], s_md[kBlock];
   __shared__ uint32_t s_c[kBlock][kC];
+  __shared__ float w_mlp[kMlpLds];
+  // the network's weights are loaded now and stored to LDS after the
+  // reduction, whose latency hides theirs
+  MlpStage<kBlock> wst;
+  if (mlp) wst.load(mlp);
   unsigned long long m = 0, md = 0;
   uint32_t c[kC] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t b = threadIdx.x; b < nparts; b += kBlock) {
@@ -1089,12 +1098,13 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
   next->min_val = mv;
   make_features(d, V, E, d.visited, d.iter, feat);
   }
+  if (mlp) wst.store(w_mlp);
   __syncthreads();
   if (mlp) {
     // the learned schedule of the next iteration, from this frontier's
     // features; the network defers (-1) unless its output is a finite
     // distribution whose top choice has probability >= kLearnedMinProb
-    mlp_forward_block(mlp, feat, mlp_sh, probs, &best);
+    mlp_forward_lds(w_mlp, feat, mlp_sh, probs, &best);
   }
   if (threadIdx.x != 0) return;
   int choice = -1;
