@@ -255,10 +255,13 @@ int cg_engine_create(const cg_graph* g, int algorithm, int engine_type, int use_
                      const cg_options* opt, cg_engine** e);
 /* Runs one query.  out: uint32[V] (BFS/SSSP/CC) or float[V] (PR) in host
  * memory, or NULL to leave the result in HBM (read it with cg_engine_result).
- * stats may be NULL. */
+ * stats may be NULL.  With out == NULL the call may return while iterations
+ * that find nothing to do are still queued on the engine's stream (the
+ * values are final); the next run and both result calls wait for them. */
 int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats);
 int cg_engine_result(cg_engine* e, void* out);
-/* Device address of the engine's value array (uint32[V] or float[V]). */
+/* Device address of the engine's value array (uint32[V] or float[V]), once
+ * every kernel of the last run has finished. */
 int cg_engine_result_device(cg_engine* e, void** dev_ptr);
 void cg_engine_destroy(cg_engine* e);
 
