@@ -645,6 +645,8 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     CG_TRY(e->contrib2.ensure(V));
     CG_TRY(pr_plan_create(g, e->opt.pr_layout, e->opt.pr_segment_log2, e->opt.pr_kernel, &e->plan, e->stream));
   } else {
+    // the frontier compaction reads out-degrees rather than offset pairs
+    if (!e->partial && !e->host_only) CG_TRY(ensure_degrees(g, e->stream));
     CG_TRY(e->val.ensure(V));
     CG_TRY(e->bm[0].ensure(words));
     CG_TRY(e->bm[1].ensure(words));

@@ -138,6 +138,9 @@ struct cg_graph {
   bool dev_valid = false;
   cg::DevBuf<uint64_t> d_off;
   cg::DevBuf<uint32_t> d_dst, d_w;
+  // out-degrees u32, padded to whole 64-vertex words, built on demand
+  bool dev_deg_valid = false;
+  cg::DevBuf<uint32_t> d_deg;
   // transposed graph (in-edges sorted by source), built on demand
   bool dev_t_valid = false;
   cg::DevBuf<uint64_t> d_in_off;

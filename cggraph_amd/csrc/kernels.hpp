@@ -85,6 +85,9 @@ struct DevGraph {
   const uint32_t* in_src;
   uint32_t V;
   const uint32_t* in_w = nullptr;  // weights of the in-edges (bottom-up SSSP)
+  // out-degrees, zero-padded to whole 64-vertex words (ensure_degrees); the
+  // frontier compaction reads them instead of the offsets when present
+  const uint32_t* deg = nullptr;
 };
 
 struct Bins {
@@ -282,6 +285,8 @@ int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, u
 // zero on entry); with new_of, the values in the caller's vertex ids
 // (out[i] = val[new_of[i]]) for an engine that runs on a renumbered copy.
 constexpr int kAccSlots = kStripes * kStripeStride / 2;
+// Builds the graph's device out-degree array once (DevGraph::deg).
+int ensure_degrees(cg_graph* g, hipStream_t s);
 int launch_finish_query(const DevGraph& g, const uint32_t* val, unsigned long long* acc, const uint32_t* new_of,
                         uint32_t* out, hipStream_t s);
 

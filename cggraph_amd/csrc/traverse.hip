@@ -70,6 +70,22 @@ __device__ __forceinline__ uint32_t hist_bin(uint64_t deg) {
   return l > 7 ? 7 : l;
 }
 
+// A lane's 8-bin degree histogram in two registers of four 16-bit counts
+// (an array indexed by the bin would live in scratch memory).  A lane counts
+// at most 16 vertices per slice and a grid-stride pass gives it fewer than
+// 2^12 slices, so no count reaches 2^16.
+struct Hist8 {
+  uint64_t lo = 0, hi = 0;
+  __device__ __forceinline__ void add(uint32_t b) {
+    const uint64_t inc = 1ull << (16 * (b & 3u));
+    if (b & 4u) hi += inc;
+    else lo += inc;
+  }
+  __device__ __forceinline__ uint32_t get(int k) const {
+    return (uint32_t)(((k & 4) ? hi : lo) >> (16 * (k & 3))) & 0xFFFFu;
+  }
+};
+
 __device__ __forceinline__ void stripe_add(uint32_t* stripes, uint32_t v) {
   if (v) atomicAdd(stripes + (blockIdx.x % kStripes) * kStripeStride, v);
 }
@@ -594,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
   p.min_val = 0xFFFFFFFFu;
   uint32_t c_act = 0, c_zero = 0, c_near = 0, mn = 0xFFFFFFFFu;
   uint64_t m = 0;
-  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  Hist8 h;
   // a capped grid strides over the slices (fewer workgroup partials)
   const uint64_t slices = words * kSlicesPerWord;
   for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += (uint64_t)gridDim.x * kBlock) {
@@ -608,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
     c_act += (uint32_t)__popc(bits);
     for_each_in_slice(bits, si * kSliceBits, g.off, g.V, [&](uint32_t v, uint64_t deg) {
       m += deg;
-      h[hist_bin(deg)]++;
+      h.add(hist_bin(deg));
       c_zero += deg == 0;
       if (val) {
         const uint32_t x = val[v];
@@ -621,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const uint64_t* __restrict__ i
   p.m_active = wave_sum_u64(m);
   p.n_zero = wave_sum_u32(c_zero);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h[k]);
+  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(h.get(k));
   if (val) {
     p.n_near = wave_sum_u32(c_near);
 #pragma unroll
@@ -817,13 +833,17 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
 }
 
 constexpr uint32_t kDenseBlock = 256;  // frontier vertices per 4096 that pay for staging offsets
-constexpr uint64_t kFusedMaxBlocks = 1024;  // LDS staging: four workgroups per CU
+constexpr uint64_t kFusedMaxBlocks = 1024;     // LDS staging: four workgroups per CU
+constexpr uint64_t kFusedMaxBlocksDeg = 2048;  // degree array: eight per CU
+// each workgroup of the fused compaction writes one statistics partial
+static_assert(kFusedMaxBlocks <= kStatsMaxBlocks && kFusedMaxBlocksDeg <= kStatsMaxBlocks,
+              "the partials buffer (stats_parts_needed) holds kStatsMaxBlocks + 1 entries");
 
 // Statistics the compaction counts while it reads the frontier's offsets.
 struct CompactAcc {
   uint32_t c_act = 0, c_zero = 0, n_def = 0;
   uint64_t m = 0, m_def = 0;
-  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  Hist8 h;
 };
 
 // One compaction pass over frontier `in`, by every thread of the workgroup:
@@ -835,6 +855,15 @@ struct CompactAcc {
 // clears it.  A workgroup whose 4096 vertices hold at least kDenseBlock
 // frontier vertices stages its offsets in LDS.  queue = false: nothing is
 // queued and no hub waits (statistics only).
+//
+// DEG: the degrees come from the graph's degree array (g.deg) instead: a lane
+// with more than kFewBits frontier vertices loads its slice's 16 degrees
+// with four 16-byte loads into its own LDS row, one with fewer loads each
+// vertex's degree; no offsets are staged, so the workgroup needs a tenth of
+// the LDS and four times as many workgroups fit on a CU.
+constexpr int kFewBits = 4;
+constexpr int kDegRow = (int)kSliceBits + 1;  // odd stride: lanes' rows on distinct banks
+template <bool DEG>
 __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                  uint64_t words, const DevGraph& g, const Bins& bins,
                                                  uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
@@ -844,10 +873,12 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   __shared__ uint32_t s_cnt[kWavesPerBlock];
-  __shared__ uint64_t s_off[kWavesPerBlock][kStageEntries];
+  __shared__ uint64_t s_off[DEG ? 1 : kWavesPerBlock][DEG ? 1 : kStageEntries];
+  __shared__ uint32_t s_deg[DEG ? kWavesPerBlock : 1][DEG ? kWave * kDegRow : 1];
   __shared__ WaveHubs s_hubs[kWavesPerBlock];
   WaveHubs& hubs = s_hubs[wave_id()];
-  uint64_t* const row = s_off[wave_id()];
+  uint64_t* const row = s_off[DEG ? 0 : wave_id()];
+  uint32_t* const drow = &s_deg[DEG ? wave_id() : 0][DEG ? lane_id() * kDegRow : 0];
   // hub threshold: the frontier's smallest value (left in st by the previous
   // iteration's finish) + delta
   uint32_t thr = 0xFFFFFFFFu;
@@ -864,21 +895,41 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
     uint32_t word = si < slices ? slice_bits(in[wi], si) : 0u;
     a.c_act += (uint32_t)__popc(word);
     const bool any = __ballot(word != 0) != 0;  // wave-uniform
-    // block-uniform choice of the offsets' source
-    const uint32_t wcnt = wave_sum_u32((uint32_t)__popc(word));
-    if (lane_id() == 0) s_cnt[wave_id()] = wcnt;
-    __syncthreads();
-    uint32_t bcnt = 0;
+    bool dense = false, staged = false;
+    if constexpr (DEG) {
+      staged = __popc(word) > kFewBits;
+      if (staged) {
+        const uint4* p = reinterpret_cast<const uint4*>(g.deg + vbase);
+        const uint4 d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3];
+        const uint32_t d[16] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w,
+                                d2.x, d2.y, d2.z, d2.w, d3.x, d3.y, d3.z, d3.w};
 #pragma unroll
-    for (int w = 0; w < kWavesPerBlock; ++w) bcnt += s_cnt[w];
-    const bool dense = bcnt >= kDenseBlock;
-    if (dense) {
-      stage_offsets(g.off, g.V, (base + (uint64_t)wave_id() * kWave) * kSliceBits, row);
+        for (int k = 0; k < 16; ++k) drow[k] = d[k];
+      }
+    } else {
+      // block-uniform choice of the offsets' source
+      const uint32_t wcnt = wave_sum_u32((uint32_t)__popc(word));
+      if (lane_id() == 0) s_cnt[wave_id()] = wcnt;
       __syncthreads();
+      uint32_t bcnt = 0;
+#pragma unroll
+      for (int w = 0; w < kWavesPerBlock; ++w) bcnt += s_cnt[w];
+      dense = bcnt >= kDenseBlock;
+      if (dense) {
+        stage_offsets(g.off, g.V, (base + (uint64_t)wave_id() * kWave) * kSliceBits, row);
+        __syncthreads();
+      }
     }
     // sparse workgroups: one offset pair per frontier vertex
     auto slice_each = [&](uint32_t bits, auto&& f) {
-      if (dense) {
+      if constexpr (DEG) {
+        // a lane reads only its own LDS row, written by itself above
+        for (; bits; bits &= bits - 1) {
+          const uint32_t j = (uint32_t)__builtin_ctz(bits);
+          const uint32_t v = (uint32_t)(vbase + j);
+          f(v, (uint64_t)(staged ? drow[j] : g.deg[v]));
+        }
+      } else if (dense) {
         for_each_in_slice_staged(bits, vbase, row, f);
       } else {
         for (; bits; bits &= bits - 1) {
@@ -895,7 +946,7 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
       uint32_t wait = 0, wait_min = 0xFFFFFFFFu;
       slice_each(word, [&](uint32_t v, uint64_t deg) {
         a.m += deg;
-        a.h[hist_bin(deg)]++;
+        a.h.add(hist_bin(deg));
         if (deg == 0) {
           ++a.c_zero;
           return;
@@ -964,6 +1015,7 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
 // Compaction with the frontier statistics folded in (the GPU and learned
 // schedules' SSSP / CC iterations), so the frontier and its offsets are read
 // once per iteration instead of twice.
+template <bool DEG>
 __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __restrict__ in,
                                                           uint64_t* __restrict__ out, uint64_t words, DevGraph g,
                                                           Bins bins, uint32_t* __restrict__ q_small,
@@ -994,13 +1046,13 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __rest
     }
   }
   CompactAcc a;
-  compact_frontier(in, out, words, g, bins, q_small, q_med, q_chunk, st, val, hub, queue, a);
+  compact_frontier<DEG>(in, out, words, g, bins, q_small, q_med, q_chunk, st, val, hub, queue, a);
   StatsPart p{};
   p.n_active = wave_sum_u32(a.c_act);
   p.m_active = wave_sum_u64(a.m);
   p.n_zero = wave_sum_u32(a.c_zero);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(a.h[k]);
+  for (int k = 0; k < 8; ++k) p.hist[k] = wave_sum_u32(a.h.get(k));
   p.min_val = 0xFFFFFFFFu;
   p.n_def = wave_sum_u32(a.n_def);
   p.m_def = wave_sum_u64(a.m_def);
@@ -1275,6 +1327,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_bfs(DevGraph g, uint32_t* __res
 // frontier among its in-edges, a wavefront writes its word of the next
 // bitmap and counts the vertices found and their out-degrees.
 // Iteration one's front work (frontier {root}) is done by k_init_traverse.
+template <bool DEG>
 __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                       uint64_t words, DevGraph g, uint64_t E, Bins bins,
                                                       uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
@@ -1336,7 +1389,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict
   // (the next run's initialisation clears both bitmaps).
   if (pull || n == 0) return;
   CompactAcc a;
-  compact_frontier(in, out, words, g, bins, q_small, q_med, q_chunk, st, nullptr, HubDefer{0, 1}, true, a);
+  compact_frontier<DEG>(in, out, words, g, bins, q_small, q_med, q_chunk, st, nullptr, HubDefer{0, 1}, true, a);
 }
 
 template <bool EXACT>
@@ -1702,10 +1755,15 @@ int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, cons
   if (hub.degree && (!val || !hub.delta)) return set_error(CG_ERR_INVALID, "hub deferral needs values and delta >= 1");
   const uint64_t slices = words * kSlicesPerWord;
   uint64_t blocks = slices ? (slices + kBlock - 1) / kBlock : 1;
-  if (blocks > kFusedMaxBlocks) blocks = kFusedMaxBlocks;
+  const uint64_t cap = g.deg ? kFusedMaxBlocksDeg : kFusedMaxBlocks;
+  if (blocks > cap) blocks = cap;
   const TierTable tiers = choice_tiers ? *choice_tiers : TierTable{};
-  hipLaunchKernelGGL(k_compact_fused, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, out, words, g, bins, q_small,
-                     q_med, q_chunk, st, val, hub, parts, choice_tiers ? 1 : 0, tiers, rs);
+  if (g.deg)
+    hipLaunchKernelGGL(k_compact_fused<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, out, words, g, bins,
+                       q_small, q_med, q_chunk, st, val, hub, parts, choice_tiers ? 1 : 0, tiers, rs);
+  else
+    hipLaunchKernelGGL(k_compact_fused<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, out, words, g, bins,
+                       q_small, q_med, q_chunk, st, val, hub, parts, choice_tiers ? 1 : 0, tiers, rs);
   CG_LAUNCH_CHECK();
   *nparts = (uint32_t)blocks;
   return CG_OK;
@@ -1790,9 +1848,14 @@ int launch_bfs_front(const uint64_t* in, uint64_t* out, uint64_t words, const De
                      const RunState* rs_prev, RunState* rs_cur, DevStats* host_out, hipStream_t s) {
   const uint64_t slices = words * kSlicesPerWord;
   uint64_t fblocks = slices ? (slices + kBlock - 1) / kBlock : 1;
-  if (fblocks > kFusedMaxBlocks) fblocks = kFusedMaxBlocks;
-  hipLaunchKernelGGL(k_bfs_front, dim3((unsigned)fblocks), dim3(kBlock), 0, s, in, out, words, g, E, bins, q_small,
-                     q_med, q_chunk, st, st_next, cnt_prev, cnt_cur, rs_prev, rs_cur, host_out);
+  const uint64_t cap = g.deg ? kFusedMaxBlocksDeg : kFusedMaxBlocks;
+  if (fblocks > cap) fblocks = cap;
+  if (g.deg)
+    hipLaunchKernelGGL(k_bfs_front<true>, dim3((unsigned)fblocks), dim3(kBlock), 0, s, in, out, words, g, E, bins,
+                       q_small, q_med, q_chunk, st, st_next, cnt_prev, cnt_cur, rs_prev, rs_cur, host_out);
+  else
+    hipLaunchKernelGGL(k_bfs_front<false>, dim3((unsigned)fblocks), dim3(kBlock), 0, s, in, out, words, g, E, bins,
+                       q_small, q_med, q_chunk, st, st_next, cnt_prev, cnt_cur, rs_prev, rs_cur, host_out);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
@@ -1875,6 +1938,24 @@ int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, u
   hipLaunchKernelGGL(k_gather_values, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ids, n,
                      val, out);
   CG_LAUNCH_CHECK();
+  return CG_OK;
+}
+
+__global__ void k_degrees(const uint64_t* __restrict__ off, uint32_t V, uint64_t n, uint32_t* __restrict__ deg) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    deg[i] = i < V ? (uint32_t)(off[i + 1] - off[i]) : 0u;
+}
+
+int ensure_degrees(cg_graph* g, hipStream_t s) {
+  if (g->dev_deg_valid) return CG_OK;
+  CG_TRY(g->ensure_device());
+  const uint64_t n = ((uint64_t)g->V + 63) / 64 * 64 + 64;  // whole slices of every bitmap word
+  CG_TRY(g->d_deg.ensure(n));
+  hipLaunchKernelGGL(k_degrees, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, s, g->d_off.p, g->V, n,
+                     g->d_deg.p);
+  CG_LAUNCH_CHECK();
+  CG_HIP(hipStreamSynchronize(s));
+  g->dev_deg_valid = true;
   return CG_OK;
 }
 
