@@ -402,19 +402,26 @@ def run_extra(cg, np, info24, ceiling):
         eng.run(0, copy_out=False)
         _, st = eng.run(0, copy_out=False)
         eng.close()
-        return st["expand_kernel_ms"] / max(st["iterations"], 1), st["iterations"]
+        return st["expand_kernel_ms"] / max(st["iterations"], 1), st
 
-    # default (automatic: propagation blocking at this size)
-    per_it, iters = pr_time()
+    layouts = {4: "propagation blocking per edge", 5: "propagation blocking per (source, bin) pair, renumbered"}
+    # default (automatic: pair blocking at this size); the bytes its layout
+    # streams per iteration by design come with the run's statistics
+    per_it, st = pr_time()
     gbs = alg / (per_it * 1e-3) / 1e9
-    # streamed bytes of the blocked layout: entries 4 B read + contributions 4 B
-    # written (scatter), contributions 4 B + destinations 2 B read (gather)
-    streamed = 14.0 * inf["E"] + 12.0 * inf["V"]
-    extra["pr_s25_ef44"] = {"layout": "propagation blocking (automatic)", "ms_per_iteration": round(per_it, 4),
+    streamed = st["pr_stream_bytes"]
+    extra["pr_s25_ef44"] = {"layout": layouts.get(st["pr_layout"], str(st["pr_layout"])) + " (automatic)",
+                            "ms_per_iteration": round(per_it, 4),
                             "alg_bytes_per_iteration": alg, "GBps": round(gbs, 1),
-                            "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "E": inf["E"], "iterations": iters,
+                            "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "E": inf["E"],
+                            "iterations": st["iterations"], "streamed_bytes_per_iteration": streamed,
                             "streamed_GBps": round(streamed / (per_it * 1e-3) / 1e9, 1),
                             "frac_of_copy_rate": round(streamed / (per_it * 1e-3) / 1e9 / ceiling["copy_GBps"], 4)}
+    # the per-edge blocking it replaces
+    edge_it, st4 = pr_time(pr_layout=4)
+    extra["pr_s25_ef44_per_edge"] = {"ms_per_iteration": round(edge_it, 4),
+                                     "frac_of_peak": round(alg / (edge_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                     "streamed_bytes_per_iteration": st4["pr_stream_bytes"]}
     # the renumbered pull: one 4-byte gather per in-edge, against the measured
     # gather rate over 2^25 values
     pull_it, _ = pr_time(pr_layout=2)

@@ -83,6 +83,8 @@ class RunStats(ctypes.Structure):
         ("expand_alg_bytes", ctypes.c_double),
         ("learned_fallbacks", ctypes.c_uint32),
         ("resident_edges", ctypes.c_uint64),
+        ("pr_layout", ctypes.c_uint32),
+        ("pr_stream_bytes", ctypes.c_double),
     ]
 
     def as_dict(self):

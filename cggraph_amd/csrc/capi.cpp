@@ -705,6 +705,7 @@ int cg_engine_run(cg_engine* e, uint32_t root, void* out, cg_run_stats* stats) {
     rs.expand_kernel_ms = kms;
     rs.edges_relaxed = g.E * (uint64_t)e->opt.pr_iterations;
     rs.edges_traversed = g.E * (uint64_t)e->opt.pr_iterations;
+    pr_plan_info(e->plan, &rs.pr_layout, &rs.pr_stream_bytes);
     rs.expand_alg_bytes = (double)e->opt.pr_iterations *
                           (8.0 * (double)g.E + 8.0 * ((double)g.V + 1) + 4.0 * (double)g.V);
   } else if (e->alg == CG_CC && g.symmetric &&

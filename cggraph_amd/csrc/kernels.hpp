@@ -319,6 +319,8 @@ struct PrPlan;  // per-graph layout for the pull kernels (tiers, relabelled CSC 
 int pr_plan_create(cg_graph* g, uint32_t layout, uint32_t seg_log2, uint32_t kernel, PrPlan** plan,
                    hipStream_t s);
 void pr_plan_destroy(PrPlan* plan);
+// The layout a plan uses and the bytes it streams per iteration by design.
+void pr_plan_info(const PrPlan* plan, uint32_t* layout, double* stream_bytes);
 int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* rank, float* tmp_rank,
            float* contrib, float* contrib2, hipStream_t s, int profile, double* kernel_ms,
            uint32_t* launches);

@@ -43,10 +43,18 @@
 
 namespace cg {
 
-enum { kLayoutAuto = 0, kLayoutPlain = 1, kLayoutRelabel = 2, kLayoutSegmented = 3, kLayoutBlocked = 4 };
+enum {
+  kLayoutAuto = 0,
+  kLayoutPlain = 1,
+  kLayoutRelabel = 2,
+  kLayoutSegmented = 3,
+  kLayoutBlocked = 4,
+  kLayoutPairs = 5
+};
 
 struct PrPlan {
   int layout = kLayoutPlain;
+  double stream_bytes = 0;  // per iteration, by design (edge and pair terms + 16 B per vertex)
   // plain and relabelled pull: tiers over the rows of the CSC in use (one
   // set per pass; the two-band sweep has two passes)
   struct Tiers {
@@ -93,6 +101,17 @@ struct PrPlan {
   DevBuf<uint64_t> pb_t1_lo, pb_t1_hi, pb_t2_lo, pb_t2_hi;
   DevBuf<uint32_t> pb_t2_multi;         // tile's bin is cut into several tiles
   DevBuf<unsigned long long> pb_sum;    // fixed-point sums of the bins cut into several tiles
+  // propagation blocking over (source, bin) pairs (layout 5): pb_ent and
+  // pb_val hold one entry per pair, pb_dloc one per edge (destination within
+  // its bin | 0x8000 on a pair's first edge), pb_gbase the pair of every 64th
+  // edge; the graph is renumbered by descending out-degree (perm)
+  DevBuf<uint32_t> pb_gbase;
+  uint64_t pb_pairs = 0;
+  // bins cut into several gather tiles: each tile writes its 8192 sums to a
+  // row of its own (pb_t2_multi = row + 1), bin i's rows are
+  // [pb_mrow[i], pb_mrow[i + 1]); one pass adds them up
+  DevBuf<uint32_t> pb_mrow;
+  DevBuf<unsigned long long> pb_part;
 };
 
 namespace {
@@ -1100,6 +1119,9 @@ int build_blocked(const DevGraph& g, uint64_t E, PrPlan* p, hipStream_t s) {
   p->layout = kLayoutBlocked;
   p->pb_nb = nb;
   p->pb_narrow = E < (1ull << 31);
+  // entries 4 B read + contributions 4 B written, contributions 4 B +
+  // destinations 2 B read; contributions, degrees, ranks per vertex
+  p->stream_bytes = 14.0 * (double)E + 16.0 * (double)V;
   CG_TRY(build_degrees(g, p, s));
   // chunk bounds -> tiles of at most kPbTile edges inside one chunk
   DevBuf<uint64_t> bounds;
@@ -1197,6 +1219,398 @@ int build_blocked(const DevGraph& g, uint64_t E, PrPlan* p, hipStream_t s) {
   return CG_OK;
 }
 
+// ---- propagation blocking over (source, bin) pairs (layout 5) ----
+//
+// An edge's contribution is its source's, so the edges of one source into one
+// bin carry the same value.  Writing and reading it once per (source, bin)
+// pair instead of once per edge leaves a pass of 2 bytes per edge (the
+// destination) plus 12 bytes per pair.  On a graph renumbered by descending
+// out-degree the hot destinations share few bins, so the pairs are few:
+// R-MAT edge factor 44 has 0.08 / 0.13 / 0.18 pairs per edge at scale
+// 18 / 20 / 22 (8192-vertex bins), against 1 per edge before.
+//   scatter  k_pb_scatter over the pairs' chunk-major entries (source within
+//            its chunk | bin << 13): one store per pair into its bin-major slot;
+//   gather   k_pbp_gather: a wavefront takes 256 consecutive edges of a bin
+//            (four per lane, one 8-byte load); the pair of each edge is the
+//            group's first pair (pb_gbase) plus the pair starts (0x8000
+//            flags) before it, counted from one ballot per edge slot; its value is
+//            one load from the pair's slot (neighbouring edges share pairs,
+//            so these loads hit the same lines), added in 64-bit fixed point
+//            into the bin's LDS sums.
+// Pairs inside one (chunk, bin) run are ordered by source in both orders
+// (a radix sort on (chunk, bin, source)), so one delta per run maps a pair's
+// chunk-major index to its bin-major one, as for the edges of layout 4.
+constexpr int kPpEdges = 4;                       // edges per lane in a group (one 8-byte load)
+constexpr int kPpGroupShift = 8;                  // edges per group: 64 lanes x kPpEdges = 2^8
+constexpr uint64_t kPpGroup = 1ull << kPpGroupShift;
+// Gather shape, measured at R-MAT s25 / ef44 (ms per iteration): workgroups
+// of 256 / 512 / 1024 threads with 16 / 8 / 2 groups in flight per wavefront
+// 4.85-4.90 / 3.60-3.63 / 2.89-2.91; at 1024 threads 1 / 3 / 4 groups
+// 2.92-2.95 / 3.77-3.85 / 3.81-3.84 (three groups and more need more than the
+// 64 registers that eight wavefronts per SIMD leave); tiles of 2^18 / 2^19 /
+// 2^20 edges 3.05 / 2.89-2.91 / 2.79-2.91.  The LDS sums (64 KB) allow two
+// workgroups per CU, so the workgroup size sets the wavefronts per SIMD.
+constexpr int kPpGroups = 2;                      // groups in flight per wavefront
+constexpr int kPpBlock = 1024;                    // gather workgroup: 16 wavefronts share the LDS sums
+constexpr uint64_t kPpBinTile = 1ull << 20;       // gather: edges per workgroup
+constexpr uint64_t kPpPad = 2 * kPpGroup;         // zero entries past the last edge
+
+// set bits of m below this lane (v_mbcnt)
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__global__ void k_pp_flags(const uint64_t* __restrict__ off, const uint32_t* __restrict__ dst,
+                           const uint32_t* __restrict__ src, uint64_t E, uint32_t* __restrict__ flags) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x)
+    flags[e] = (e == off[src[e]] || (dst[e] >> kPbShift) != (dst[e - 1] >> kPbShift)) ? 1u : 0u;
+}
+
+// incl[e]: pair starts in [0, e], so edge e belongs to pair incl[e] - 1
+// (pairs numbered in source-major order: source, then bin)
+__global__ void k_pp_pairs(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ incl,
+                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
+                           uint32_t nb, uint64_t* __restrict__ pfirst, uint64_t* __restrict__ key,
+                           uint32_t* __restrict__ id, uint32_t* __restrict__ cnt) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
+    if (!flags[e]) continue;
+    const uint32_t j = incl[e] - 1, u = src[e], b = dst[e] >> kPbShift, c = u >> kPbChunkShift;
+    pfirst[j] = e;
+    key[j] = ((uint64_t)c << 25) | ((uint64_t)b << kPbChunkShift) | (u & (kPbChunk - 1));
+    id[j] = j;
+    atomicAdd(&cnt[(uint64_t)c * nb + b], 1u);
+  }
+}
+
+// Chunk-major start of chunk c: the pair of its first edge (pairs of a
+// source are contiguous, sources ascending), or the pair count.
+__global__ void k_pp_bounds(const uint64_t* __restrict__ off, const uint32_t* __restrict__ incl, uint32_t V,
+                            uint64_t E, uint64_t P, uint32_t nc, uint64_t* __restrict__ out) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= nc; c += gridDim.x * blockDim.x) {
+    const uint64_t v = (uint64_t)c << kPbChunkShift;
+    const uint64_t e = off[v < V ? v : V];
+    out[c] = e < E ? (uint64_t)incl[e] - 1 : P;
+  }
+}
+
+__global__ void k_pp_place(const uint64_t* __restrict__ key, const uint32_t* __restrict__ id, uint64_t P,
+                           uint32_t nb, const int64_t* __restrict__ delta, const uint64_t* __restrict__ pfirst,
+                           uint32_t* __restrict__ ent, uint32_t* __restrict__ bm_of, uint32_t* __restrict__ count) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < P; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = key[i];
+    const uint32_t c = (uint32_t)(k >> 25), b = (uint32_t)(k >> kPbChunkShift) & (kPbMaxBins - 1);
+    ent[i] = (uint32_t)(k & (kPbChunk - 1)) | (b << kPbChunkShift);
+    const uint64_t j = (uint64_t)((int64_t)i + delta[(uint64_t)c * nb + b]);
+    const uint32_t sm = id[i];
+    bm_of[sm] = (uint32_t)j;
+    count[j] = (uint32_t)(pfirst[sm + 1] - pfirst[sm]);
+  }
+}
+
+__global__ void k_pp_edges(const uint32_t* __restrict__ dst, const uint32_t* __restrict__ incl, uint64_t E,
+                           const uint64_t* __restrict__ pfirst, const uint32_t* __restrict__ bm_of,
+                           const uint64_t* __restrict__ eoff, uint16_t* __restrict__ dloc,
+                           uint32_t* __restrict__ gbase) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t sm = incl[e] - 1, j = bm_of[sm];
+    const uint64_t k = e - pfirst[sm], pos = eoff[j] + k;
+    dloc[pos] = (uint16_t)((dst[e] & (kPbSize - 1)) | (k == 0 ? 0x8000u : 0u));
+    if ((pos & (kPpGroup - 1)) == 0) gbase[pos >> kPpGroupShift] = j;
+  }
+}
+
+// first edge (bin-major) of every bin: the first edge of its first pair
+__global__ void k_pp_bin_edges(const uint64_t* __restrict__ bin_off, const uint64_t* __restrict__ eoff,
+                               uint32_t nb, uint64_t* __restrict__ out) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += gridDim.x * blockDim.x)
+    out[b] = eoff[bin_off[b]];
+}
+
+__global__ __launch_bounds__(kPpBlock) void k_pbp_gather(const float* __restrict__ val,
+                                                         const uint16_t* __restrict__ dloc,
+                                                         const uint32_t* __restrict__ gbase,
+                                                         const uint32_t* __restrict__ t_bin,
+                                                         const uint64_t* __restrict__ t_lo,
+                                                         const uint64_t* __restrict__ t_hi,
+                                                         const uint32_t* __restrict__ t_multi, uint32_t V,
+                                                         const uint32_t* __restrict__ deg, float base,
+                                                         float damping, unsigned long long* __restrict__ rows,
+                                                         float* __restrict__ rank, float* __restrict__ contrib) {
+  __shared__ unsigned long long acc[kPbSize];
+  const uint32_t t = blockIdx.x;
+  for (uint32_t k = threadIdx.x; k < kPbSize; k += kPpBlock) acc[k] = 0;
+  __syncthreads();
+  const uint64_t lo = t_lo[t], hi = t_hi[t];
+  if (hi > lo) {
+    // a group is 2^8 consecutive edges, four per lane (lane l: 4l .. 4l + 3);
+    // an edge's pair is the group's first pair plus the pair starts (0x8000
+    // flags) after the group's first edge up to it: per edge slot k, a
+    // ballot of the flags; the lanes below count with v_mbcnt
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    constexpr uint32_t kWaves = kPpBlock / 64;
+    const uint64_t* dl4 = reinterpret_cast<const uint64_t*>(dloc);
+    const uint64_t g0 = lo >> kPpGroupShift, g1 = (hi - 1) >> kPpGroupShift;  // inclusive
+    for (uint64_t g = g0 + wave; g <= g1; g += (uint64_t)kPpGroups * kWaves) {
+      uint64_t dd[kPpGroups];
+      uint32_t gb[kPpGroups];
+#pragma unroll
+      for (int u = 0; u < kPpGroups; ++u) {
+        const uint64_t gg = g + (uint64_t)u * kWaves;
+        const bool ok = gg <= g1;
+        dd[u] = ok ? dl4[gg * 64 + lane] : 0ull;
+        gb[u] = ok ? gbase[gg] : 0u;
+      }
+      float x[kPpGroups * kPpEdges];
+#pragma unroll
+      for (int u = 0; u < kPpGroups; ++u) {
+        uint32_t pi = gb[u];
+        uint32_t f[kPpEdges];
+#pragma unroll
+        for (int k = 0; k < kPpEdges; ++k) {
+          f[k] = (uint32_t)(dd[u] >> (16 * k + 15)) & 1u;
+          uint64_t b = __ballot(f[k] != 0);
+          if (k == 0) b &= ~1ull;  // the group's first edge belongs to pair gb
+          pi += lanes_below(b);
+        }
+        if (lane == 0) f[0] = 0;
+        const uint64_t gg = g + (uint64_t)u * kWaves, e0 = (gg << kPpGroupShift) + 4 * lane;
+        const bool whole = gg <= g1 && (gg << kPpGroupShift) >= lo && ((gg + 1) << kPpGroupShift) <= hi;  // uniform
+#pragma unroll
+        for (int k = 0; k < kPpEdges; ++k) {
+          pi += f[k];
+          const bool in = whole || (gg <= g1 && e0 + k >= lo && e0 + k < hi);
+          x[u * kPpEdges + k] = in ? val[pi] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kPpGroups; ++u) {
+        const uint64_t gg = g + (uint64_t)u * kWaves, e0 = (gg << kPpGroupShift) + 4 * lane;
+        const bool whole = gg <= g1 && (gg << kPpGroupShift) >= lo && ((gg + 1) << kPpGroupShift) <= hi;
+#pragma unroll
+        for (int k = 0; k < kPpEdges; ++k) {
+          const bool in = whole || (gg <= g1 && e0 + k >= lo && e0 + k < hi);
+          if (in) atomicAdd(&acc[(uint32_t)(dd[u] >> (16 * k)) & (kPbSize - 1)], to_fixed(x[u * kPpEdges + k]));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const uint64_t v0 = (uint64_t)t_bin[t] << kPbShift;
+  if (t_multi[t]) {
+    unsigned long long* row = rows + (uint64_t)(t_multi[t] - 1) * kPbSize;
+    for (uint32_t k = threadIdx.x; k < kPbSize; k += kPpBlock) row[k] = acc[k];
+  } else {
+    for (uint32_t k = threadIdx.x; k < kPbSize && v0 + k < V; k += kPpBlock)
+      finish((uint32_t)(v0 + k), (float)((double)acc[k] * kFixInv), base, damping, deg, rank, contrib);
+  }
+}
+
+// Bins cut into several gather tiles: the tiles' rows added up per vertex
+// (in row order), then the ranks.
+__global__ __launch_bounds__(kBlock) void k_pbp_finish(const uint32_t* __restrict__ bins,
+                                                       const uint32_t* __restrict__ mrow, uint32_t V,
+                                                       const uint32_t* __restrict__ deg, float base, float damping,
+                                                       const unsigned long long* __restrict__ part,
+                                                       float* __restrict__ rank, float* __restrict__ contrib) {
+  const uint32_t i = blockIdx.x / (kPbSize / kBlock);
+  const uint32_t k = (blockIdx.x % (kPbSize / kBlock)) * kBlock + threadIdx.x;
+  const uint64_t v = ((uint64_t)bins[i] << kPbShift) + k;
+  if (v >= V) return;
+  unsigned long long x = 0;
+  for (uint32_t r = mrow[i]; r < mrow[i + 1]; ++r) x += part[(uint64_t)r * kPbSize + k];
+  finish((uint32_t)v, (float)((double)x * kFixInv), base, damping, deg, rank, contrib);
+}
+
+// Tiles of the two passes: chunk c's pairs [hb[c], hb[c + 1]) in pieces of
+// kPbTile, bin b's edges [he[b], he[b + 1]) in pieces of bin_tile (every
+// bin gets a tile, which writes its ranks even when it has no in-edges).
+int pb_tiles(PrPlan* p, const std::vector<uint64_t>& hb, const std::vector<uint64_t>& he, uint64_t bin_tile,
+             hipStream_t s) {
+  const uint32_t nc = (uint32_t)hb.size() - 1, nb = (uint32_t)he.size() - 1;
+  std::vector<uint32_t> t1c, t2b, t2m, multi, mrow;
+  std::vector<uint64_t> t1l, t1h, t2l, t2h;
+  uint32_t rows = 0;
+  for (uint32_t c = 0; c < nc; ++c)
+    for (uint64_t lo = hb[c]; lo < hb[c + 1]; lo += kPbTile) {
+      t1c.push_back(c);
+      t1l.push_back(lo);
+      t1h.push_back(std::min(hb[c + 1], lo + kPbTile));
+    }
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint64_t n = he[b + 1] - he[b];
+    const uint32_t k = n > bin_tile ? (uint32_t)((n + bin_tile - 1) / bin_tile) : 1u;
+    if (k > 1) {
+      multi.push_back(b);
+      mrow.push_back(rows);
+    }
+    for (uint32_t i = 0; i < k; ++i) {
+      t2b.push_back(b);
+      t2l.push_back(he[b] + (uint64_t)i * bin_tile);
+      t2h.push_back(std::min(he[b + 1], he[b] + (uint64_t)(i + 1) * bin_tile));
+      t2m.push_back(k > 1 ? ++rows : 0u);
+    }
+  }
+  mrow.push_back(rows);
+  p->pb_t1 = (uint32_t)t1c.size();
+  p->pb_t2 = (uint32_t)t2b.size();
+  p->pb_multi = (uint32_t)multi.size();
+  CG_TRY(p->pb_t1_chunk.ensure(t1c.size() + 1));
+  CG_TRY(p->pb_t1_lo.ensure(t1c.size() + 1));
+  CG_TRY(p->pb_t1_hi.ensure(t1c.size() + 1));
+  CG_TRY(p->pb_t2_bin.ensure(t2b.size()));
+  CG_TRY(p->pb_t2_lo.ensure(t2b.size()));
+  CG_TRY(p->pb_t2_hi.ensure(t2b.size()));
+  CG_TRY(p->pb_t2_multi.ensure(t2b.size()));
+  CG_TRY(p->pb_multi_bins.ensure(multi.size() + 1));
+  if (!t1c.empty()) {
+    CG_HIP(hipMemcpyAsync(p->pb_t1_chunk.p, t1c.data(), t1c.size() * 4, hipMemcpyHostToDevice, s));
+    CG_HIP(hipMemcpyAsync(p->pb_t1_lo.p, t1l.data(), t1l.size() * 8, hipMemcpyHostToDevice, s));
+    CG_HIP(hipMemcpyAsync(p->pb_t1_hi.p, t1h.data(), t1h.size() * 8, hipMemcpyHostToDevice, s));
+  }
+  CG_HIP(hipMemcpyAsync(p->pb_t2_bin.p, t2b.data(), t2b.size() * 4, hipMemcpyHostToDevice, s));
+  CG_HIP(hipMemcpyAsync(p->pb_t2_lo.p, t2l.data(), t2l.size() * 8, hipMemcpyHostToDevice, s));
+  CG_HIP(hipMemcpyAsync(p->pb_t2_hi.p, t2h.data(), t2h.size() * 8, hipMemcpyHostToDevice, s));
+  CG_HIP(hipMemcpyAsync(p->pb_t2_multi.p, t2m.data(), t2m.size() * 4, hipMemcpyHostToDevice, s));
+  if (!multi.empty())
+    CG_HIP(hipMemcpyAsync(p->pb_multi_bins.p, multi.data(), multi.size() * 4, hipMemcpyHostToDevice, s));
+  CG_TRY(p->pb_mrow.ensure(mrow.size()));
+  CG_HIP(hipMemcpyAsync(p->pb_mrow.p, mrow.data(), mrow.size() * 4, hipMemcpyHostToDevice, s));
+  CG_TRY(p->pb_part.ensure(((uint64_t)rows + 1) << kPbShift));
+  CG_HIP(hipStreamSynchronize(s));  // the host vectors must outlive the copies
+  return CG_OK;
+}
+
+int build_pairs(cg_graph* graph, PrPlan* p, hipStream_t s) {
+  const uint32_t V = graph->V;
+  const uint64_t E = graph->E;
+  const uint32_t nb = (uint32_t)(((uint64_t)V + kPbSize - 1) >> kPbShift);
+  const uint32_t nc = (uint32_t)(((uint64_t)V + kPbChunk - 1) >> kPbChunkShift);
+  if (nb > kPbMaxBins) return set_error(CG_ERR_INVALID, "blocked PageRank holds at most 2^25 vertices");
+  if (E == 0 || E >= 0xFFFFFFFFull) return set_error(CG_ERR_INVALID, "pair blocking needs 0 < E < 2^32");
+  p->layout = kLayoutPairs;
+  p->pb_nb = nb;
+  // the graph renumbered by descending out-degree, rows sorted by destination
+  cg_graph rg;
+  {
+    DevBuf<uint32_t> new_of;
+    CG_TRY(degree_order(graph, p->perm, new_of, nullptr));
+    CG_TRY(relabel_graph(graph, new_of.p, &rg));
+  }
+  const DevGraph g{rg.d_off.p, rg.d_dst.p, nullptr, nullptr, nullptr, V};
+  CG_TRY(build_degrees(g, p, s));
+  DevBuf<uint32_t> src, flags, incl;
+  CG_TRY(edge_sources(g.off, V, E, src, s));
+  CG_TRY(flags.ensure(E));
+  CG_TRY(incl.ensure(E));
+  hipLaunchKernelGGL(k_pp_flags, dim3(grid_of(E)), dim3(kBlock), 0, s, g.off, g.dst, src.p, E, flags.p);
+  PR_LAUNCH_OK();
+  {
+    size_t tb = 0;
+    CG_HIP(rocprim::inclusive_scan(nullptr, tb, flags.p, incl.p, E, rocprim::plus<uint32_t>(), s));
+    DevBuf<uint8_t> tmp;
+    CG_TRY(tmp.ensure(tb));
+    CG_HIP(rocprim::inclusive_scan(tmp.p, tb, flags.p, incl.p, E, rocprim::plus<uint32_t>(), s));
+  }
+  uint32_t P32 = 0;
+  CG_HIP(hipMemcpyAsync(&P32, incl.p + E - 1, 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  const uint64_t P = P32;
+  p->pb_pairs = P;
+  // pairs: entry 4 B read, contribution 4 B written and 4 B read; edges:
+  // 2 B read (+ one 4-byte group pair per 256); per vertex as layout 4
+  p->stream_bytes = 12.0 * (double)P + (2.0 + 4.0 / (double)kPpGroup) * (double)E + 16.0 * (double)V;
+  p->pb_narrow = P < (1ull << 31);
+  // the pairs in source-major order, and the pairs of every (chunk, bin) run
+  const uint64_t cells = (uint64_t)nc * nb;
+  DevBuf<uint64_t> pfirst, key, key2;
+  DevBuf<uint32_t> id, id2, cnt;
+  CG_TRY(pfirst.ensure(P + 1));
+  CG_TRY(key.ensure(P));
+  CG_TRY(key2.ensure(P));
+  CG_TRY(id.ensure(P));
+  CG_TRY(id2.ensure(P));
+  CG_TRY(cnt.ensure(cells));
+  CG_HIP(hipMemsetAsync(cnt.p, 0, cells * 4, s));
+  CG_HIP(hipMemcpyAsync(pfirst.p + P, &E, 8, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_pp_pairs, dim3(grid_of(E)), dim3(kBlock), 0, s, flags.p, incl.p, src.p, g.dst, E, nb,
+                     pfirst.p, key.p, id.p, cnt.p);
+  PR_LAUNCH_OK();
+  flags.reset();
+  // chunk-major order: (chunk, bin, source)
+  {
+    rocprim::double_buffer<uint64_t> kb(key.p, key2.p);
+    rocprim::double_buffer<uint32_t> vb(id.p, id2.p);
+    size_t tb = 0;
+    CG_HIP(rocprim::radix_sort_pairs(nullptr, tb, kb, vb, P, 0, 37, s));
+    DevBuf<uint8_t> tmp;
+    CG_TRY(tmp.ensure(tb));
+    CG_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kb, vb, P, 0, 37, s));
+    if (kb.current() != key.p) std::swap(key, key2);
+    if (vb.current() != id.p) std::swap(id, id2);
+  }
+  key2.reset();
+  id2.reset();
+  // run starts in both orders and their differences (the kernels of layout 4)
+  DevBuf<uint64_t> bounds, in_base, col_pre, bin_size, bin_off;
+  CG_TRY(bounds.ensure((uint64_t)nc + 1));
+  CG_TRY(in_base.ensure(cells));
+  CG_TRY(col_pre.ensure(cells));
+  CG_TRY(bin_size.ensure(nb));
+  CG_TRY(bin_off.ensure((uint64_t)nb + 1));
+  CG_TRY(p->pb_delta.ensure(cells));
+  hipLaunchKernelGGL(k_pp_bounds, dim3(grid_of((uint64_t)nc + 1)), dim3(kBlock), 0, s, g.off, incl.p, V, E, P, nc,
+                     bounds.p);
+  hipLaunchKernelGGL(k_pb_rows, dim3(nc), dim3(kBlock), 0, s, cnt.p, bounds.p, nb, in_base.p);
+  hipLaunchKernelGGL(k_pb_cols, dim3(nb), dim3(kBlock), 0, s, cnt.p, nc, nb, col_pre.p, bin_size.p);
+  hipLaunchKernelGGL(k_pb_bins, dim3(1), dim3(kBlock), 0, s, bin_size.p, nb, bin_off.p);
+  hipLaunchKernelGGL(k_pb_delta, dim3(grid_of(cells)), dim3(kBlock), 0, s, in_base.p, col_pre.p, bin_off.p, nb,
+                     cells, p->pb_delta.p);
+  PR_LAUNCH_OK();
+  cnt.reset();
+  col_pre.reset();
+  in_base.reset();
+  // entries, bin-major slots and counts of the pairs
+  DevBuf<uint32_t> bm_of, count;
+  CG_TRY(p->pb_ent.ensure(P));
+  CG_TRY(p->pb_val.ensure(P));
+  CG_TRY(bm_of.ensure(P));
+  CG_TRY(count.ensure(P));
+  hipLaunchKernelGGL(k_pp_place, dim3(grid_of(P)), dim3(kBlock), 0, s, key.p, id.p, P, nb, p->pb_delta.p, pfirst.p,
+                     p->pb_ent.p, bm_of.p, count.p);
+  PR_LAUNCH_OK();
+  key.reset();
+  id.reset();
+  DevBuf<uint64_t> eoff;
+  CG_TRY(eoff.ensure(P + 1));
+  {
+    size_t tb = 0;
+    CG_HIP(rocprim::exclusive_scan(nullptr, tb, count.p, eoff.p, (uint64_t)0, P, rocprim::plus<uint64_t>(), s));
+    DevBuf<uint8_t> tmp;
+    CG_TRY(tmp.ensure(tb));
+    CG_HIP(rocprim::exclusive_scan(tmp.p, tb, count.p, eoff.p, (uint64_t)0, P, rocprim::plus<uint64_t>(), s));
+  }
+  CG_HIP(hipMemcpyAsync(eoff.p + P, &E, 8, hipMemcpyHostToDevice, s));
+  count.reset();
+  // the edges: destination within its bin, flagged at each pair's first edge;
+  // zero entries past the end keep a tile's last group in bounds
+  CG_TRY(p->pb_dloc.ensure(E + kPpPad));
+  CG_TRY(p->pb_gbase.ensure((E + kPpGroup - 1) / kPpGroup + 2));
+  CG_HIP(hipMemsetAsync(p->pb_dloc.p + E, 0, kPpPad * sizeof(uint16_t), s));
+  hipLaunchKernelGGL(k_pp_edges, dim3(grid_of(E)), dim3(kBlock), 0, s, g.dst, incl.p, E, pfirst.p, bm_of.p, eoff.p,
+                     p->pb_dloc.p, p->pb_gbase.p);
+  PR_LAUNCH_OK();
+  DevBuf<uint64_t> bin_edges;
+  CG_TRY(bin_edges.ensure((uint64_t)nb + 1));
+  hipLaunchKernelGGL(k_pp_bin_edges, dim3(grid_of((uint64_t)nb + 1)), dim3(kBlock), 0, s, bin_off.p, eoff.p, nb,
+                     bin_edges.p);
+  PR_LAUNCH_OK();
+  std::vector<uint64_t> hb(nc + 1), he(nb + 1);
+  CG_HIP(hipMemcpyAsync(hb.data(), bounds.p, hb.size() * 8, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipMemcpyAsync(he.data(), bin_edges.p, he.size() * 8, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  return pb_tiles(p, hb, he, kPpBinTile, s);
+}
+
 template <int P>
 void launch_tiers(const Csc& g, const PrPlan::Tiers& t, unsigned vblocks, const float* contrib_in, float base,
                   float damping, float* part, float* rank, float* contrib_out, hipStream_t s) {
@@ -1221,15 +1635,29 @@ void launch_tiers(const Csc& g, const PrPlan::Tiers& t, unsigned vblocks, const 
 int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, uint32_t kernel, PrPlan** plan,
                    hipStream_t s) {
   if (kernel > 3) return set_error(CG_ERR_INVALID, "pr_kernel %u outside [0, 3]", kernel);
-  if (layout > kLayoutBlocked) return set_error(CG_ERR_INVALID, "pr_layout %u outside [0, 4]", layout);
+  if (layout > kLayoutPairs) return set_error(CG_ERR_INVALID, "pr_layout %u outside [0, 5]", layout);
   // automatic: propagation blocking from 2^22 vertices up to its 2^25 limit
   // (measured per iteration on R-MAT edge factor 16 against the renumbered
   // pull: 0.30-0.35 vs 0.39-0.43 ms at 2^22, 0.54 vs 0.93 at 2^23; at 2^21
   // the two are within box-to-box spread, 0.17-0.20 vs 0.17 ms, and below it
   // the pull wins: 0.10 vs 0.12-0.16 ms at 2^20)
+  // and over (source, bin) pairs from 2^24 (per iteration, R-MAT: ef16 0.82 vs
+  // 1.09 ms at 2^24, 1.26 vs 2.12 at 2^25, ef44 2.92 vs 4.61 at 2^25; at 2^23
+  // 0.66 vs 0.57 ms and below it the 2^13-vertex bins are too few to fill the
+  // chip)
   if (layout == kLayoutAuto && seg_log2 == 0 && kernel == 0 && graph->V >= (1u << 22) &&
       ((uint64_t)graph->V + kPbSize - 1) / kPbSize <= kPbMaxBins && graph->E > 0)
-    layout = kLayoutBlocked;
+    layout = graph->V >= (1u << 24) && graph->E < 0xFFFFFFFFull ? kLayoutPairs : kLayoutBlocked;
+  if (layout == kLayoutPairs) {
+    PrPlan* p = new PrPlan();
+    const int st = build_pairs(graph, p, s);
+    if (st != CG_OK) {
+      delete p;
+      return st;
+    }
+    *plan = p;
+    return CG_OK;
+  }
   if (layout == kLayoutBlocked) {
     // out-edges binned by destination; no transposed graph needed
     const DevGraph g{graph->d_off.p, graph->d_dst.p, nullptr, nullptr, nullptr, graph->V};
@@ -1293,6 +1721,11 @@ int pr_plan_create(cg_graph* graph, uint32_t layout, uint32_t seg_log2, uint32_t
 
 void pr_plan_destroy(PrPlan* plan) { delete plan; }
 
+void pr_plan_info(const PrPlan* plan, uint32_t* layout, double* stream_bytes) {
+  *layout = (uint32_t)plan->layout;
+  *stream_bytes = plan->stream_bytes;
+}
+
 int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* rank, float* tmp_rank,
            float* contrib, float* contrib2, hipStream_t s, int profile, double* kernel_ms,
            uint32_t* launches) {
@@ -1300,7 +1733,7 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
   if (!V) return CG_OK;
   const float base = (1.0f - damping) / (float)V;
   const unsigned vblocks = (unsigned)(((uint64_t)V + kBlock - 1) / kBlock);
-  const bool relabel = plan->layout == kLayoutRelabel;
+  const bool relabel = plan->layout == kLayoutRelabel || plan->layout == kLayoutPairs;
   const Csc csc = relabel ? Csc{plan->r_off.p, plan->r_src.p, plan->deg.p, V, plan->hot.p}
                           : Csc{g.in_off, g.in_src, plan->deg.p, V, nullptr};
   const uint64_t plan_edges = plan->edges;
@@ -1318,7 +1751,7 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
   double total = 0.0;
   for (int it = 0; it < iters; ++it) {
     if (profile) CG_HIP(hipEventRecord(a, s));
-    if (plan->layout == kLayoutBlocked) {
+    if (plan->layout == kLayoutBlocked || plan->layout == kLayoutPairs) {
       if (plan->pb_t1) {
         if (plan->pb_narrow)
           hipLaunchKernelGGL(k_pb_scatter<int32_t>, dim3(plan->pb_t1), dim3(kBlock),
@@ -1331,10 +1764,19 @@ int pr_run(const DevGraph& g, PrPlan* plan, int iters, float damping, float* ran
                              plan->pb_t1_chunk.p, plan->pb_t1_lo.p, plan->pb_t1_hi.p, plan->pb_nb, V,
                              plan->pb_delta.p, c_cur, plan->pb_val.p);
       }
-      hipLaunchKernelGGL(k_pb_gather, dim3(plan->pb_t2), dim3(kBlock), 0, s, plan->pb_val.p, plan->pb_dloc.p,
-                         plan->pb_t2_bin.p, plan->pb_t2_lo.p, plan->pb_t2_hi.p, plan->pb_t2_multi.p, V, plan->deg.p,
-                         base, damping, plan->pb_sum.p, r_nxt, c_nxt);
-      if (plan->pb_multi)
+      if (plan->layout == kLayoutPairs)
+        hipLaunchKernelGGL(k_pbp_gather, dim3(plan->pb_t2), dim3(kPpBlock), 0, s, plan->pb_val.p, plan->pb_dloc.p,
+                           plan->pb_gbase.p, plan->pb_t2_bin.p, plan->pb_t2_lo.p, plan->pb_t2_hi.p,
+                           plan->pb_t2_multi.p, V, plan->deg.p, base, damping, plan->pb_part.p, r_nxt, c_nxt);
+      else
+        hipLaunchKernelGGL(k_pb_gather, dim3(plan->pb_t2), dim3(kBlock), 0, s, plan->pb_val.p, plan->pb_dloc.p,
+                           plan->pb_t2_bin.p, plan->pb_t2_lo.p, plan->pb_t2_hi.p, plan->pb_t2_multi.p, V,
+                           plan->deg.p, base, damping, plan->pb_sum.p, r_nxt, c_nxt);
+      if (plan->pb_multi && plan->layout == kLayoutPairs)
+        hipLaunchKernelGGL(k_pbp_finish, dim3(plan->pb_multi * (kPbSize / kBlock)), dim3(kBlock), 0, s,
+                           plan->pb_multi_bins.p, plan->pb_mrow.p, V, plan->deg.p, base, damping, plan->pb_part.p,
+                           r_nxt, c_nxt);
+      else if (plan->pb_multi)
         hipLaunchKernelGGL(k_pb_finish, dim3(plan->pb_multi * (kPbSize / kBlock)), dim3(kBlock), 0, s,
                            plan->pb_multi_bins.p, V, plan->deg.p, base, damping, plan->pb_sum.p, r_nxt, c_nxt);
     } else if (plan->layout == kLayoutSegmented) {

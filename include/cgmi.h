@@ -124,10 +124,13 @@ typedef struct cg_options {
                                      out-degree (gather locality on skewed graphs), 3 source
                                      windows of 2^pr_segment_log2 vertices, 4 propagation blocking
                                      (out-edges binned by 8192-vertex destination blocks, sums in
-                                     64-bit fixed point; at most 2^25 vertices).  Automatic: a
-                                     window size in [1, 30] selects 3, 31 selects 1; 0 selects 4
-                                     from 2^22 to 2^25 vertices (with pr_kernel 0), else 2 above
-                                     65 536 vertices */
+                                     64-bit fixed point; at most 2^25 vertices), 5 propagation
+                                     blocking over (source, block) pairs of a copy renumbered by
+                                     descending out-degree (one contribution per pair, 2 bytes
+                                     per edge; at most 2^25 vertices, fewer than 2^32 edges).
+                                     Automatic: a window size in [1, 30] selects 3, 31 selects 1;
+                                     0 selects 5 from 2^24 to 2^25 vertices and 4 from 2^22 (with
+                                     pr_kernel 0), else 2 above 65 536 vertices */
   uint32_t relabel;               /* BFS / SSSP: run on an engine-owned copy renumbered by
                                      descending out-degree (results come back in the caller's
                                      ids): 0 automatic (at least 2^21 vertices for SSSP, 2^23
@@ -195,6 +198,9 @@ typedef struct cg_run_stats {
   double expand_alg_bytes;        /* algorithmic bytes those launches moved */
   uint32_t learned_fallbacks;     /* iterations where the learned scheduler deferred */
   uint64_t resident_edges;        /* edges held in HBM (E unless the graph ran partly resident) */
+  uint32_t pr_layout;             /* PageRank: the layout the engine chose (pr_layout values) */
+  double pr_stream_bytes;         /* PageRank, layouts 4 and 5: bytes streamed per iteration by
+                                     design (0 for the pull layouts, whose reads are gathers) */
 } cg_run_stats;
 
 typedef struct cg_partition_info {

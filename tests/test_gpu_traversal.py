@@ -209,10 +209,11 @@ def test_pagerank_l1_and_determinism(oracle):
 
 @pytest.mark.parametrize("layout,seg,kernel", [(0, 31, 0), (0, 8, 0), (0, 11, 0), (0, 14, 0), (0, 0, 0), (1, 0, 1),
                                                (1, 0, 2), (2, 0, 1), (2, 0, 2), (2, 0, 3), (2, 12, 3), (2, 20, 3),
-                                               (3, 9, 0), (4, 0, 0)])
+                                               (3, 9, 0), (4, 0, 0), (5, 0, 0)])
 def test_pagerank_layouts(oracle, layout, seg, kernel):
     """Every in-edge layout (plain, renumbered by out-degree, source windows
-    smaller and larger than the graph, propagation blocking) and every sweep
+    smaller and larger than the graph, propagation blocking per edge and per
+    (source, bin) pair) and every sweep
     (row tiers, edge tiles, two source bands) against the FP64 oracle;
     run-to-run bit-identical."""
     og = oracle.rmat_graph(17, 16, seed=2, weights=None)
@@ -268,11 +269,43 @@ def test_pagerank_blocked_builds(oracle):
     assert float(np.abs(runs[0].astype(np.float64) - ref).sum()) <= 1e-6
 
 
+def test_pagerank_pair_blocking(oracle):
+    """Propagation blocking over (source, bin) pairs on the graph renumbered
+    by out-degree: ranks in the caller's ids within 1e-6 (L1) of the FP64
+    oracle, bit-identical across engines, on a graph whose hottest sources
+    also receive the most edges, so that after renumbering their bin holds
+    more than 2^20 edges (a bin cut into several gather tiles) and many pairs
+    carry several edges; a graph whose size is not a multiple of 8192."""
+    rng = np.random.default_rng(7)
+    hot, m = 2600000, 600000
+    V = 30001
+    src = np.concatenate([rng.integers(0, 8192, hot), rng.integers(0, V, m)])
+    dst = np.concatenate([rng.integers(0, 8192, hot), rng.integers(0, V, m)])
+    og = oracle.csr_build(V, src.astype(np.uint32), dst.astype(np.uint32))
+    assert int(np.count_nonzero(og.dest < 8192)) > 2 * 2**20
+    ref = oracle.pagerank64(og, 20, 0.85)
+    g = cg.Graph.from_csr(og.offsets, og.dest)
+    runs = []
+    for _ in range(2):
+        e = cg.Engine(g, PR, pr_layout=5)
+        runs.append(e.run()[0])
+        runs.append(e.run()[0])
+        e.close()
+    for r in runs[1:]:
+        assert np.array_equal(runs[0].view(np.uint32), r.view(np.uint32))
+    assert float(np.abs(runs[0].astype(np.float64) - ref).sum()) <= 1e-6
+    # the same ranks as the per-edge blocking, to FP32 rounding of the ranks
+    e = cg.Engine(g, PR, pr_layout=4)
+    r4 = e.run()[0]
+    e.close()
+    assert float(np.abs(r4.astype(np.float64) - runs[0].astype(np.float64)).sum()) <= 1e-7
+
+
 def test_pagerank_bad_layout(oracle):
     og = oracle.rmat_graph(8, 4, seed=1, weights=None)
     g = cg.Graph.from_csr(og.offsets, og.dest)
     with pytest.raises(cg.CgError):
-        cg.Engine(g, PR, pr_layout=5)
+        cg.Engine(g, PR, pr_layout=6)
     with pytest.raises(cg.CgError):
         cg.Engine(g, PR, pr_kernel=4)
 
