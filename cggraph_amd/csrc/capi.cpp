@@ -647,6 +647,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
   } else {
     // the frontier compaction reads out-degrees rather than offset pairs
     if (!e->partial && !e->host_only) CG_TRY(ensure_degrees(g, e->stream));
+    if (algorithm == CG_SSSP && !e->partial && !e->host_only) CG_TRY(ensure_narrow_weights(g, e->stream));
     CG_TRY(e->val.ensure(V));
     CG_TRY(e->bm[0].ensure(words));
     CG_TRY(e->bm[1].ensure(words));

@@ -141,6 +141,10 @@ struct cg_graph {
   // out-degrees u32, padded to whole 64-vertex words, built on demand
   bool dev_deg_valid = false;
   cg::DevBuf<uint32_t> d_deg;
+  // the weights in 8 or 16 bits when all of them fit (at most one is built)
+  bool dev_wn_valid = false;
+  cg::DevBuf<uint8_t> d_w8;
+  cg::DevBuf<uint16_t> d_w16;
   // transposed graph (in-edges sorted by source), built on demand
   bool dev_t_valid = false;
   cg::DevBuf<uint64_t> d_in_off;

@@ -499,7 +499,9 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
 
   DevGraph dg{g.d_off.p, g.d_dst.p, g.has_w ? g.d_w.p : nullptr,
               g.dev_t_valid ? g.d_in_off.p : nullptr, g.dev_t_valid ? g.d_in_src.p : nullptr, V,
-              g.dev_t_valid && g.has_w ? g.d_in_w.p : nullptr, g.dev_deg_valid && !partial ? g.d_deg.p : nullptr};
+              g.dev_t_valid && g.has_w ? g.d_in_w.p : nullptr, g.dev_deg_valid && !partial ? g.d_deg.p : nullptr,
+              g.dev_wn_valid && !partial && g.d_w8.p ? g.d_w8.p : nullptr,
+              g.dev_wn_valid && !partial && g.d_w16.p ? g.d_w16.p : nullptr};
 
   cg_run_stats local{};
   const auto t0 = clk::now();

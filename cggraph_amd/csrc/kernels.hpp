@@ -88,6 +88,10 @@ struct DevGraph {
   // out-degrees, zero-padded to whole 64-vertex words (ensure_degrees); the
   // frontier compaction reads them instead of the offsets when present
   const uint32_t* deg = nullptr;
+  // narrow copies of w (ensure_narrow_weights): at most one is set, when
+  // every weight fits it; the push expansion of SSSP reads it instead of w
+  const uint8_t* w8 = nullptr;
+  const uint16_t* w16 = nullptr;
 };
 
 struct Bins {
@@ -287,6 +291,9 @@ int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, u
 constexpr int kAccSlots = kStripes * kStripeStride / 2;
 // Builds the graph's device out-degree array once (DevGraph::deg).
 int ensure_degrees(cg_graph* g, hipStream_t s);
+// Builds, once, a copy of the weights in 8 or 16 bits when every weight fits
+// (DevGraph::w8 / w16); the same values, fewer bytes per relaxed edge.
+int ensure_narrow_weights(cg_graph* g, hipStream_t s);
 int launch_finish_query(const DevGraph& g, const uint32_t* val, unsigned long long* acc, const uint32_t* new_of,
                         uint32_t* out, hipStream_t s);
 

@@ -115,8 +115,8 @@ enum { ALG_BFS = CG_BFS, ALG_SSSP = CG_SSSP, ALG_CC = CG_CC };
 // the strict-less pre-check: a pre-check that passes means the value is
 // lowered in this iteration by this or another update (or the read was
 // stale), so the active set is a superset and the fixed point is the same.
-template <int ALG>
-__device__ __forceinline__ uint32_t candidate(uint32_t sv, uint64_t e, const uint32_t* __restrict__ w) {
+template <int ALG, class WT>
+__device__ __forceinline__ uint32_t candidate(uint32_t sv, uint64_t e, const WT* __restrict__ w) {
   if constexpr (ALG == ALG_BFS) return sv + 1;
   else if constexpr (ALG == ALG_SSSP) return sv + w[e];
   else return sv;
@@ -182,9 +182,9 @@ __device__ __forceinline__ uint32_t commit(uint32_t cand, uint32_t seen, uint32_
   return 0u;
 }
 
-template <int ALG, bool EXACT, bool TRACK = false, class A = MinTrack>
+template <int ALG, bool EXACT, bool TRACK = false, class A = MinTrack, class WT = uint32_t>
 __device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_t* __restrict__ dst,
-                                          const uint32_t* __restrict__ w, uint32_t* __restrict__ val,
+                                          const WT* __restrict__ w, uint32_t* __restrict__ val,
                                           uint64_t* __restrict__ out, A& acc) {
   const uint32_t d = dst[e];
   return commit<ALG, EXACT, TRACK>(candidate<ALG>(sv, e, w), val[d], d, val, out, acc);
@@ -193,9 +193,9 @@ __device__ __forceinline__ uint32_t relax(uint32_t sv, uint64_t e, const uint32_
 // Edges [s, e) of one source with value sv, lane-strided over a wavefront.
 // Four edges per lane are loaded (destination, weight, destination value)
 // before any of them is committed, so four gathers are in flight per lane.
-template <int ALG, bool EXACT, bool TRACK, class A>
+template <int ALG, bool EXACT, bool TRACK, class A, class WT>
 __device__ __forceinline__ uint32_t relax_strided(uint32_t sv, uint64_t s, uint64_t e, uint32_t lane,
-                                                  const uint32_t* __restrict__ dst, const uint32_t* __restrict__ w,
+                                                  const uint32_t* __restrict__ dst, const WT* __restrict__ w,
                                                   uint32_t* __restrict__ val, uint64_t* __restrict__ out, A& acc) {
   uint32_t n = 0;
   uint64_t k = s + lane;
@@ -1188,7 +1188,17 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
 // tier these items cover (the rest is the host team's, or another launch's)
 // and whether the chunked and medium tiers come with it.  Returns the
 // wavefront-lane's first-visit count; activations are folded into acc.
-template <int ALG, bool EXACT, bool TRACK, class A>
+// The weight array of the width the expansion was instantiated for: the
+// graph's own (uint32), or its narrow copy when every weight fits 8 or 16
+// bits (the same values; fewer bytes streamed per edge).
+template <class WT>
+__device__ __forceinline__ const WT* graph_weights(const DevGraph& g) {
+  if constexpr (sizeof(WT) == 1) return g.w8;
+  else if constexpr (sizeof(WT) == 2) return g.w16;
+  else return g.w;
+}
+
+template <int ALG, bool EXACT, bool TRACK, class A, class WT = uint32_t>
 __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __restrict__ val,
                                                  uint64_t* __restrict__ out, const uint32_t* __restrict__ q_small,
                                                  const uint32_t* __restrict__ q_med,
@@ -1198,6 +1208,7 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
   // capped near one chip's worth of resident workgroups stride over the items,
   // so no wavefront is launched only to find nothing to do.
   const uint32_t n_chunk = range.big ? st->n_chunk : 0u, n_med = range.big ? st->n_med : 0u;
+  const WT* const w = graph_weights<WT>(g);
   const uint32_t s_end = st->n_small < range.end ? st->n_small : range.end;
   const uint32_t s_beg = range.begin < s_end ? range.begin : s_end;
   const uint32_t n_small = s_end - s_beg;
@@ -1213,10 +1224,10 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
       const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
       const uint64_t end = g.off[v + 1];
       const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
-      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], s, e, lane, g.dst, g.w, val, out, acc);
+      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], s, e, lane, g.dst, w, val, out, acc);
     } else if (item < (uint64_t)n_chunk + n_med) {
       const uint32_t v = q_med[item - n_chunk];
-      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], g.off[v], g.off[v + 1], lane, g.dst, g.w, val, out, acc);
+      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], g.off[v], g.off[v + 1], lane, g.dst, w, val, out, acc);
     } else {
       // 64 low-degree vertices per wavefront, their edges spread evenly over
       // the lanes: lane j of round r takes edge r*64+j of the concatenated
@@ -1245,14 +1256,14 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
         const uint64_t os = __shfl(s, (int)lo, kWave);
         const uint32_t osv = __shfl(sv, (int)lo, kWave);
         const uint32_t oex = __shfl(excl, (int)lo, kWave);
-        if (j < total) n_new += relax<ALG, EXACT, TRACK>(osv, os + (j - oex), g.dst, g.w, val, out, acc);
+        if (j < total) n_new += relax<ALG, EXACT, TRACK>(osv, os + (j - oex), g.dst, w, val, out, acc);
       }
     }
   }
   return n_new;
 }
 
-template <int ALG, bool EXACT, bool TRACK>
+template <int ALG, bool EXACT, bool TRACK, class WT = uint32_t>
 __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restrict__ val,
                                                    uint64_t* __restrict__ out,
                                                    const uint32_t* __restrict__ q_small,
@@ -1263,7 +1274,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(DevGraph g, uint32_t* __restr
                                                    uint32_t* __restrict__ stripes) {
   MinTrack mt;  // smallest activating candidate (TRACK)
   const uint32_t n_new =
-      expand_items<ALG, EXACT, TRACK>(g, val, out, q_small, q_med, q_chunk, range, chunk_edges, st, mt);
+      expand_items<ALG, EXACT, TRACK, MinTrack, WT>(g, val, out, q_small, q_med, q_chunk, range, chunk_edges, st, mt);
   if constexpr (ALG != ALG_CC) {
     const uint32_t tot = wave_sum_u32(n_new);
     if (lane_id() == 0) stripe_add(stripes, tot);
@@ -1797,27 +1808,37 @@ int launch_expand_push(int alg, const DevGraph& g, uint32_t* val, uint64_t* out,
 #define CG_EXPAND(A, X)                                                                          \
   hipLaunchKernelGGL((k_expand<A, X, false>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk, range, \
                      bins.chunk_edges, st, stripes)
+  // SSSP reads the narrowest weight copy the graph has
+#define CG_EXPAND_SSSP(X, T)                                                                              \
+  do {                                                                                                    \
+    if (g.w8)                                                                                             \
+      hipLaunchKernelGGL((k_expand<ALG_SSSP, X, T, uint8_t>), grid, block, 0, s, g, val, out, q_small, q_med, \
+                         q_chunk, range, bins.chunk_edges, st, stripes);                                  \
+    else if (g.w16)                                                                                       \
+      hipLaunchKernelGGL((k_expand<ALG_SSSP, X, T, uint16_t>), grid, block, 0, s, g, val, out, q_small,      \
+                         q_med, q_chunk, range, bins.chunk_edges, st, stripes);                           \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_expand<ALG_SSSP, X, T, uint32_t>), grid, block, 0, s, g, val, out, q_small,      \
+                         q_med, q_chunk, range, bins.chunk_edges, st, stripes);                           \
+  } while (0)
   if (track_min) {
-    if (exact)
-      hipLaunchKernelGGL((k_expand<ALG_SSSP, true, true>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         range, bins.chunk_edges, st, stripes);
-    else
-      hipLaunchKernelGGL((k_expand<ALG_SSSP, false, true>), grid, block, 0, s, g, val, out, q_small, q_med, q_chunk,
-                         range, bins.chunk_edges, st, stripes);
+    if (exact) CG_EXPAND_SSSP(true, true);
+    else CG_EXPAND_SSSP(false, true);
     CG_LAUNCH_CHECK();
     return CG_OK;
   }
   switch (alg * 2 + (exact ? 1 : 0)) {
     case CG_BFS * 2: CG_EXPAND(ALG_BFS, false); break;
     case CG_BFS * 2 + 1: CG_EXPAND(ALG_BFS, true); break;
-    case CG_SSSP * 2: CG_EXPAND(ALG_SSSP, false); break;
-    case CG_SSSP * 2 + 1: CG_EXPAND(ALG_SSSP, true); break;
+    case CG_SSSP * 2: CG_EXPAND_SSSP(false, false); break;
+    case CG_SSSP * 2 + 1: CG_EXPAND_SSSP(true, false); break;
     case CG_CC * 2: CG_EXPAND(ALG_CC, false); break;
     case CG_CC * 2 + 1: CG_EXPAND(ALG_CC, true); break;
     default:
       return set_error(CG_ERR_UNSUPPORTED, "push expansion: algorithm %d", alg);
   }
 #undef CG_EXPAND
+#undef CG_EXPAND_SSSP
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
@@ -1944,6 +1965,49 @@ int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, u
 __global__ void k_degrees(const uint64_t* __restrict__ off, uint32_t V, uint64_t n, uint32_t* __restrict__ deg) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     deg[i] = i < V ? (uint32_t)(off[i + 1] - off[i]) : 0u;
+}
+
+__global__ void k_max_u32(const uint32_t* __restrict__ x, uint64_t n, uint32_t* __restrict__ mx) {
+  uint32_t m = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    m = x[i] > m ? x[i] : m;
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) {
+    const uint32_t o = __shfl_xor(m, d, kWave);
+    m = o > m ? o : m;
+  }
+  if (lane_id() == 0 && m) atomicMax(mx, m);
+}
+
+template <class T>
+__global__ void k_narrow(const uint32_t* __restrict__ w, uint64_t n, T* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (T)w[i];
+}
+
+int ensure_narrow_weights(cg_graph* g, hipStream_t s) {
+  if (g->dev_wn_valid || !g->has_w || !g->E) return CG_OK;
+  CG_TRY(g->ensure_device());
+  DevBuf<uint32_t> mx;
+  CG_TRY(mx.ensure(1));
+  CG_HIP(hipMemsetAsync(mx.p, 0, 4, s));
+  const unsigned grid = grid_for(g->E, kBlock, 8192);
+  hipLaunchKernelGGL(k_max_u32, dim3(grid), dim3(kBlock), 0, s, g->d_w.p, g->E, mx.p);
+  CG_LAUNCH_CHECK();
+  uint32_t m = 0;
+  CG_HIP(hipMemcpyAsync(&m, mx.p, 4, hipMemcpyDeviceToHost, s));
+  CG_HIP(hipStreamSynchronize(s));
+  if (m < 256u) {
+    CG_TRY(g->d_w8.ensure(g->E));
+    hipLaunchKernelGGL(k_narrow<uint8_t>, dim3(grid), dim3(kBlock), 0, s, g->d_w.p, g->E, g->d_w8.p);
+  } else if (m < 65536u) {
+    CG_TRY(g->d_w16.ensure(g->E));
+    hipLaunchKernelGGL(k_narrow<uint16_t>, dim3(grid), dim3(kBlock), 0, s, g->d_w.p, g->E, g->d_w16.p);
+  }
+  CG_LAUNCH_CHECK();
+  CG_HIP(hipStreamSynchronize(s));
+  g->dev_wn_valid = true;
+  return CG_OK;
 }
 
 int ensure_degrees(cg_graph* g, hipStream_t s) {

@@ -530,3 +530,24 @@ def test_back_to_back_runs(oracle):
             assert np.array_equal(e.result(), ref), (alg, relabel)
             e.close()
     g.close()
+
+
+@pytest.mark.parametrize("wmax", [255, 256, 65535, 65536, 10**8])
+def test_sssp_weight_widths(oracle, wmax):
+    """The push expansion of SSSP reads an 8-bit or 16-bit copy of the weights
+    when every weight fits (largest 255 / 65535), the 32-bit array otherwise:
+    distances bit-exact against the oracle at each boundary, under the GPU
+    schedule with and without hub deferral, on the caller's ids and on a
+    renumbered copy."""
+    og = random_graph(oracle, 31, 30000, 240000, hub=20000)
+    rng = np.random.default_rng(wmax % 1000)
+    og.weight = rng.integers(1, wmax + 1, og.E).astype(np.uint32)
+    og.weight[0] = wmax  # the largest weight is present
+    g = gpu_graph(og)
+    root = oracle.max_outdeg_vertex(og)
+    ref = oracle.sssp(og, root)
+    for opt in ({}, {"hub_degree": 16, "hub_delta": 3}, {"relabel": 2}):
+        got, st = run(g, SSSP, root, sched_mode=_lib.CG_SCHED_GPU, **opt)
+        assert np.array_equal(got, ref), (wmax, opt)
+        assert st["edges_traversed"] == oracle.reached_edges(og, ref)
+    g.close()
