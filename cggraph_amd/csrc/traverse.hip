@@ -834,7 +834,10 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint64_t* __restrict__
 
 constexpr uint32_t kDenseBlock = 256;  // frontier vertices per 4096 that pay for staging offsets
 constexpr uint64_t kFusedMaxBlocks = 1024;     // LDS staging: four workgroups per CU
-constexpr uint64_t kFusedMaxBlocksDeg = 2048;  // degree array: eight per CU
+// degree array, two passes: fewer workgroups take fewer queue atomics
+// (SSSP s24 per query: 512 / 1024 / 2048 workgroups 4.28-4.30 / 4.19-4.24 /
+// 4.35 ms; the one-pass form with 2048: 4.33-4.35 ms)
+constexpr uint64_t kFusedMaxBlocksDeg = 1024;
 // each workgroup of the fused compaction writes one statistics partial
 static_assert(kFusedMaxBlocks <= kStatsMaxBlocks && kFusedMaxBlocksDeg <= kStatsMaxBlocks,
               "the partials buffer (stats_parts_needed) holds kStatsMaxBlocks + 1 entries");
@@ -856,13 +859,137 @@ struct CompactAcc {
 // frontier vertices stages its offsets in LDS.  queue = false: nothing is
 // queued and no hub waits (statistics only).
 //
-// DEG: the degrees come from the graph's degree array (g.deg) instead: a lane
-// with more than kFewBits frontier vertices loads its slice's 16 degrees
-// with four 16-byte loads into its own LDS row, one with fewer loads each
-// vertex's degree; no offsets are staged, so the workgroup needs a tenth of
-// the LDS and four times as many workgroups fit on a CU.
+// With the graph's out-degree array (g.deg; the GPU and learned schedules)
+// the workgroup makes two passes over its slices instead: it counts (the
+// statistics, the hub decisions, its queue sizes), takes its three queue
+// ranges with one atomic each, then places; the queue counters see one atomic
+// per workgroup instead of one per workgroup and stride step, and neither
+// pass has a barrier.  A lane places the items of all its slices
+// contiguously.  The second pass decides the hubs again (no value changes
+// during the compaction, so the decisions agree).  A lane with more than
+// kFewBits frontier vertices in a slice loads the slice's 16 degrees with
+// four 16-byte loads into its own LDS row; one with fewer loads each
+// vertex's degree.
 constexpr int kFewBits = 4;
 constexpr int kDegRow = (int)kSliceBits + 1;  // odd stride: lanes' rows on distinct banks
+__device__ __forceinline__ void compact_frontier_deg(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                     uint64_t words, const DevGraph& g, const Bins& bins,
+                                                     uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
+                                                     uint64_t* __restrict__ q_chunk, DevStats* __restrict__ st,
+                                                     const uint32_t* __restrict__ val, const HubDefer& hub,
+                                                     bool queue, CompactAcc& a) {
+  __shared__ uint32_t s_tot[kWavesPerBlock][3];
+  __shared__ uint32_t s_base[kWavesPerBlock][3];
+  __shared__ uint32_t s_deg[kWavesPerBlock][kWave * kDegRow];
+  __shared__ WaveHubs s_hubs[kWavesPerBlock];
+  WaveHubs& hubs = s_hubs[wave_id()];
+  uint32_t* const drow = &s_deg[wave_id()][lane_id() * kDegRow];
+  uint32_t thr = 0xFFFFFFFFu;
+  if (hub.degree) {
+    const uint64_t t = (uint64_t)st->min_val + hub.delta;
+    thr = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+  }
+  const uint64_t slices = words * kSlicesPerWord;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  // calls f(v, deg) for the vertices of the lane's slice si in `bits`
+  auto each = [&](uint64_t si, uint32_t bits, auto&& f) {
+    const uint64_t vbase = si * kSliceBits;
+    if (__popc(bits) > kFewBits) {
+      const uint4* p = reinterpret_cast<const uint4*>(g.deg + vbase);
+      const uint4 d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3];
+      const uint32_t d[16] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w,
+                              d2.x, d2.y, d2.z, d2.w, d3.x, d3.y, d3.z, d3.w};
+#pragma unroll
+      for (int k = 0; k < 16; ++k) drow[k] = d[k];
+      for (; bits; bits &= bits - 1) {
+        const uint32_t j = (uint32_t)__builtin_ctz(bits);
+        f((uint32_t)(vbase + j), (uint64_t)drow[j]);
+      }
+    } else {
+      for (; bits; bits &= bits - 1) {
+        const uint32_t v = (uint32_t)(vbase + (uint32_t)__builtin_ctz(bits));
+        f(v, (uint64_t)g.deg[v]);
+      }
+    }
+  };
+  auto waits = [&](uint32_t v, uint64_t deg) -> bool {
+    return hub.degree && deg >= hub.degree && val[v] >= thr;
+  };
+  // pass 1: statistics, hubs kept waiting (written to the next bitmap, whose
+  // every word this pass writes), queue sizes
+  uint32_t c_small = 0, c_med = 0, c_chunk = 0, wait_min = 0xFFFFFFFFu;
+  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += stride) {
+    const uint64_t si = base + threadIdx.x;
+    const uint64_t wi = si / kSlicesPerWord;
+    const uint32_t word = si < slices ? slice_bits(in[wi], si) : 0u;
+    a.c_act += (uint32_t)__popc(word);
+    uint32_t wait = 0;
+    each(si, word, [&](uint32_t v, uint64_t deg) {
+      a.m += deg;
+      a.h.add(hist_bin(deg));
+      if (deg == 0) {
+        ++a.c_zero;
+        return;
+      }
+      if (waits(v, deg)) {
+        const uint32_t x = val[v];
+        wait |= 1u << (v - si * kSliceBits);
+        wait_min = x < wait_min ? x : wait_min;
+        ++a.n_def;
+        a.m_def += deg;
+        return;
+      }
+      if (deg < bins.bin_wave) c_small++;
+      else if (deg < bins.bin_chunk) c_med++;
+      else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+    });
+    const uint64_t wait_word = join_slices(wait, si);
+    if (si < slices && si % kSlicesPerWord == 0) out[wi] = wait_word;
+  }
+  // the waiting vertices are part of the next frontier's smallest value
+  if (hub.track) stripe_min(hub.track, wave_min_u32(wait_min));
+  if (!queue) c_small = c_med = c_chunk = 0;
+  uint32_t t_small, t_med, t_chunk;
+  const uint32_t o_small = wave_excl_scan(c_small, &t_small);
+  const uint32_t o_med = wave_excl_scan(c_med, &t_med);
+  const uint32_t o_chunk = wave_excl_scan(c_chunk, &t_chunk);
+  if (lane_id() == 0) {
+    s_tot[wave_id()][0] = t_small;
+    s_tot[wave_id()][1] = t_med;
+    s_tot[wave_id()][2] = t_chunk;
+    hubs.n = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) { s_base[w][q] = tot; tot += s_tot[w][q]; }
+    uint32_t b = 0;
+    if (tot) b = atomicAdd(q == 0 ? &st->n_small : (q == 1 ? &st->n_med : &st->n_chunk), tot);
+    for (int w = 0; w < kWavesPerBlock; ++w) s_base[w][q] += b;
+  }
+  __syncthreads();
+  uint32_t any = 0;
+#pragma unroll
+  for (int w = 0; w < kWavesPerBlock; ++w) any |= s_tot[w][0] | s_tot[w][1] | s_tot[w][2];
+  if (!any) return;  // uniform: nothing to place
+  // pass 2: the same slices, placed
+  uint32_t b_small = s_base[wave_id()][0] + o_small;
+  uint32_t b_med = s_base[wave_id()][1] + o_med;
+  uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
+  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += stride) {
+    const uint64_t si = base + threadIdx.x;
+    const uint32_t word = si < slices ? slice_bits(in[si / kSlicesPerWord], si) : 0u;
+    each(si, word, [&](uint32_t v, uint64_t deg) {
+      if (deg == 0 || waits(v, deg)) return;
+      if (deg < bins.bin_wave) q_small[b_small++] = v;
+      else if (deg < bins.bin_chunk) q_med[b_med++] = v;
+      else put_chunks(q_chunk, b_chunk, v, (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges), hubs);
+    });
+  }
+  write_hub_chunks(q_chunk, hubs);  // the wavefront's own list
+}
+
 template <bool DEG>
 __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                  uint64_t words, const DevGraph& g, const Bins& bins,
@@ -870,15 +997,17 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
                                                  uint64_t* __restrict__ q_chunk, DevStats* __restrict__ st,
                                                  const uint32_t* __restrict__ val, const HubDefer& hub, bool queue,
                                                  CompactAcc& a) {
+  if constexpr (DEG) {
+    compact_frontier_deg(in, out, words, g, bins, q_small, q_med, q_chunk, st, val, hub, queue, a);
+    return;
+  }
   __shared__ uint32_t s_tot[kWavesPerBlock][3];
   __shared__ uint32_t s_base[kWavesPerBlock][3];
   __shared__ uint32_t s_cnt[kWavesPerBlock];
   __shared__ uint64_t s_off[DEG ? 1 : kWavesPerBlock][DEG ? 1 : kStageEntries];
-  __shared__ uint32_t s_deg[DEG ? kWavesPerBlock : 1][DEG ? kWave * kDegRow : 1];
   __shared__ WaveHubs s_hubs[kWavesPerBlock];
   WaveHubs& hubs = s_hubs[wave_id()];
   uint64_t* const row = s_off[DEG ? 0 : wave_id()];
-  uint32_t* const drow = &s_deg[DEG ? wave_id() : 0][DEG ? lane_id() * kDegRow : 0];
   // hub threshold: the frontier's smallest value (left in st by the previous
   // iteration's finish) + delta
   uint32_t thr = 0xFFFFFFFFu;
@@ -895,18 +1024,8 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
     uint32_t word = si < slices ? slice_bits(in[wi], si) : 0u;
     a.c_act += (uint32_t)__popc(word);
     const bool any = __ballot(word != 0) != 0;  // wave-uniform
-    bool dense = false, staged = false;
-    if constexpr (DEG) {
-      staged = __popc(word) > kFewBits;
-      if (staged) {
-        const uint4* p = reinterpret_cast<const uint4*>(g.deg + vbase);
-        const uint4 d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3];
-        const uint32_t d[16] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w,
-                                d2.x, d2.y, d2.z, d2.w, d3.x, d3.y, d3.z, d3.w};
-#pragma unroll
-        for (int k = 0; k < 16; ++k) drow[k] = d[k];
-      }
-    } else {
+    bool dense = false;
+    {
       // block-uniform choice of the offsets' source
       const uint32_t wcnt = wave_sum_u32((uint32_t)__popc(word));
       if (lane_id() == 0) s_cnt[wave_id()] = wcnt;
@@ -922,14 +1041,7 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
     }
     // sparse workgroups: one offset pair per frontier vertex
     auto slice_each = [&](uint32_t bits, auto&& f) {
-      if constexpr (DEG) {
-        // a lane reads only its own LDS row, written by itself above
-        for (; bits; bits &= bits - 1) {
-          const uint32_t j = (uint32_t)__builtin_ctz(bits);
-          const uint32_t v = (uint32_t)(vbase + j);
-          f(v, (uint64_t)(staged ? drow[j] : g.deg[v]));
-        }
-      } else if (dense) {
+      if (dense) {
         for_each_in_slice_staged(bits, vbase, row, f);
       } else {
         for (; bits; bits &= bits - 1) {
