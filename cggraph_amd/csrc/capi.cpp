@@ -649,6 +649,7 @@ int cg_engine_create(const cg_graph* cg, int algorithm, int engine_type, int use
     if (!e->partial && !e->host_only) CG_TRY(ensure_degrees(g, e->stream));
     if (algorithm == CG_SSSP && !e->partial && !e->host_only) CG_TRY(ensure_narrow_weights(g, e->stream));
     CG_TRY(e->val.ensure(V));
+    if (algorithm != CG_BFS && !e->partial && !e->host_only) CG_TRY(e->xval.ensure(V));
     CG_TRY(e->bm[0].ensure(words));
     CG_TRY(e->bm[1].ensure(words));
     CG_TRY(e->q_small.ensure(V));

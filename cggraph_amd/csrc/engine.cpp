@@ -561,6 +561,9 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // statistics are counted by the compaction itself and reduced behind the
   // expansion (three kernels per iteration, one pass over the frontier)
   const bool fused = run_ahead && alg != CG_BFS && !delta;
+  // the fused compaction drops frontier vertices already expanded with their
+  // present value (it needs the degree array)
+  if (fused && dg.deg && e->xval.p) dg.xval = e->xval.p;
   {
     TraverseInit ti{};
     ti.val = e->val.p;
@@ -596,6 +599,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     ti.bfs_host_out = e->hst_map_dev + 1;
     ti.acc = e->acc.p;
     ti.epoch = e->epoch;
+    ti.xval = dg.xval;
     CG_TRY(launch_init_traverse(ti, s));
   }
   bool early_timed[4] = {false, false, false, false};

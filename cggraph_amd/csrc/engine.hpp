@@ -27,6 +27,7 @@ struct cg_engine {
 
   // values (uint32 for BFS/SSSP/CC, float for PR) and frontier state
   cg::DevBuf<uint32_t> val;
+  cg::DevBuf<uint32_t> xval;  // SSSP / CC fused pipeline: DevGraph::xval
   cg::DevBuf<uint64_t> bm[2];
   cg::DevBuf<uint32_t> q_small, q_med;
   cg::DevBuf<uint64_t> q_chunk;

@@ -92,6 +92,11 @@ struct DevGraph {
   // every weight fits it; the push expansion of SSSP reads it instead of w
   const uint8_t* w8 = nullptr;
   const uint16_t* w16 = nullptr;
+  // SSSP / CC push on the fused pipeline: per vertex, one plus a value with
+  // which every out-edge has been relaxed (0: none yet).  The expansion
+  // writes it; the compaction drops a frontier vertex whose value equals it,
+  // since expanding it again would offer every neighbour the same candidates.
+  uint32_t* xval = nullptr;
 };
 
 struct Bins {
@@ -163,7 +168,10 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
 // or the hubs kept waiting under `hub`) and one statistics partial per
 // workgroup into `parts` (*nparts set by the launcher, at most
 // stats_parts_needed(words) - 1).  The hub threshold is st->min_val +
-// hub.delta.  launch_fused_finish reduces the partials after the expansion
+// hub.delta.  With g.deg and g.xval, a frontier vertex whose value is the
+// one its out-edges were last relaxed with is dropped: its bit is cleared in
+// `in`, and it is neither queued nor counted.  launch_fused_finish reduces
+// the partials after the expansion
 // has run: it publishes the iteration's record (as k_stats_reduce does, plus
 // n_def / m_def of this iteration), folds the expansion's first-visit stripes
 // into the run state, and prepares `next` (the following iteration's slot):
@@ -176,7 +184,7 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
 // is -1), and for a host-team choice queues nothing and keeps no hub waiting.
 // rs: the run state k_fused_finish keeps; once it has counted an empty
 // frontier, the iterations queued behind return at once.
-int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
+int launch_compact_fused(uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
                          uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
                          const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts,
                          const RunState* rs, hipStream_t s, const TierTable* choice_tiers = nullptr);
@@ -220,6 +228,7 @@ struct TraverseInit {
   DevStats* bfs_host_out;
   unsigned long long* acc;  // the query's striped accumulators (kAccSlots), cleared
   uint32_t epoch;           // the run's tag (record_tag)
+  uint32_t* xval = nullptr; // DevGraph::xval of the run, cleared (V entries)
 };
 int launch_init_traverse(const TraverseInit& a, hipStream_t s);
 // BFS under the GPU schedule, two kernels per iteration (traverse.hip): the

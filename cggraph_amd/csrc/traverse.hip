@@ -271,7 +271,10 @@ __global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
   const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
   const bool cc = a.alg == CG_CC;
-  for (uint64_t i = t0; i < a.V; i += nt) a.val[i] = cc ? (uint32_t)i : (i == a.root ? 0u : kUnreached);
+  for (uint64_t i = t0; i < a.V; i += nt) {
+    a.val[i] = cc ? (uint32_t)i : (i == a.root ? 0u : kUnreached);
+    if (a.xval) a.xval[i] = 0;
+  }
   for (uint64_t i = t0; i < a.bm_words; i += nt) {
     uint64_t w = 0;
     if (cc) {
@@ -870,9 +873,17 @@ struct CompactAcc {
 // kFewBits frontier vertices in a slice loads the slice's 16 degrees with
 // four 16-byte loads into its own LDS row; one with fewer loads each
 // vertex's degree.
+//
+// With g.xval (SSSP / CC on the fused pipeline) the counting pass also drops
+// every frontier vertex whose value is one its out-edges were all relaxed
+// with (the expansion that read the value has finished, so each neighbour
+// has seen that candidate already): it is not counted, queued or kept
+// waiting, and its bit is cleared in `in`, which the placing pass reads
+// after the workgroup's barriers.  A vertex queued in chunks has its record
+// cleared here; its chunks raise it to the largest value they read.
 constexpr int kFewBits = 4;
 constexpr int kDegRow = (int)kSliceBits + 1;  // odd stride: lanes' rows on distinct banks
-__device__ __forceinline__ void compact_frontier_deg(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+__device__ __forceinline__ void compact_frontier_deg(uint64_t* in, uint64_t* __restrict__ out,
                                                      uint64_t words, const DevGraph& g, const Bins& bins,
                                                      uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
                                                      uint64_t* __restrict__ q_chunk, DevStats* __restrict__ st,
@@ -915,24 +926,35 @@ __device__ __forceinline__ void compact_frontier_deg(const uint64_t* __restrict_
   auto waits = [&](uint32_t v, uint64_t deg) -> bool {
     return hub.degree && deg >= hub.degree && val[v] >= thr;
   };
+  const bool drops = queue && val && g.xval;  // kernel-uniform
   // pass 1: statistics, hubs kept waiting (written to the next bitmap, whose
   // every word this pass writes), queue sizes
   uint32_t c_small = 0, c_med = 0, c_chunk = 0, wait_min = 0xFFFFFFFFu;
   for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += stride) {
     const uint64_t si = base + threadIdx.x;
     const uint64_t wi = si / kSlicesPerWord;
-    const uint32_t word = si < slices ? slice_bits(in[wi], si) : 0u;
-    a.c_act += (uint32_t)__popc(word);
-    uint32_t wait = 0;
+    const uint64_t full = si < slices ? in[wi] : 0ull;
+    const uint32_t word = slice_bits(full, si);
+    uint32_t wait = 0, drop = 0;
     each(si, word, [&](uint32_t v, uint64_t deg) {
+      uint32_t x = 0;
+      bool have_x = false;
+      if (drops && deg != 0) {
+        x = val[v];
+        have_x = true;
+        const uint32_t xr = g.xval[v];
+        if (xr != 0u && xr - 1u == x) {
+          drop |= 1u << (v - si * kSliceBits);
+          return;
+        }
+      }
       a.m += deg;
       a.h.add(hist_bin(deg));
       if (deg == 0) {
         ++a.c_zero;
         return;
       }
-      if (waits(v, deg)) {
-        const uint32_t x = val[v];
+      if (hub.degree && deg >= hub.degree && (have_x ? x : (x = val[v])) >= thr) {
         wait |= 1u << (v - si * kSliceBits);
         wait_min = x < wait_min ? x : wait_min;
         ++a.n_def;
@@ -943,8 +965,13 @@ __device__ __forceinline__ void compact_frontier_deg(const uint64_t* __restrict_
       else if (deg < bins.bin_chunk) c_med++;
       else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
     });
+    a.c_act += (uint32_t)__popc(word & ~drop);
     const uint64_t wait_word = join_slices(wait, si);
     if (si < slices && si % kSlicesPerWord == 0) out[wi] = wait_word;
+    if (drops) {
+      const uint64_t kept = full & ~join_slices(drop, si);
+      if (kept != full && si % kSlicesPerWord == 0) in[wi] = kept;
+    }
   }
   // the waiting vertices are part of the next frontier's smallest value
   if (hub.track) stripe_min(hub.track, wave_min_u32(wait_min));
@@ -984,14 +1011,17 @@ __device__ __forceinline__ void compact_frontier_deg(const uint64_t* __restrict_
       if (deg == 0 || waits(v, deg)) return;
       if (deg < bins.bin_wave) q_small[b_small++] = v;
       else if (deg < bins.bin_chunk) q_med[b_med++] = v;
-      else put_chunks(q_chunk, b_chunk, v, (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges), hubs);
+      else {
+        if (drops) g.xval[v] = 0u;
+        put_chunks(q_chunk, b_chunk, v, (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges), hubs);
+      }
     });
   }
   write_hub_chunks(q_chunk, hubs);  // the wavefront's own list
 }
 
 template <bool DEG>
-__device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+__device__ __forceinline__ void compact_frontier(uint64_t* in, uint64_t* __restrict__ out,
                                                  uint64_t words, const DevGraph& g, const Bins& bins,
                                                  uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_med,
                                                  uint64_t* __restrict__ q_chunk, DevStats* __restrict__ st,
@@ -1128,7 +1158,7 @@ __device__ __forceinline__ void compact_frontier(const uint64_t* __restrict__ in
 // schedules' SSSP / CC iterations), so the frontier and its offsets are read
 // once per iteration instead of twice.
 template <bool DEG>
-__global__ __launch_bounds__(kBlock) void k_compact_fused(const uint64_t* __restrict__ in,
+__global__ __launch_bounds__(kBlock) void k_compact_fused(uint64_t* __restrict__ in,
                                                           uint64_t* __restrict__ out, uint64_t words, DevGraph g,
                                                           Bins bins, uint32_t* __restrict__ q_small,
                                                           uint32_t* __restrict__ q_med,
@@ -1327,6 +1357,9 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
   const uint64_t n_items = (uint64_t)n_chunk + n_med + (n_small + kWave - 1) / kWave;
   const uint32_t lane = lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
+  // the value each source's edges are relaxed with is recorded (plus one) for
+  // the next compaction; a vertex in chunks keeps the largest its chunks read
+  uint32_t* const xval = ALG == ALG_BFS ? nullptr : g.xval;
   uint32_t n_new = 0;
   for (uint64_t item = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); item < n_items; item += stride) {
     if (item < n_chunk) {
@@ -1336,10 +1369,14 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
       const uint64_t s = g.off[v] + (uint64_t)(uint32_t)pk * chunk_edges;
       const uint64_t end = g.off[v + 1];
       const uint64_t e = s + chunk_edges < end ? s + chunk_edges : end;
-      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], s, e, lane, g.dst, w, val, out, acc);
+      const uint32_t sv = val[v];
+      if (xval && lane == 0) atomicMax(xval + v, sv + 1u);
+      n_new += relax_strided<ALG, EXACT, TRACK>(sv, s, e, lane, g.dst, w, val, out, acc);
     } else if (item < (uint64_t)n_chunk + n_med) {
       const uint32_t v = q_med[item - n_chunk];
-      n_new += relax_strided<ALG, EXACT, TRACK>(val[v], g.off[v], g.off[v + 1], lane, g.dst, w, val, out, acc);
+      const uint32_t sv = val[v];
+      if (xval && lane == 0) xval[v] = sv + 1u;
+      n_new += relax_strided<ALG, EXACT, TRACK>(sv, g.off[v], g.off[v + 1], lane, g.dst, w, val, out, acc);
     } else {
       // 64 low-degree vertices per wavefront, their edges spread evenly over
       // the lanes: lane j of round r takes edge r*64+j of the concatenated
@@ -1354,6 +1391,7 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
         s = g.off[v];
         deg = (uint32_t)(g.off[v + 1] - s);
         sv = val[v];
+        if (xval) xval[v] = sv + 1u;
       }
       uint32_t total;
       const uint32_t excl = wave_excl_scan(deg, &total);
@@ -1512,7 +1550,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_front(const uint64_t* __restrict
   // (the next run's initialisation clears both bitmaps).
   if (pull || n == 0) return;
   CompactAcc a;
-  compact_frontier<DEG>(in, out, words, g, bins, q_small, q_med, q_chunk, st, nullptr, HubDefer{0, 1}, true, a);
+  compact_frontier<DEG>(const_cast<uint64_t*>(in), out, words, g, bins, q_small, q_med, q_chunk, st, nullptr,
+                        HubDefer{0, 1}, true, a);  // nothing dropped without values: `in` is only read
 }
 
 template <bool EXACT>
@@ -1871,7 +1910,7 @@ int launch_compact(const uint64_t* in, uint64_t words, const DevGraph& g, const 
   return CG_OK;
 }
 
-int launch_compact_fused(const uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
+int launch_compact_fused(uint64_t* in, uint64_t* out, uint64_t words, const DevGraph& g, const Bins& bins,
                          uint32_t* q_small, uint32_t* q_med, uint64_t* q_chunk, DevStats* st,
                          const uint32_t* val, HubDefer hub, StatsPart* parts, uint32_t* nparts,
                          const RunState* rs, hipStream_t s, const TierTable* choice_tiers) {

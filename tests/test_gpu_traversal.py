@@ -551,3 +551,29 @@ def test_sssp_weight_widths(oracle, wmax):
         assert np.array_equal(got, ref), (wmax, opt)
         assert st["edges_traversed"] == oracle.reached_edges(og, ref)
     g.close()
+
+
+@pytest.mark.parametrize("tiers", [(4, 32, 64), (16, 1024, 2048), (64, 128, 64)])
+def test_propagated_vertices_dropped(oracle, tiers):
+    """SSSP and CC under the GPU schedule skip a frontier vertex whose value is
+    one its out-edges were already all relaxed with (the expansion records
+    it; a vertex split into chunks keeps the largest value its chunks read).
+    Tiers that put many vertices into chunks, with and without hub deferral,
+    and several roots on one engine (the records are cleared per query):
+    values bit-exact against the oracle, reached edges exact."""
+    og = random_graph(oracle, 37, 25000, 220000, hub=25000)
+    g = gpu_graph(og)
+    bw, bc, ce = tiers
+    roots = [oracle.max_outdeg_vertex(og), 3, 24999]
+    for alg in (SSSP, CC):
+        for hub in ((0, 1), (8, 2)) if alg == SSSP else ((0, 1),):
+            e = cg.Engine(g, alg, sched_mode=_lib.CG_SCHED_GPU, bin_wave=bw, bin_chunk=bc, chunk_edges=ce,
+                          hub_degree=hub[0], hub_delta=hub[1])
+            for r in roots + roots[:1]:
+                ref = oracle.sssp(og, r) if alg == SSSP else oracle.cc_directed(og)
+                got, st = e.run(r)
+                assert np.array_equal(got, ref), (alg, hub, r)
+                if alg == SSSP:
+                    assert st["edges_traversed"] == oracle.reached_edges(og, ref), (hub, r)
+            e.close()
+    g.close()
