@@ -41,6 +41,10 @@ using clk = std::chrono::steady_clock;
 // BFS on the GPU schedule: a frontier with fewer out-edges than this, not
 // larger than the one before, ends the queueing ahead (see engine_run)
 constexpr unsigned long long kBfsTailEdges = 256;
+// fused SSSP / CC: a frontier below kTailEdges edges that did not grow has the
+// expansions queued after it launched with at most kTailGrid workgroups
+constexpr unsigned long long kTailEdges = 1ull << 21;
+constexpr uint32_t kTailGrid = 2048;
 
 double ms_since(clk::time_point t0) {
   return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
@@ -612,11 +616,19 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     early_timed[k] = false;
     return CG_OK;
   };
+  // Fused pipeline: once the frontiers shrink below kTailEdges edges, the
+  // expansions queued next launch a grid of kTailGrid workgroups instead of
+  // the tuned cap (the grid strides over whatever the queues hold, so any
+  // grid gives the same result; an idle workgroup still costs dispatch time)
+  uint64_t last_m = ~0ull, prev_m = 0;  // frontier edges of the last two records (none yet)
   // iteration k reads bm[(k - 1) & 1], writes bm[k & 1], uses stats slot k & 1
   auto enqueue = [&](uint32_t k) -> int {
     const int sl = k & 1, c = (k - 1) & 1;
     DevStats* dk = e->st.p + sl;
     if (fused) {
+      Bins xb = fbins;
+      if (last_m < kTailEdges && last_m <= prev_m && (xb.grid_cap == 0 || xb.grid_cap > kTailGrid))
+        xb.grid_cap = kTailGrid;
       uint32_t nparts = 0;
       CG_TRY(launch_compact_fused(e->bm[c].p, e->bm[c ^ 1].p, words, dg, fbins, e->q_small.p, e->q_med.p,
                                   e->q_chunk.p, dk, e->val.p, hub, e->parts.p, &nparts, e->run_state.p, s,
@@ -624,7 +636,7 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       const int tp = k & 3;
       if (e->opt.profile) CG_HIP(hipEventRecord(e->ev_pair[tp][0], s));
       CG_TRY(launch_expand_push(alg, dg, e->val.p, e->bm[c ^ 1].p, e->q_small.p, e->q_med.p, e->q_chunk.p,
-                                ~0ull >> 8, SmallRange{}, fbins, dk, e->stripes.p, e->opt.atomic_return != 0, s,
+                                ~0ull >> 8, SmallRange{}, xb, dk, e->stripes.p, e->opt.atomic_return != 0, s,
                                 hub_on));
       if (e->opt.profile) {
         CG_HIP(hipEventRecord(e->ev_pair[tp][1], s));
@@ -737,6 +749,8 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
       local.expand_alg_bytes -= 20.0 * n_def + 12.0 * (double)m_def;
     }
     if (sst.n_active == 0) break;
+    prev_m = it > 1 ? last_m : 0;
+    last_m = sst.m_active;
     if (early_bfs) {
       if (!bfs_ahead) {
         CG_TRY(bfs_queue_upto(it));

@@ -36,7 +36,7 @@ struct cg_engine {
   cg::PinnedBuf<cg::DevStats> hst_map;  // mapped: k_stats publishes here
   cg::DevStats* hst_map_dev = nullptr;  // its device address
   cg::DevStats hst_last{};
-  cg::DevBuf<unsigned long long> acc;  // kAccSlots striped accumulators
+  cg::DevBuf<unsigned long long> acc;  // kAccSlots accumulators (kAccLines used, a line apart)
   // run number tagged into the records (record_tag), and whether the last
   // run returned normally: only no-op iterations can then be left on the
   // stream, and the next run need not drain it (an error return may leave

@@ -294,10 +294,13 @@ int launch_gather_values(const uint32_t* ids, uint32_t n, const uint32_t* val, u
                          hipStream_t s);
 // The query's last pass over its vertices, one kernel: with acc, the sum of
 // out-degrees of the vertices whose value != CG_UNREACHED (the TEPS
-// numerator), added into kStripes striped slots (acc[k * kStripeStride / 2],
+// numerator), added into kAccLines slots one 128-byte line apart
+// (acc[k * kAccStride], zero on entry; one atomic per workgroup, the
+// workgroups spread over the lines),
 // zero on entry); with new_of, the values in the caller's vertex ids
 // (out[i] = val[new_of[i]]) for an engine that runs on a renumbered copy.
-constexpr int kAccSlots = kStripes * kStripeStride / 2;
+constexpr int kAccLines = 256, kAccStride = 16;
+constexpr int kAccSlots = kAccLines * kAccStride;
 // Builds the graph's device out-degree array once (DevGraph::deg).
 int ensure_degrees(cg_graph* g, hipStream_t s);
 // Builds, once, a copy of the weights in 8 or 16 bits when every weight fits

@@ -35,6 +35,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kFinishBlock = 256;  // the fused finish (kFinishParts compaction partials per thread)
+constexpr int kFinishParts = 4;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / kWave; }
@@ -271,7 +273,23 @@ __global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
   const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
   const bool cc = a.alg == CG_CC;
-  for (uint64_t i = t0; i < a.V; i += nt) {
+  // values (and records) four at a time with 16-byte stores, then the rest
+  const uint64_t n4 = a.V / 4;
+  for (uint64_t i = t0; i < n4; i += nt) {
+    const uint32_t b = (uint32_t)(4 * i);
+    uint4 x;
+    if (cc) {
+      x = make_uint4(b, b + 1, b + 2, b + 3);
+    } else {
+      const bool here = (a.root >> 2) == i;
+      const uint32_t r = a.root & 3u;
+      x = make_uint4(here && r == 0 ? 0u : kUnreached, here && r == 1 ? 0u : kUnreached,
+                     here && r == 2 ? 0u : kUnreached, here && r == 3 ? 0u : kUnreached);
+    }
+    reinterpret_cast<uint4*>(a.val)[i] = x;
+    if (a.xval) reinterpret_cast<uint4*>(a.xval)[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  for (uint64_t i = 4 * n4 + t0; i < a.V; i += nt) {
     a.val[i] = cc ? (uint32_t)i : (i == a.root ? 0u : kUnreached);
     if (a.xval) a.xval[i] = 0;
   }
@@ -878,10 +896,12 @@ struct CompactAcc {
 // every frontier vertex whose value is one its out-edges were all relaxed
 // with (the expansion that read the value has finished, so each neighbour
 // has seen that candidate already): it is not counted, queued or kept
-// waiting, and its bit is cleared in `in`, which the placing pass reads
-// after the workgroup's barriers.  A vertex queued in chunks has its record
-// cleared here; its chunks raise it to the largest value they read.
+// waiting, and the placing pass skips it (its bit is cleared in the cached
+// word, or in `in` when the workgroup has more strides than it caches).  A
+// vertex queued in chunks has its record cleared here; its chunks raise it
+// to the largest value they read.
 constexpr int kFewBits = 4;
+constexpr int kPre = 4;
 constexpr int kDegRow = (int)kSliceBits + 1;  // odd stride: lanes' rows on distinct banks
 __device__ __forceinline__ void compact_frontier_deg(uint64_t* in, uint64_t* __restrict__ out,
                                                      uint64_t words, const DevGraph& g, const Bins& bins,
@@ -927,50 +947,73 @@ __device__ __forceinline__ void compact_frontier_deg(uint64_t* in, uint64_t* __r
     return hub.degree && deg >= hub.degree && val[v] >= thr;
   };
   const bool drops = queue && val && g.xval;  // kernel-uniform
+  // The workgroup's slices, kPre strides at a time: their frontier words are
+  // loaded together (kPre loads in flight per lane instead of one), and when
+  // the workgroup has at most kPre strides the placing pass takes the words
+  // from registers instead of reading them again.
+  const uint64_t base0 = blockIdx.x * (uint64_t)kBlock;
+  const uint64_t nstr = base0 < slices ? (slices - base0 + stride - 1) / stride : 0;  // block-uniform
+  const bool cached = nstr <= (uint64_t)kPre;
+  auto load_words = [&](uint64_t s0, uint64_t (&w)[kPre]) {
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+      const uint64_t si = base0 + (s0 + j) * stride + threadIdx.x;
+      w[j] = s0 + j < nstr && si < slices ? in[si / kSlicesPerWord] : 0ull;
+    }
+  };
+  uint64_t keep[kPre] = {};  // the cached words, dropped vertices removed
   // pass 1: statistics, hubs kept waiting (written to the next bitmap, whose
   // every word this pass writes), queue sizes
   uint32_t c_small = 0, c_med = 0, c_chunk = 0, wait_min = 0xFFFFFFFFu;
-  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += stride) {
-    const uint64_t si = base + threadIdx.x;
-    const uint64_t wi = si / kSlicesPerWord;
-    const uint64_t full = si < slices ? in[wi] : 0ull;
-    const uint32_t word = slice_bits(full, si);
-    uint32_t wait = 0, drop = 0;
-    each(si, word, [&](uint32_t v, uint64_t deg) {
-      uint32_t x = 0;
-      bool have_x = false;
-      if (drops && deg != 0) {
-        x = val[v];
-        have_x = true;
-        const uint32_t xr = g.xval[v];
-        if (xr != 0u && xr - 1u == x) {
-          drop |= 1u << (v - si * kSliceBits);
+  for (uint64_t s0 = 0; s0 < nstr; s0 += kPre) {
+    uint64_t fulls[kPre];
+    load_words(s0, fulls);
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+      if (s0 + j >= nstr) break;  // block-uniform
+      const uint64_t si = base0 + (s0 + j) * stride + threadIdx.x;
+      const uint64_t wi = si / kSlicesPerWord;
+      const uint32_t word = slice_bits(fulls[j], si);
+      uint32_t wait = 0, drop = 0;
+      each(si, word, [&](uint32_t v, uint64_t deg) {
+        uint32_t x = 0;
+        bool have_x = false;
+        if (drops && deg != 0) {
+          x = val[v];
+          have_x = true;
+          const uint32_t xr = g.xval[v];
+          if (xr != 0u && xr - 1u == x) {
+            drop |= 1u << (v - si * kSliceBits);
+            return;
+          }
+        }
+        a.m += deg;
+        a.h.add(hist_bin(deg));
+        if (deg == 0) {
+          ++a.c_zero;
           return;
         }
+        if (hub.degree && deg >= hub.degree && (have_x ? x : (x = val[v])) >= thr) {
+          wait |= 1u << (v - si * kSliceBits);
+          wait_min = x < wait_min ? x : wait_min;
+          ++a.n_def;
+          a.m_def += deg;
+          return;
+        }
+        if (deg < bins.bin_wave) c_small++;
+        else if (deg < bins.bin_chunk) c_med++;
+        else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
+      });
+      a.c_act += (uint32_t)__popc(word & ~drop);
+      const uint64_t wait_word = join_slices(wait, si);
+      if (si < slices && si % kSlicesPerWord == 0) out[wi] = wait_word;
+      uint64_t kept = fulls[j];
+      if (drops) {
+        kept &= ~join_slices(drop, si);
+        // read back by the placing pass after the workgroup's barriers
+        if (!cached && kept != fulls[j] && si % kSlicesPerWord == 0) in[wi] = kept;
       }
-      a.m += deg;
-      a.h.add(hist_bin(deg));
-      if (deg == 0) {
-        ++a.c_zero;
-        return;
-      }
-      if (hub.degree && deg >= hub.degree && (have_x ? x : (x = val[v])) >= thr) {
-        wait |= 1u << (v - si * kSliceBits);
-        wait_min = x < wait_min ? x : wait_min;
-        ++a.n_def;
-        a.m_def += deg;
-        return;
-      }
-      if (deg < bins.bin_wave) c_small++;
-      else if (deg < bins.bin_chunk) c_med++;
-      else c_chunk += (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges);
-    });
-    a.c_act += (uint32_t)__popc(word & ~drop);
-    const uint64_t wait_word = join_slices(wait, si);
-    if (si < slices && si % kSlicesPerWord == 0) out[wi] = wait_word;
-    if (drops) {
-      const uint64_t kept = full & ~join_slices(drop, si);
-      if (kept != full && si % kSlicesPerWord == 0) in[wi] = kept;
+      keep[j] = kept;
     }
   }
   // the waiting vertices are part of the next frontier's smallest value
@@ -1004,18 +1047,28 @@ __device__ __forceinline__ void compact_frontier_deg(uint64_t* in, uint64_t* __r
   uint32_t b_small = s_base[wave_id()][0] + o_small;
   uint32_t b_med = s_base[wave_id()][1] + o_med;
   uint32_t b_chunk = s_base[wave_id()][2] + o_chunk;
-  for (uint64_t base = blockIdx.x * (uint64_t)kBlock; base < slices; base += stride) {
-    const uint64_t si = base + threadIdx.x;
-    const uint32_t word = si < slices ? slice_bits(in[si / kSlicesPerWord], si) : 0u;
-    each(si, word, [&](uint32_t v, uint64_t deg) {
-      if (deg == 0 || waits(v, deg)) return;
-      if (deg < bins.bin_wave) q_small[b_small++] = v;
-      else if (deg < bins.bin_chunk) q_med[b_med++] = v;
-      else {
-        if (drops) g.xval[v] = 0u;
-        put_chunks(q_chunk, b_chunk, v, (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges), hubs);
-      }
-    });
+  for (uint64_t s0 = 0; s0 < nstr; s0 += kPre) {
+    uint64_t fulls[kPre];
+    if (cached) {
+#pragma unroll
+      for (int j = 0; j < kPre; ++j) fulls[j] = keep[j];
+    } else {
+      load_words(s0, fulls);
+    }
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+      if (s0 + j >= nstr) break;  // block-uniform
+      const uint64_t si = base0 + (s0 + j) * stride + threadIdx.x;
+      each(si, slice_bits(fulls[j], si), [&](uint32_t v, uint64_t deg) {
+        if (deg == 0 || waits(v, deg)) return;
+        if (deg < bins.bin_wave) q_small[b_small++] = v;
+        else if (deg < bins.bin_chunk) q_med[b_med++] = v;
+        else {
+          if (drops) g.xval[v] = 0u;
+          put_chunks(q_chunk, b_chunk, v, (uint32_t)((deg + bins.chunk_edges - 1) / bins.chunk_edges), hubs);
+        }
+      });
+    }
   }
   write_hub_chunks(q_chunk, hubs);  // the wavefront's own list
 }
@@ -1206,42 +1259,63 @@ __global__ __launch_bounds__(kBlock) void k_compact_fused(uint64_t* __restrict__
 // (published to mapped host memory with `iter` last), the first-visit
 // stripes of the expansion folded into the run state, and the next slot
 // prepared for the next compaction.
-__global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __restrict__ parts, uint32_t nparts,
-                                                         uint32_t* __restrict__ stripes, RunState* __restrict__ rs,
-                                                         DevStats* __restrict__ st, DevStats* __restrict__ next,
-                                                         DevStats* __restrict__ host_out, int track,
-                                                         const float* __restrict__ mlp, uint32_t V, uint64_t E) {
+__global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* __restrict__ parts,
+                                                               uint32_t nparts, uint32_t* __restrict__ stripes,
+                                                               RunState* __restrict__ rs, DevStats* __restrict__ st,
+                                                               DevStats* __restrict__ next,
+                                                               DevStats* __restrict__ host_out, int track,
+                                                               const float* __restrict__ mlp, uint32_t V, uint64_t E) {
   if (rs->done) return;  // queued behind the traversal's end (uniform)
-  constexpr int kC = 12;  // n_active, n_zero, hist[8], n_def, spare
-  __shared__ unsigned long long s_m[kBlock], s_md[kBlock];
-  __shared__ uint32_t s_c[kBlock][kC];
+  constexpr int kC = 11;  // n_active, n_zero, hist[8], n_def
+  constexpr int kFW = kFinishBlock / kWave;
+  __shared__ unsigned long long s_m[kFW], s_md[kFW];
+  __shared__ uint32_t s_c[kFW][kC];
   __shared__ float w_mlp[kMlpLds];
   // the network's weights are loaded now and stored to LDS after the
   // reduction, whose latency hides theirs
-  MlpStage<kBlock> wst;
+  MlpStage<kFinishBlock> wst;
   if (mlp) wst.load(mlp);
+  // kFinishParts partials per thread, loaded together (a further round only
+  // past kFinishBlock * kFinishParts partials); wavefront sums, then the
+  // wavefronts' sums in order
   unsigned long long m = 0, md = 0;
-  uint32_t c[kC] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t b = threadIdx.x; b < nparts; b += kBlock) {
-    const StatsPart q = parts[b];
-    m += q.m_active;
-    md += q.m_def;
-    c[0] += q.n_active;
-    c[1] += q.n_zero;
-    for (int k = 0; k < 8; ++k) c[2 + k] += q.hist[k];
-    c[10] += q.n_def;
-  }
-  s_m[threadIdx.x] = m;
-  s_md[threadIdx.x] = md;
-  for (int k = 0; k < kC; ++k) s_c[threadIdx.x][k] = c[k];
-  __syncthreads();
-  for (int w = kBlock / 2; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s_m[threadIdx.x] += s_m[threadIdx.x + w];
-      s_md[threadIdx.x] += s_md[threadIdx.x + w];
-      for (int k = 0; k < kC; ++k) s_c[threadIdx.x][k] += s_c[threadIdx.x + w][k];
+  uint32_t c[kC] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b0 = 0; b0 < nparts; b0 += kFinishBlock * kFinishParts) {
+    StatsPart q[kFinishParts];
+#pragma unroll
+    for (int j = 0; j < kFinishParts; ++j) {
+      const uint32_t b = b0 + j * kFinishBlock + threadIdx.x;
+      if (b < nparts) q[j] = parts[b];
+      else q[j] = StatsPart{};
     }
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kFinishParts; ++j) {
+      m += q[j].m_active;
+      md += q[j].m_def;
+      c[0] += q[j].n_active;
+      c[1] += q[j].n_zero;
+      for (int k = 0; k < 8; ++k) c[2 + k] += q[j].hist[k];
+      c[10] += q[j].n_def;
+    }
+  }
+  m = wave_sum_u64(m);
+  md = wave_sum_u64(md);
+#pragma unroll
+  for (int k = 0; k < kC; ++k) c[k] = wave_sum_u32(c[k]);
+  if (lane_id() == 0) {
+    s_m[wave_id()] = m;
+    s_md[wave_id()] = md;
+#pragma unroll
+    for (int k = 0; k < kC; ++k) s_c[wave_id()][k] = c[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    // wavefront 0 sums the wavefronts' sums, one per lane
+    const uint32_t l = threadIdx.x;
+    m = wave_sum_u64(l < kFW ? s_m[l] : 0ull);
+    md = wave_sum_u64(l < kFW ? s_md[l] : 0ull);
+#pragma unroll
+    for (int k = 0; k < kC; ++k) c[k] = wave_sum_u32(l < kFW ? s_c[l][k] : 0u);
   }
   __shared__ float feat[CG_NUM_FEATURES];
   __shared__ float mlp_sh[2 * kMlpH + kMlpOut];
@@ -1258,12 +1332,12 @@ __global__ __launch_bounds__(kBlock) void k_fused_finish(const StatsPart* __rest
   d.n_med = st->n_med;
   d.n_chunk = st->n_chunk;
   d.min_val = st->min_val;
-  d.n_active = s_c[0][0];
-  d.n_zero_deg = s_c[0][1];
-  d.m_active = s_m[0];
-  for (int k = 0; k < 8; ++k) d.hist[k] = s_c[0][2 + k];
-  d.n_def = s_c[0][10];
-  d.m_def = s_md[0];
+  d.n_active = c[0];
+  d.n_zero_deg = c[1];
+  d.m_active = m;
+  for (int k = 0; k < 8; ++k) d.hist[k] = c[2 + k];
+  d.n_def = c[10];
+  d.m_def = md;
   // the record describes the frontier this iteration expanded: vertices
   // reached before it, and the first visits of the previous expansion
   d.visited = rs->visited;
@@ -1764,9 +1838,22 @@ __global__ __launch_bounds__(kBlock) void k_finish_query(DevGraph g, const uint3
                                                          uint32_t* __restrict__ out) {
   __shared__ unsigned long long s[kWavesPerBlock];
   uint64_t sum = 0;
-  for (uint64_t v = blockIdx.x * (uint64_t)kBlock + threadIdx.x; v < g.V;
-       v += (uint64_t)gridDim.x * kBlock) {
-    if (acc && val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
+  const uint64_t t0 = blockIdx.x * (uint64_t)kBlock + threadIdx.x, nt = (uint64_t)gridDim.x * kBlock;
+  uint64_t v0 = 0;  // vertices below v0 are summed four at a time
+  if (acc && g.deg) {
+    // 16-byte loads of four values and four out-degrees
+    const uint64_t n4 = g.V / 4;
+    const uint4* v4 = reinterpret_cast<const uint4*>(val);
+    const uint4* d4 = reinterpret_cast<const uint4*>(g.deg);
+    for (uint64_t i = t0; i < n4; i += nt) {
+      const uint4 x = v4[i], d = d4[i];
+      sum += (uint64_t)(x.x != kUnreached ? d.x : 0u) + (x.y != kUnreached ? d.y : 0u) +
+             (x.z != kUnreached ? d.z : 0u) + (x.w != kUnreached ? d.w : 0u);
+    }
+    v0 = n4 * 4;
+  }
+  for (uint64_t v = t0; v < g.V; v += nt) {
+    if (acc && v >= v0 && val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
     if (new_of) out[v] = val[new_of[v]];
   }
   if (!acc) return;  // uniform
@@ -1776,7 +1863,7 @@ __global__ __launch_bounds__(kBlock) void k_finish_query(DevGraph g, const uint3
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (int w = 0; w < kWavesPerBlock; ++w) t += s[w];
-    if (t) atomicAdd(acc + (blockIdx.x % kStripes) * (kStripeStride / 2), t);
+    if (t) atomicAdd(acc + (blockIdx.x % kAccLines) * kAccStride, t);
   }
 }
 
@@ -1934,7 +2021,7 @@ int launch_compact_fused(uint64_t* in, uint64_t* out, uint64_t words, const DevG
 int launch_fused_finish(const StatsPart* parts, uint32_t nparts, uint32_t* stripes, RunState* rs, DevStats* st,
                         DevStats* next, DevStats* host_out, bool track, const float* mlp, uint32_t V, uint64_t E,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_fused_finish, dim3(1), dim3(kBlock), 0, s, parts, nparts, stripes, rs, st, next, host_out,
+  hipLaunchKernelGGL(k_fused_finish, dim3(1), dim3(kFinishBlock), 0, s, parts, nparts, stripes, rs, st, next, host_out,
                      track ? 1 : 0, mlp, V, E);
   CG_LAUNCH_CHECK();
   return CG_OK;
@@ -2007,7 +2094,7 @@ int launch_pull_bfs(const DevGraph& g, uint32_t* val, const uint64_t* in, uint64
 }
 
 int launch_init_traverse(const TraverseInit& a, hipStream_t s) {
-  uint64_t n = a.V > a.bm_words ? a.V : a.bm_words;
+  uint64_t n = a.V / 4 > a.bm_words ? a.V / 4 : a.bm_words;
   if (n < 2ull * kBfsCntWords) n = 2ull * kBfsCntWords;
   hipLaunchKernelGGL(k_init_traverse, dim3(grid_for(n, kBlock, 65536)), dim3(kBlock), 0, s, a);
   CG_LAUNCH_CHECK();
@@ -2183,12 +2270,16 @@ int launch_finish_query(const DevGraph& g, const uint32_t* val, unsigned long lo
   return CG_OK;
 }
 
-__global__ void k_query_done(const unsigned long long* __restrict__ acc, DevStats* __restrict__ host_out,
-                             uint32_t tag) {
+__global__ __launch_bounds__(kAccLines) void k_query_done(const unsigned long long* __restrict__ acc,
+                                                           DevStats* __restrict__ host_out, uint32_t tag) {
+  // one line's slot per thread, wavefront sums, the wavefronts' sums in order
+  __shared__ unsigned long long s[kAccLines / kWave];
+  const uint64_t x = wave_sum_u64(acc ? acc[threadIdx.x * kAccStride] : 0ull);
+  if (lane_id() == 0) s[wave_id()] = x;
+  __syncthreads();
   if (threadIdx.x != 0) return;
   DevStats d{};
-  if (acc)
-    for (int k = 0; k < kStripes; ++k) d.m_active += acc[k * (kStripeStride / 2)];
+  for (int w = 0; w < kAccLines / kWave; ++w) d.m_active += s[w];
   d.iter = 0xFFFFFFFFu;
   *host_out = d;
   // the release store orders the record before `iter` at system scope
@@ -2196,7 +2287,7 @@ __global__ void k_query_done(const unsigned long long* __restrict__ acc, DevStat
 }
 
 int launch_query_done(const unsigned long long* acc, DevStats* host_out, uint32_t tag, hipStream_t s) {
-  hipLaunchKernelGGL(k_query_done, dim3(1), dim3(64), 0, s, acc, host_out, tag);
+  hipLaunchKernelGGL(k_query_done, dim3(1), dim3(kAccLines), 0, s, acc, host_out, tag);
   CG_LAUNCH_CHECK();
   return CG_OK;
 }
