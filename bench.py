@@ -333,7 +333,9 @@ def run_learned(cg, g, root, tuned, ref_vals):
 
 def run_batched(cg, np):
     """256 BFS + 256 SSSP queries over R-MAT s14-22 and lattices 256^2-2048^2:
-    one HIP-graph-captured batch vs one query per engine launch."""
+    the batched run (the R-MAT s18-22 queries one by one on engines of their
+    graphs from a host thread, the rest level by level in one HIP-graph-
+    captured loop, both at once) vs one query per engine launch."""
     from cggraph_amd import autotune, batch
     graphs, qg, qa, qr = batch.mixed_workload()
     opts = autotune.best_options("batch/mixed") or {"sssp_delta": 10000}
@@ -348,6 +350,7 @@ def run_batched(cg, np):
     res = {"queries": len(qr), "graphs": len(graphs), "options": opts,
            "batched_ms": round(st_b["ms"], 2), "batched_gteps": round(st_b["edges_traversed"] / st_b["ms"] / 1e6, 3),
            "batched_levels": st_b["iterations"], "graph_replays": st_b["expand_launches"],
+           "queries_on_engines": st_b["heavy_iterations"],
            "one_per_launch_ms": round(st_1["ms"], 2),
            "one_per_launch_gteps": round(st_1["edges_traversed"] / st_1["ms"] / 1e6, 3),
            "one_per_launch_wall_s": round(wall_1, 2), "speedup": round(st_1["ms"] / st_b["ms"], 2),

@@ -62,6 +62,7 @@ class Options(ctypes.Structure):
         ("hub_delta", ctypes.c_uint32),
         ("probe_stride", ctypes.c_uint32),
         ("hbm_budget", ctypes.c_uint64),
+        ("batch_engine_v", ctypes.c_uint32),
     ]
 
 

@@ -40,6 +40,7 @@ BATCH_SPACE = {
     "batch_block": [256, 512, 1024],
     "batch_levels": [8, 32, 128],
     "batch_group": [16, 32, 64],
+    "batch_engine_v": [1 << 17, 1 << 18, 1 << 19, 1 << 20, 0xFFFFFFFF],
 }
 
 

@@ -30,6 +30,8 @@
 #include <chrono>
 #include <memory>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -336,6 +338,93 @@ static int build_batch_relabel(cg_graph* g) {
   return CG_OK;
 }
 
+// Queries on a small-world graph of at least batch_engine_v vertices run one
+// by one on an engine of their own graph -- renumbering, hub deferral, degree
+// tiers, direction-optimising BFS: what one query of a large skewed graph
+// runs best with -- from a host thread of their own, while the other queries
+// (lattices and small graphs, whose many shallow levels are what batching
+// amortises) advance level by level on the batch's stream.  The values of
+// each engine query are copied on the engine's stream into a slot of the
+// batch before the engine takes its next root.
+constexpr uint32_t kBatchEngineMinV = 1u << 18;
+
+// Whether the batch hands g's queries to an engine: at least min_v vertices
+// and a largest out-degree of at least 16 times the average, the mark of a
+// small-world graph (few, wide levels; a lattice or a road network has many
+// narrow ones, which is what the level-synchronous batch amortises).
+static bool batch_on_engine(const cg_graph* g, uint32_t min_v) {
+  if (min_v == 0xFFFFFFFFu || g->V < min_v || !g->V) return false;
+  const uint64_t avg = (g->E + g->V - 1) / g->V;
+  return g->max_outdeg >= 16 * (avg ? avg : 1);
+}
+
+// The graph's engine for alg, created on first use and destroyed with the
+// graph: GPU schedule, default tiers; SSSP defers hubs of out-degree >= 128
+// while their distance is >= the frontier's smallest + max weight / 50 (the
+// window the s24 tuning found for weights up to 100 scaled to the graph's
+// weights: with weights up to 1000 a window of 2 took 3-4x more iterations).
+static int batch_engine(cg_graph* g, int alg, cg_engine** out) {
+  std::lock_guard<std::mutex> lock(batch_relabel_mu);
+  auto& slot = g->batch_eng[alg == CG_SSSP ? 1 : 0];
+  if (!slot) {
+    cg_options eo;
+    cg_options_default(&eo);
+    eo.sched_mode = CG_SCHED_GPU;
+    if (alg == CG_SSSP) {
+      CG_TRY(ensure_narrow_weights(g, nullptr));
+      eo.hub_degree = 128;
+      eo.hub_delta = g->max_w / 50 > 1 ? g->max_w / 50 : 1;
+    }
+    cg_engine* e = nullptr;
+    const int rc = cg_engine_create(g, alg, CG_COORDINATION, 1, &eo, &e);
+    if (rc != CG_OK) return rc;
+    slot = std::unique_ptr<cg_engine, void (*)(cg_engine*)>(e, cg_engine_destroy);
+  }
+  *out = slot.get();
+  return CG_OK;
+}
+
+struct EngineQuery {
+  cg_engine* e;
+  uint32_t root;
+  uint32_t* slot;  // the query's V values, in the caller's ids
+};
+
+// The engine queries in order, on the calling host thread; returns once every
+// copy into a slot has finished.  Errors come back in rc / msg (the error
+// text is per thread).
+static void run_engine_queries(int dev, const std::vector<EngineQuery>* jobs, int* rc, std::string* msg,
+                               unsigned long long* edges) {
+  auto fail = [&](int code, const char* what) {
+    if (*rc == CG_OK) {
+      *rc = code;
+      *msg = what;
+    }
+  };
+  *rc = CG_OK;
+  *edges = 0;
+  if (hipSetDevice(dev) != hipSuccess) return fail(CG_ERR_HIP, "hipSetDevice on the engine thread failed");
+  std::vector<cg_engine*> used;
+  for (const EngineQuery& j : *jobs) {
+    cg_run_stats st{};
+    const int r = cg_engine_run(j.e, j.root, nullptr, &st);
+    if (r != CG_OK) {
+      fail(r, cg_last_error());
+      break;
+    }
+    *edges += st.edges_traversed;
+    const uint32_t* src = j.e->renumbered ? j.e->val_out.p : j.e->val.p;
+    if (hipMemcpyAsync(j.slot, src, (size_t)j.e->caller_g->V * 4, hipMemcpyDeviceToDevice, j.e->stream) !=
+        hipSuccess) {
+      fail(CG_ERR_HIP, "copy of an engine query's values failed");
+      break;
+    }
+    if (std::find(used.begin(), used.end(), j.e) == used.end()) used.push_back(j.e);
+  }
+  for (cg_engine* e : used)
+    if (hipStreamSynchronize(e->stream) != hipSuccess) fail(CG_ERR_HIP, "engine stream failed");
+}
+
 int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots, int nq, uint32_t* const* outs,
                 const cg_options& opt, cg_run_stats* rs) {
   CG_TRY(require_device());
@@ -343,6 +432,7 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
   const uint32_t grid = opt.batch_grid ? opt.batch_grid : 2048;
   const uint32_t levels = opt.batch_levels ? opt.batch_levels : 16;
   const uint32_t gsz = opt.batch_group ? opt.batch_group : 32;
+  const uint32_t min_v = opt.batch_engine_v ? opt.batch_engine_v : kBatchEngineMinV;
   if (gsz > 64) return set_error(CG_ERR_INVALID, "batch_group must be in [1, 64]");
   if (block % 64 || block > 1024) return set_error(CG_ERR_INVALID, "batch_block must be a multiple of 64, <= 1024");
   for (int i = 0; i < nq; ++i) {
@@ -353,18 +443,46 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
     if (roots[i] >= graphs[i]->V) return set_error(CG_ERR_INVALID, "query %d: root %u >= V", i, roots[i]);
     CG_TRY(graphs[i]->ensure_device());
   }
-  // the graph each query runs on, its root there, and the map back
-  std::vector<cg_graph*> qgraph(graphs, graphs + nq);
-  std::vector<uint32_t> qroot(roots, roots + nq);
-  std::vector<const uint32_t*> back(nq, nullptr);
-  uint32_t back_v = 0;
+  // the queries engines run (jq, in order) and those the batch runs (bq)
+  std::vector<int> bq, jq;
+  std::vector<EngineQuery> jobs;
   for (int i = 0; i < nq; ++i) {
-    cg_graph* g = graphs[i];
+    if (batch_on_engine(graphs[i], min_v)) {
+      cg_engine* e = nullptr;
+      CG_TRY(batch_engine(graphs[i], algs[i], &e));
+      jobs.push_back(EngineQuery{e, roots[i], nullptr});
+      jq.push_back(i);
+    } else {
+      bq.push_back(i);
+    }
+  }
+  uint64_t slot_v = 0;
+  for (int i : jq) slot_v += graphs[i]->V;
+  DevBuf<uint32_t> slots;
+  if (slot_v) CG_TRY(slots.ensure(slot_v));
+  std::vector<uint32_t*> jslot(jq.size());
+  {
+    uint64_t o = 0;
+    for (size_t k = 0; k < jq.size(); ++k) {
+      jslot[k] = jobs[k].slot = slots.p + o;
+      o += graphs[jq[k]]->V;
+    }
+  }
+  const int nb = (int)bq.size();
+  // the graph each batched query runs on, its root there, and the map back
+  std::vector<cg_graph*> qgraph(nb);
+  std::vector<uint32_t> qroot(nb);
+  std::vector<const uint32_t*> back(nb, nullptr);
+  uint32_t back_v = 0;
+  for (int k = 0; k < nb; ++k) {
+    cg_graph* g = graphs[bq[k]];
+    qgraph[k] = g;
+    qroot[k] = roots[bq[k]];
     if (g->V >= kBatchRelabelMinV) CG_TRY(build_batch_relabel(g));
     if (g->batch_rg) {
-      qgraph[i] = g->batch_rg.get();
-      qroot[i] = g->batch_new_of_h[roots[i]];
-      back[i] = g->batch_new_of.p;
+      qgraph[k] = g->batch_rg.get();
+      qroot[k] = g->batch_new_of_h[roots[bq[k]]];
+      back[k] = g->batch_new_of.p;
       back_v = std::max(back_v, g->V);
     }
   }
@@ -372,101 +490,124 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
   CG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   std::unique_ptr<std::remove_pointer<hipStream_t>::type, decltype(&hipStreamDestroy)> sguard(s, hipStreamDestroy);
   // per-query device arrays: values, stamps and two frontier queues
-  std::vector<uint64_t> vpre(nq + 1, 0);
-  for (int i = 0; i < nq; ++i) vpre[i + 1] = vpre[i] + qgraph[i]->V;
-  const uint64_t totalV = vpre[nq];
+  std::vector<uint64_t> vpre(nb + 1, 0);
+  for (int k = 0; k < nb; ++k) vpre[k + 1] = vpre[k] + qgraph[k]->V;
+  const uint64_t totalV = vpre[nb];
   // the batch indexes its items (vertex slots of all queries) with 32 bits
   if (totalV > 0xFFFFFFFFull)
     return set_error(CG_ERR_INVALID, "the batch holds %llu vertex slots over its queries; at most 2^32 - 1",
                      (unsigned long long)totalV);
   DevBuf<uint32_t> pool;
-  CG_TRY(pool.ensure(4 * totalV));
-  std::vector<QueryDesc> hq(nq);
-  for (int i = 0; i < nq; ++i) {
-    cg_graph* g = qgraph[i];
-    uint32_t* b = pool.p + 4 * vpre[i];
-    const uint32_t delta = algs[i] == CG_SSSP ? opt.sssp_delta : 0;
-    const uint32_t hub_deg = algs[i] == CG_SSSP ? opt.hub_degree : 0;
-    const uint32_t hub_delta = opt.hub_delta ? opt.hub_delta : kHubDeltaDefault;
-    hq[i] = QueryDesc{g->d_off.p, g->d_dst.p, g->has_w ? g->d_w.p : nullptr, b, b + g->V,
-                      {b + 2 * (uint64_t)g->V, b + 3 * (uint64_t)g->V}, g->V, algs[i], qroot[i], delta, hub_deg,
-                      hub_delta};
-  }
+  std::vector<QueryDesc> hq(nb);
   DevBuf<QueryDesc> dq;
   DevBuf<uint64_t> dvpre;
   DevBuf<uint32_t> cnt, item_off, qs;
   DevBuf<BatchState> bs;
-  CG_TRY(dq.ensure(nq));
-  CG_TRY(dvpre.ensure(nq + 1));
-  CG_TRY(cnt.ensure(2 * (uint64_t)nq));
-  CG_TRY(qs.ensure(3 * (uint64_t)nq));
-  CG_TRY(item_off.ensure(nq + 1));
-  CG_TRY(bs.ensure(1));
-  CG_HIP(hipMemcpyAsync(dq.p, hq.data(), nq * sizeof(QueryDesc), hipMemcpyHostToDevice, s));
-  CG_HIP(hipMemcpyAsync(dvpre.p, vpre.data(), (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-  CG_HIP(hipStreamSynchronize(s));
-
-  // capture `levels` levels once
   GraphExec ge;
-  CG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (uint32_t l = 0; l < levels; ++l) {
-    hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanBlock), 0, s, dq.p, cnt.p, qs.p, nq, item_off.p, bs.p);
-    hipLaunchKernelGGL(k_batch_expand, dim3(grid), dim3(block), 0, s, dq.p, nq, item_off.p, cnt.p, qs.p, qs.p,
-                       bs.p, gsz);
-  }
-  hipError_t ce = hipStreamEndCapture(s, &ge.graph);
-  if (ce != hipSuccess) return set_error(CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
-  CG_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+  if (nb) {
+    CG_TRY(pool.ensure(4 * totalV));
+    for (int k = 0; k < nb; ++k) {
+      cg_graph* g = qgraph[k];
+      const int alg = algs[bq[k]];
+      uint32_t* b = pool.p + 4 * vpre[k];
+      const uint32_t delta = alg == CG_SSSP ? opt.sssp_delta : 0;
+      const uint32_t hub_deg = alg == CG_SSSP ? opt.hub_degree : 0;
+      const uint32_t hub_delta = opt.hub_delta ? opt.hub_delta : kHubDeltaDefault;
+      hq[k] = QueryDesc{g->d_off.p, g->d_dst.p, g->has_w ? g->d_w.p : nullptr, b, b + g->V,
+                        {b + 2 * (uint64_t)g->V, b + 3 * (uint64_t)g->V}, g->V, alg, qroot[k], delta, hub_deg,
+                        hub_delta};
+    }
+    CG_TRY(dq.ensure(nb));
+    CG_TRY(dvpre.ensure(nb + 1));
+    CG_TRY(cnt.ensure(2 * (uint64_t)nb));
+    CG_TRY(qs.ensure(3 * (uint64_t)nb));
+    CG_TRY(item_off.ensure(nb + 1));
+    CG_TRY(bs.ensure(1));
+    CG_HIP(hipMemcpyAsync(dq.p, hq.data(), nb * sizeof(QueryDesc), hipMemcpyHostToDevice, s));
+    CG_HIP(hipMemcpyAsync(dvpre.p, vpre.data(), (nb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    CG_HIP(hipStreamSynchronize(s));
 
-  const auto t0 = std::chrono::steady_clock::now();
-  const unsigned ig = (unsigned)std::min<uint64_t>((totalV + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_batch_init, dim3(ig ? ig : 1), dim3(256), 0, s, dq.p, nq, dvpre.p, totalV);
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1024), 0, s, dq.p, cnt.p, qs.p, nq, bs.p);
-  BATCH_LAUNCH_OK();
+    // capture `levels` levels once
+    CG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (uint32_t l = 0; l < levels; ++l) {
+      hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanBlock), 0, s, dq.p, cnt.p, qs.p, nb, item_off.p, bs.p);
+      hipLaunchKernelGGL(k_batch_expand, dim3(grid), dim3(block), 0, s, dq.p, nb, item_off.p, cnt.p, qs.p, qs.p,
+                         bs.p, gsz);
+    }
+    hipError_t ce = hipStreamEndCapture(s, &ge.graph);
+    if (ce != hipSuccess) return set_error(CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+    CG_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+  }
+  int dev = 0;
+  CG_HIP(hipGetDevice(&dev));
+
+  // ---- timed: the engine thread and the level-synchronous batch together ----
   BatchState hb{};
   uint32_t replays = 0;
-  for (;;) {
-    CG_HIP(hipGraphLaunch(ge.exec, s));
-    ++replays;
-    CG_HIP(hipMemcpyAsync(&hb, bs.p, sizeof(hb), hipMemcpyDeviceToHost, s));
-    CG_HIP(hipStreamSynchronize(s));
-    if (hb.done) break;
-    if (replays > (1u << 22)) return set_error(CG_ERR_INTERNAL, "batch did not converge");
-  }
+  auto batch_loop = [&]() -> int {
+    const unsigned ig = (unsigned)std::min<uint64_t>((totalV + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_batch_init, dim3(ig ? ig : 1), dim3(256), 0, s, dq.p, nb, dvpre.p, totalV);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1024), 0, s, dq.p, cnt.p, qs.p, nb, bs.p);
+    BATCH_LAUNCH_OK();
+    for (;;) {
+      CG_HIP(hipGraphLaunch(ge.exec, s));
+      ++replays;
+      CG_HIP(hipMemcpyAsync(&hb, bs.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+      CG_HIP(hipStreamSynchronize(s));
+      if (hb.done) return CG_OK;
+      if (replays > (1u << 22)) return set_error(CG_ERR_INTERNAL, "batch did not converge");
+    }
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  int wrc = CG_OK;
+  std::string wmsg;
+  unsigned long long wedges = 0;
+  std::thread worker;
+  if (!jobs.empty()) worker = std::thread(run_engine_queries, dev, &jobs, &wrc, &wmsg, &wedges);
+  const int brc = nb ? batch_loop() : CG_OK;
+  if (worker.joinable()) worker.join();
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (brc != CG_OK) return brc;
+  if (wrc != CG_OK) return set_error(wrc, "batched query on an engine: %s", wmsg.c_str());
 
   // TEPS numerator per query and the outputs, outside the timed region
   DevBuf<unsigned long long> acc;
-  CG_TRY(acc.ensure(64 * 16));
-  CG_HIP(hipMemsetAsync(acc.p, 0, 64 * 16 * sizeof(unsigned long long), s));
-  for (int i = 0; i < nq; ++i) {
-    const unsigned bg = (unsigned)std::min<uint64_t>(((uint64_t)hq[i].V + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_batch_reached, dim3(bg ? bg : 1), dim3(256), 0, s, dq.p, i, acc.p);
+  std::vector<unsigned long long> hacc(64 * 16, 0ull);
+  if (nb) {
+    CG_TRY(acc.ensure(64 * 16));
+    CG_HIP(hipMemsetAsync(acc.p, 0, 64 * 16 * sizeof(unsigned long long), s));
+    for (int k = 0; k < nb; ++k) {
+      const unsigned bg = (unsigned)std::min<uint64_t>(((uint64_t)hq[k].V + 255) / 256, 2048);
+      hipLaunchKernelGGL(k_batch_reached, dim3(bg ? bg : 1), dim3(256), 0, s, dq.p, k, acc.p);
+    }
+    BATCH_LAUNCH_OK();
+    CG_HIP(hipMemcpyAsync(hacc.data(), acc.p, hacc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   }
-  BATCH_LAUNCH_OK();
-  std::vector<unsigned long long> hacc(64 * 16);
-  CG_HIP(hipMemcpyAsync(hacc.data(), acc.p, hacc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   DevBuf<uint32_t> unmapped;  // values of a renumbered query in the caller's ids
   if (outs && back_v) CG_TRY(unmapped.ensure(back_v));
-  for (int i = 0; i < nq; ++i) {
-    if (!outs || !outs[i]) continue;
-    const uint32_t* src = hq[i].val;
-    if (back[i]) {
+  for (int k = 0; k < nb && outs; ++k) {
+    if (!outs[bq[k]]) continue;
+    const uint32_t* src = hq[k].val;
+    if (back[k]) {
       // stream order: the previous query's copy out of `unmapped` is done
-      CG_TRY(launch_renumber_back(back[i], hq[i].val, hq[i].V, unmapped.p, s));
+      CG_TRY(launch_renumber_back(back[k], hq[k].val, hq[k].V, unmapped.p, s));
       src = unmapped.p;
     }
-    CG_HIP(hipMemcpyAsync(outs[i], src, (size_t)hq[i].V * 4, hipMemcpyDeviceToHost, s));
+    CG_HIP(hipMemcpyAsync(outs[bq[k]], src, (size_t)hq[k].V * 4, hipMemcpyDeviceToHost, s));
   }
+  for (size_t k = 0; k < jq.size() && outs; ++k)
+    if (outs[jq[k]])
+      CG_HIP(hipMemcpyAsync(outs[jq[k]], jslot[k], (size_t)graphs[jq[k]]->V * 4, hipMemcpyDeviceToHost, s));
   CG_HIP(hipStreamSynchronize(s));
   if (rs) {
     std::memset(rs, 0, sizeof(*rs));
     rs->ms = ms;
     rs->iterations = hb.levels_run;
     rs->expand_launches = replays;
-    unsigned long long m = 0;
+    unsigned long long m = wedges;
     for (auto x : hacc) m += x;
     rs->edges_traversed = m;
+    rs->heavy_iterations = (uint32_t)jq.size();  // queries run on engines
   }
   return CG_OK;
 }

@@ -335,6 +335,7 @@ int cg_options_default(cg_options* o) {
   o->hub_degree = 0;
   o->hub_delta = 0;
   o->probe_stride = 0;
+  o->batch_engine_v = 0;
   return CG_OK;
 }
 

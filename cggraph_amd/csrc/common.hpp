@@ -143,6 +143,7 @@ struct cg_graph {
   cg::DevBuf<uint32_t> d_deg;
   // the weights in 8 or 16 bits when all of them fit (at most one is built)
   bool dev_wn_valid = false;
+  uint32_t max_w = 0;  // the largest weight (set with dev_wn_valid)
   cg::DevBuf<uint8_t> d_w8;
   cg::DevBuf<uint16_t> d_w16;
   // transposed graph (in-edges sorted by source), built on demand
@@ -156,6 +157,11 @@ struct cg_graph {
   std::unique_ptr<cg_graph> batch_rg;
   cg::DevBuf<uint32_t> batch_new_of;
   std::vector<uint32_t> batch_new_of_h;
+  // batched queries that run on an engine of their own (a large
+  // small-world graph): the engines (BFS, SSSP), created on first use with
+  // cg_engine_destroy as their deleter; declared last, so they go before
+  // the buffers they refer to
+  std::unique_ptr<cg_engine, void (*)(cg_engine*)> batch_eng[2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 
   int ensure_host();    // D2H when only the device copy exists
   int ensure_device();  // H2D when only the host copy exists

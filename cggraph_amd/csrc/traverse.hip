@@ -2244,6 +2244,7 @@ int ensure_narrow_weights(cg_graph* g, hipStream_t s) {
   }
   CG_LAUNCH_CHECK();
   CG_HIP(hipStreamSynchronize(s));
+  g->max_w = m;
   g->dev_wn_valid = true;
   return CG_OK;
 }

@@ -174,6 +174,14 @@ typedef struct cg_options {
                                      host team relaxes the other segments' frontier vertices every
                                      iteration (GPU push for the resident part, no bottom-up steps,
                                      no renumbering).  PageRank needs the whole graph in HBM */
+  uint32_t batch_engine_v;        /* cg_batch_run (batched mode): the queries on a graph of at least
+                                     this many vertices whose largest out-degree is at least 16
+                                     times the average (a small-world graph: few, wide levels)
+                                     run one by one on an engine of their own graph (GPU
+                                     schedule; SSSP defers hubs of out-degree >= 128 within
+                                     max weight / 50 of the frontier's smallest), on a host thread
+                                     of their own while the other queries advance level by
+                                     level; 0 = 2^18, 0xFFFFFFFF = every query batched */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */
@@ -283,9 +291,14 @@ int cg_graph_device_csr(const cg_graph* g, const uint64_t** offsets, const uint3
 
 /* ---- batched queries ---- */
 /* nq queries (algorithm[i] BFS or SSSP, root[i]) over graph[i]; outs[i] is a
- * host uint32[V_i] buffer or NULL.  All queries run in one persistent kernel
- * per launch; the per-level loop is captured in a HIP graph.  mode 0: batched
- * persistent kernel; 1: one query per launch (the comparison path). */
+ * host uint32[V_i] buffer or NULL.  mode 0 (batched): the queries on large
+ * skewed graphs (opt->batch_engine_v) run one by one on an engine of their
+ * graph from a host thread of their own, while all the others advance level
+ * by level together, two launches per level, the per-level loop captured in
+ * a HIP graph; stats: ms over both, iterations = levels, expand_launches =
+ * graph replays, heavy_iterations = queries run on engines, edges_traversed =
+ * the TEPS numerator over all queries.  mode 1: one query per engine launch
+ * (the comparison path). */
 int cg_batch_run(cg_graph* const* graphs, const int* algorithms, const uint32_t* roots, int nq,
                  uint32_t* const* outs, int mode, const cg_options* opt, cg_run_stats* stats);
 /* The same batched run over engine handles: query i is engine i's algorithm

@@ -33,6 +33,43 @@ def test_batch_matches_oracle(oracle):
     assert st1["edges_traversed"] == st["edges_traversed"]
 
 
+@pytest.mark.parametrize("engine_v", [1, 3000, 0xFFFFFFFF])
+def test_batch_engine_split(oracle, engine_v):
+    """Queries on skewed graphs of at least batch_engine_v vertices run on an
+    engine of their graph (a host thread of their own, values copied into
+    the batch's slots) while the rest run level by level: every output is
+    the oracle's, the same with every query batched, repeated roots on one
+    engine included, and the counts add up."""
+    ogs = [oracle.rmat_graph(s, 16, seed=s, weights=(1, 1000)) for s in (10, 12)]
+    ogs += [oracle.grid_graph(n, 0.1, seed=n) for n in (25, 40)]
+    gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
+    rng = np.random.default_rng(3)
+    qg, qo, qa, qr = [], [], [], []
+    for i in range(36):
+        k = i % len(gs)
+        qg.append(gs[k])
+        qo.append(ogs[k])
+        qa.append((i // 4) % 2)
+        qr.append(int(rng.integers(0, ogs[k].V)) if i % 9 else 0)
+    outs, st = batch.run_batch(qg, qa, qr, batch_engine_v=engine_v, batch_levels=4, batch_grid=64)
+    for o, a, r, got in zip(qo, qa, qr, outs):
+        ref = oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)
+        assert np.array_equal(got, ref), (engine_v, a, r)
+    assert st["edges_traversed"] == sum(oracle.reached_edges(o, x) for o, x in zip(qo, outs))
+    def small_world(o):  # largest out-degree >= 16 x the average, rounded up
+        deg = np.diff(o.offsets.astype(np.int64))
+        return deg.max() >= 16 * max(-(-o.E // o.V), 1)
+    assert small_world(ogs[0]) and small_world(ogs[1]) and not small_world(ogs[2]) and not small_world(ogs[3])
+    on_engines = sum(1 for o in qo if engine_v != 0xFFFFFFFF and o.V >= engine_v and small_world(o))
+    assert st["heavy_iterations"] == on_engines
+    outs_b, st_b = batch.run_batch(qg, qa, qr, batch_engine_v=0xFFFFFFFF, batch_levels=4, batch_grid=64)
+    assert st_b["heavy_iterations"] == 0
+    for a, b in zip(outs, outs_b):
+        assert np.array_equal(a, b)
+    for g in gs:
+        g.close()
+
+
 @pytest.mark.parametrize("block,grid,levels,delta,group", [(64, 1, 1, 0, 64), (1024, 512, 32, 0, 1),
                                                            (256, 64, 8, 1, 16), (256, 2048, 16, 700, 32),
                                                            (512, 128, 4, 10**7, 7)])
