@@ -35,7 +35,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kFinishBlock = 256;  // the fused finish (kFinishParts compaction partials per thread)
+constexpr int kFinishBlock = 256;
+constexpr unsigned long long kNetEdgeShare = 256;  // the fused finish's network: frontiers of >= E / 256 edges  // the fused finish (kFinishParts compaction partials per thread)
 constexpr int kFinishParts = 4;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -1321,6 +1322,7 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
   __shared__ float mlp_sh[2 * kMlpH + kMlpOut];
   __shared__ float probs[kMlpOut];
   __shared__ int best;
+  __shared__ int s_net;
   DevStats d{};
   if (threadIdx.x == 0) {
   uint32_t nn = 0;
@@ -1365,10 +1367,17 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
   next->n_chunk = 0;
   next->min_val = mv;
   make_features(d, V, E, d.visited, d.iter, feat);
+  // the network runs on frontiers holding at least 1/kNetEdgeShare of the
+  // edges; after a smaller one the next iteration takes the GPU rule (its
+  // choices there are within noise of the tuned tiers, and the forward pass
+  // would cost more than it could save)
+  s_net = mlp && d.m_active * kNetEdgeShare >= E ? 1 : 0;
   }
-  if (mlp) wst.store(w_mlp);
   __syncthreads();
-  if (mlp) {
+  const bool net = s_net != 0;
+  if (net) wst.store(w_mlp);
+  __syncthreads();
+  if (net) {
     // the learned schedule of the next iteration, from this frontier's
     // features; the network defers (-1) unless its output is a finite
     // distribution whose top choice has probability >= kLearnedMinProb
@@ -1376,7 +1385,7 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
   }
   if (threadIdx.x != 0) return;
   int choice = -1;
-  if (mlp) {
+  if (net) {
     float sum = 0.f;
     bool finite = true;
     for (int k = 0; k < kMlpOut; ++k) {

@@ -204,7 +204,10 @@ typedef struct cg_run_stats {
   uint32_t expand_launches;       /* expansion kernels launched */
   double expand_kernel_ms;        /* summed HIP-event time of those launches (profile=1) */
   double expand_alg_bytes;        /* algorithmic bytes those launches moved */
-  uint32_t learned_fallbacks;     /* iterations where the learned scheduler deferred */
+  uint32_t learned_fallbacks;     /* iterations that took the GPU rule under the learned scheduler: the
+                                     first, those after a frontier of fewer than E / 256 edges on
+                                     the queued-ahead SSSP / CC pipeline (the network does not run
+                                     there), and those whose network output was not confident */
   uint64_t resident_edges;        /* edges held in HBM (E unless the graph ran partly resident) */
   uint32_t pr_layout;             /* PageRank: the layout the engine chose (pr_layout values) */
   double pr_stream_bytes;         /* PageRank, layouts 4 and 5: bytes streamed per iteration by
