@@ -16,12 +16,17 @@
 //                   carried to the next frontier unexpanded (delta-stepping,
 //                   Meyer and Sanders 2003); the threshold moves past the
 //                   smallest waiting distance when nothing below it is left.
-// The launch arguments never change from level to level (the level number,
+// Up to kLevelMaxQ queries a level is one launch instead, k_batch_level:
+// every workgroup scans the frontier sizes itself into LDS and takes the
+// thresholds there, with counters and minima in rotating sets.
+// The launch arguments never change from replay to replay (the level number,
 // parity, thresholds and item count live in device memory), so
 // `batch_levels` levels are captured once into a HIP graph and the graph is
 // replayed until the done flag is set -- one host synchronisation per replay
-// instead of one per level per query.  mode 1 of cg_batch_run runs the same
-// queries one per engine launch for comparison.
+// instead of one per level per query.  Queries on large small-world graphs
+// run beside the levels on engines of their graphs (run_batched).  mode 1
+// of cg_batch_run runs the same queries one per engine launch for
+// comparison.
 //
 // The relaxation is the reference's (my_project/CGgraph/device_algorithm.cuh:
 // 51-54 BFS, 253-257 SSSP), so each query ends at its unique fixed point,
@@ -62,7 +67,8 @@ struct BatchState {
   uint32_t done;
   uint32_t total;        // items of the current level
   uint32_t levels_run;   // levels with work
-  uint32_t pad[3];
+  uint32_t base;         // one kernel per level: the level of the replay's first node
+  uint32_t pad[2];
 };
 
 constexpr int kPlanBlock = 1024;
@@ -189,19 +195,16 @@ __device__ __forceinline__ void wave_append(bool push, int q, uint32_t v, uint32
   }
 }
 
-__global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restrict__ qd, int nq,
-                                                       const uint32_t* __restrict__ item_off,
-                                                       uint32_t* __restrict__ cnt, const uint32_t* __restrict__ qs,
-                                                       uint32_t* __restrict__ qs_mut, BatchState* __restrict__ bs,
-                                                       uint32_t gsz) {
-  const uint32_t total = bs->total;
-  if (total == 0) return;
-  const uint32_t L = bs->cur_level;
-  const int cp = L & 1, np = cp ^ 1;
-  uint32_t* nxt_cnt = cnt + np * nq;
-  const uint32_t* thr = qs;
-  const uint32_t* minv_cur = qs + nq + cp * nq;
-  uint32_t* minv_nxt = qs_mut + nq + np * nq;
+// One level's expansion: the wavefronts of the grid stride over groups of
+// gsz items of the level's frontiers (item_off: exclusive prefix of their
+// sizes over the queries, total items; thr / minv_cur: each query's
+// threshold and frontier minimum for this level; nxt_cnt / minv_nxt: the
+// next frontiers' counters and minima; np: the parity of the next fronts).
+__device__ __forceinline__ void expand_level(const QueryDesc* __restrict__ qd, int nq, const uint32_t* item_off,
+                                             uint32_t total, uint32_t L, const uint32_t* thr,
+                                             const uint32_t* minv_cur, uint32_t* __restrict__ nxt_cnt,
+                                             uint32_t* __restrict__ minv_nxt, int np, uint32_t gsz) {
+  const int cp = np ^ 1;
   const uint32_t lane = __lane_id();
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
   // wavefronts stride over groups of gsz items (no shared cursor to contend
@@ -276,6 +279,126 @@ __global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restri
       wave_append(push, oq, d, cand, qd, nxt_cnt, minv_nxt, np, lane);
     }
   }
+}
+
+__global__ __launch_bounds__(1024) void k_batch_expand(const QueryDesc* __restrict__ qd, int nq,
+                                                       const uint32_t* __restrict__ item_off,
+                                                       uint32_t* __restrict__ cnt, const uint32_t* __restrict__ qs,
+                                                       uint32_t* __restrict__ qs_mut, BatchState* __restrict__ bs,
+                                                       uint32_t gsz) {
+  const uint32_t total = bs->total;
+  if (total == 0) return;
+  const uint32_t L = bs->cur_level;
+  const int cp = L & 1, np = cp ^ 1;
+  expand_level(qd, nq, item_off, total, L, qs, qs + nq + cp * nq, cnt + np * nq, qs_mut + nq + np * nq, np, gsz);
+}
+
+// One kernel per level (at most kLevelMaxQ queries): every workgroup scans
+// the level's frontier sizes itself into LDS and takes each query's
+// threshold for the level (k_batch_plan's rule), so no planning launch sits
+// between two levels.  Counters and minima are in three rotating sets
+// (level L reads set L % 3, appends to set (L + 1) % 3, and workgroup 0
+// clears set (L + 2) % 3, which level L - 1 read and level L + 1 appends
+// to), thresholds in two (workgroup 0 stores level L's for level L + 1).
+// The level is bs->base + k, k the node's place in the captured replay,
+// whose last node advances bs->base.
+constexpr int kLevelMaxQ = 4096;
+
+__global__ __launch_bounds__(1024) void k_batch_level(const QueryDesc* __restrict__ qd, int nq,
+                                                      uint32_t* __restrict__ cnt3, uint32_t* __restrict__ thr2,
+                                                      uint32_t* __restrict__ minv3, BatchState* __restrict__ bs,
+                                                      uint32_t gsz, uint32_t k) {
+  extern __shared__ uint32_t sh[];
+  uint32_t* s_off = sh;            // nq + 1
+  uint32_t* s_thr = sh + nq + 1;   // nq
+  uint32_t* s_minv = s_thr + nq;   // nq
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
+  const uint32_t L = bs->base + k;
+  const uint32_t* cur = cnt3 + (size_t)(L % 3) * nq;
+  uint32_t* nxt = cnt3 + (size_t)((L + 1) % 3) * nq;
+  uint32_t* clr = cnt3 + (size_t)((L + 2) % 3) * nq;
+  const uint32_t* minv_c = minv3 + (size_t)(L % 3) * nq;
+  uint32_t* minv_n = minv3 + (size_t)((L + 1) % 3) * nq;
+  uint32_t* minv_clr = minv3 + (size_t)((L + 2) % 3) * nq;
+  const uint32_t* thr_prev = thr2 + (size_t)((L + 1) & 1) * nq;
+  uint32_t* thr_cur = thr2 + (size_t)(L & 1) * nq;
+  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64, nw = blockDim.x / 64;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  // exclusive scan of the frontier sizes over the queries
+  for (int base = 0; base < nq; base += (int)blockDim.x) {
+    const int i = base + (int)threadIdx.x;
+    const uint32_t x = i < nq ? cur[i] : 0u;
+    uint32_t v = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(v, d, 64);
+      if (lane >= (uint32_t)d) v += t;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (uint32_t w = 0; w < wave; ++w) wpre += wsum[w];
+    if (i < nq) s_off[i] = carry + wpre + v - x;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += wpre + v;
+    __syncthreads();
+  }
+  // the level's thresholds: a query with nothing below its threshold moves
+  // it past its smallest waiting distance
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) {
+    uint32_t t = thr_prev[i];
+    const uint32_t mc = minv_c[i];
+    const uint32_t d = qd[i].delta;
+    if (d && cur[i] && mc >= t) {
+      const uint64_t u = (uint64_t)mc + d;
+      t = u > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)u;
+    }
+    s_thr[i] = t;
+    s_minv[i] = mc;
+    if (blockIdx.x == 0) {
+      thr_cur[i] = t;
+      clr[i] = 0;
+      minv_clr[i] = kNone;
+    }
+  }
+  if (threadIdx.x == 0) s_off[nq] = carry;
+  __syncthreads();
+  const uint32_t total = s_off[nq];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (total == 0) bs->done = 1;
+    else bs->levels_run = L + 1;
+  }
+  if (total == 0) return;  // uniform
+  (void)nw;
+  expand_level(qd, nq, s_off, total, L, s_thr, s_minv, nxt, minv_n, (int)((L + 1) & 1), gsz);
+}
+
+// Counters for the one-kernel-per-level form: level 0's set holds each
+// query's root (minimum 0), the others are empty; thresholds of "level -1".
+__global__ void k_batch_level_reset(const QueryDesc* __restrict__ qd, uint32_t* __restrict__ cnt3,
+                                    uint32_t* __restrict__ thr2, uint32_t* __restrict__ minv3, int nq,
+                                    BatchState* __restrict__ bs) {
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) {
+    cnt3[i] = 1;
+    cnt3[nq + i] = 0;
+    cnt3[2 * nq + i] = 0;
+    minv3[i] = 0;
+    minv3[nq + i] = kNone;
+    minv3[2 * nq + i] = kNone;
+    thr2[nq + i] = qd[i].delta ? qd[i].delta : kNone;
+  }
+  if (threadIdx.x == 0) {
+    bs->done = 0;
+    bs->total = 0;
+    bs->levels_run = 0;
+    bs->base = 0;
+  }
+}
+
+__global__ void k_batch_advance(BatchState* __restrict__ bs, uint32_t levels) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) bs->base += levels;
 }
 
 __global__ void k_batch_reached(const QueryDesc* __restrict__ qd, int q, unsigned long long* __restrict__ acc) {
@@ -499,6 +622,8 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
                      (unsigned long long)totalV);
   DevBuf<uint32_t> pool;
   std::vector<QueryDesc> hq(nb);
+  const bool one_kernel = nb <= kLevelMaxQ;
+  const size_t level_lds = (3 * (size_t)nb + 1) * sizeof(uint32_t);
   DevBuf<QueryDesc> dq;
   DevBuf<uint64_t> dvpre;
   DevBuf<uint32_t> cnt, item_off, qs;
@@ -519,21 +644,29 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
     }
     CG_TRY(dq.ensure(nb));
     CG_TRY(dvpre.ensure(nb + 1));
-    CG_TRY(cnt.ensure(2 * (uint64_t)nb));
+    CG_TRY(cnt.ensure(3 * (uint64_t)nb));
     CG_TRY(qs.ensure(3 * (uint64_t)nb));
-    CG_TRY(item_off.ensure(nb + 1));
+    CG_TRY(item_off.ensure(2 * (uint64_t)nb + 1));  // one kernel per level: the two threshold sets
     CG_TRY(bs.ensure(1));
     CG_HIP(hipMemcpyAsync(dq.p, hq.data(), nb * sizeof(QueryDesc), hipMemcpyHostToDevice, s));
     CG_HIP(hipMemcpyAsync(dvpre.p, vpre.data(), (nb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     CG_HIP(hipStreamSynchronize(s));
 
-    // capture `levels` levels once
+    // capture `levels` levels once: one kernel per level (counters, minima
+    // and thresholds in cnt / qs / item_off as three, three and two sets)
+    // up to kLevelMaxQ queries, a planning and an expansion kernel beyond
     CG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (uint32_t l = 0; l < levels; ++l) {
-      hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanBlock), 0, s, dq.p, cnt.p, qs.p, nb, item_off.p, bs.p);
-      hipLaunchKernelGGL(k_batch_expand, dim3(grid), dim3(block), 0, s, dq.p, nb, item_off.p, cnt.p, qs.p, qs.p,
-                         bs.p, gsz);
+      if (one_kernel) {
+        hipLaunchKernelGGL(k_batch_level, dim3(grid), dim3(block), level_lds, s, dq.p, nb, cnt.p, item_off.p, qs.p,
+                           bs.p, gsz, l);
+      } else {
+        hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanBlock), 0, s, dq.p, cnt.p, qs.p, nb, item_off.p, bs.p);
+        hipLaunchKernelGGL(k_batch_expand, dim3(grid), dim3(block), 0, s, dq.p, nb, item_off.p, cnt.p, qs.p, qs.p,
+                           bs.p, gsz);
+      }
     }
+    if (one_kernel) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, s, bs.p, levels);
     hipError_t ce = hipStreamEndCapture(s, &ge.graph);
     if (ce != hipSuccess) return set_error(CG_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
     CG_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
@@ -547,7 +680,10 @@ int run_batched(cg_graph* const* graphs, const int* algs, const uint32_t* roots,
   auto batch_loop = [&]() -> int {
     const unsigned ig = (unsigned)std::min<uint64_t>((totalV + 255) / 256, 65536);
     hipLaunchKernelGGL(k_batch_init, dim3(ig ? ig : 1), dim3(256), 0, s, dq.p, nb, dvpre.p, totalV);
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1024), 0, s, dq.p, cnt.p, qs.p, nb, bs.p);
+    if (one_kernel)
+      hipLaunchKernelGGL(k_batch_level_reset, dim3(1), dim3(1024), 0, s, dq.p, cnt.p, item_off.p, qs.p, nb, bs.p);
+    else
+      hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1024), 0, s, dq.p, cnt.p, qs.p, nb, bs.p);
     BATCH_LAUNCH_OK();
     for (;;) {
       CG_HIP(hipGraphLaunch(ge.exec, s));

@@ -178,3 +178,29 @@ def test_batch_hub_deferral(oracle, hub, delta):
     for o, a, r, got in zip(qo, qa, qr, outs):
         assert np.array_equal(got, oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)), (a, r, hub, delta)
     assert st["edges_traversed"] == sum(oracle.reached_edges(o, x) for o, x in zip(qo, outs))
+
+
+def test_batch_many_queries(oracle):
+    """More queries than the one-kernel-per-level form takes (4096): the
+    batch plans each level in a kernel of its own; every output is the
+    oracle's."""
+    ogs = [oracle.grid_graph(9, 0.1, seed=1), oracle.rmat_graph(7, 4, seed=2, weights=(1, 50))]
+    gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
+    rng = np.random.default_rng(5)
+    qg, qo, qa, qr = [], [], [], []
+    for i in range(4200):
+        k = i % 2
+        qg.append(gs[k])
+        qo.append(ogs[k])
+        qa.append((i // 2) % 2)
+        qr.append(int(rng.integers(0, ogs[k].V)))
+    outs, st = batch.run_batch(qg, qa, qr, batch_levels=8, batch_grid=256)
+    refs = {}
+    for o, a, r, got in zip(qo, qa, qr, outs):
+        key = (id(o), a, r)
+        if key not in refs:
+            refs[key] = oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)
+        assert np.array_equal(got, refs[key]), (a, r)
+    assert st["edges_traversed"] == sum(oracle.reached_edges(o, x) for o, x in zip(qo, outs))
+    for g in gs:
+        g.close()
