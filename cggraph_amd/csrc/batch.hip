@@ -513,11 +513,17 @@ struct EngineQuery {
   uint32_t* slot;  // the query's V values, in the caller's ids
 };
 
+// A graph's batch engines are shared by every batch over the graph and run
+// one query at a time: batches running at once take turns for their engine
+// queries (their level loops still overlap).
+std::mutex batch_engine_run_mu;
+
 // The engine queries in order, on the calling host thread; returns once every
 // copy into a slot has finished.  Errors come back in rc / msg (the error
 // text is per thread).
 static void run_engine_queries(int dev, const std::vector<EngineQuery>* jobs, int* rc, std::string* msg,
                                unsigned long long* edges) {
+  std::lock_guard<std::mutex> turn(batch_engine_run_mu);
   auto fail = [&](int code, const char* what) {
     if (*rc == CG_OK) {
       *rc = code;

@@ -181,7 +181,9 @@ typedef struct cg_options {
                                      schedule; SSSP defers hubs of out-degree >= 128 within
                                      max weight / 50 of the frontier's smallest), on a host thread
                                      of their own while the other queries advance level by
-                                     level; 0 = 2^18, 0xFFFFFFFF = every query batched */
+                                     level; 0 = 2^18, 0xFFFFFFFF = every query batched.  The
+                                     engines belong to the graph, so batches running at once
+                                     from several threads take turns for their engine queries */
 } cg_options;
 
 /* Fault injection, forced from the host so every kernel still exits normally. */

@@ -33,6 +33,45 @@ def test_batch_matches_oracle(oracle):
     assert st1["edges_traversed"] == st["edges_traversed"]
 
 
+def test_batches_from_two_threads(oracle):
+    """Two batches over the same graphs at once, from two host threads: the
+    graphs' engines serve one query at a time, so both batches' outputs are
+    the oracle's."""
+    import threading
+    ogs = [oracle.rmat_graph(12, 16, seed=6, weights=(1, 1000)), oracle.grid_graph(30, 0.1, seed=6)]
+    gs = [cg.Graph.from_csr(o.offsets, o.dest, o.weight) for o in ogs]
+    rng = np.random.default_rng(9)
+    qs = []
+    for t in range(2):
+        qg, qo, qa, qr = [], [], [], []
+        for i in range(20):
+            k = i % 2
+            qg.append(gs[k])
+            qo.append(ogs[k])
+            qa.append((i // 2) % 2)
+            qr.append(int(rng.integers(0, ogs[k].V)))
+        qs.append((qg, qo, qa, qr))
+    results = [None, None]
+
+    def work(t):
+        qg, _, qa, qr = qs[t]
+        results[t] = batch.run_batch(qg, qa, qr, batch_engine_v=1, batch_levels=4, batch_grid=64)
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(2)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    for t in range(2):
+        _, qo, qa, qr = qs[t]
+        outs, st = results[t]
+        assert st["heavy_iterations"] == 10
+        for o, a, r, got in zip(qo, qa, qr, outs):
+            assert np.array_equal(got, oracle.bfs(o, r) if a == 0 else oracle.sssp(o, r)), (t, a, r)
+    for g in gs:
+        g.close()
+
+
 @pytest.mark.parametrize("engine_v", [1, 3000, 0xFFFFFFFF])
 def test_batch_engine_split(oracle, engine_v):
     """Queries on skewed graphs of at least batch_engine_v vertices run on an
