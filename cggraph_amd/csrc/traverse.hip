@@ -1266,12 +1266,32 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
                                                                DevStats* __restrict__ next,
                                                                DevStats* __restrict__ host_out, int track,
                                                                const float* __restrict__ mlp, uint32_t V, uint64_t E) {
-  if (rs->done) return;  // queued behind the traversal's end (uniform)
   constexpr int kC = 11;  // n_active, n_zero, hist[8], n_def
   constexpr int kFW = kFinishBlock / kWave;
   __shared__ unsigned long long s_m[kFW], s_md[kFW];
   __shared__ uint32_t s_c[kFW][kC];
   __shared__ float w_mlp[kMlpLds];
+  // what the record needs besides the partials -- the run state, this
+  // slot's queue counts and minimum, both stripe sets -- is loaded by lanes
+  // of wavefront 1 together with the partials, so the kernel waits on one
+  // round of loads instead of a chain of them
+  __shared__ RunState s_rs;
+  __shared__ uint32_t s_cs[kStripes], s_ms[kStripes], s_q[4];
+  {
+    const uint32_t t = threadIdx.x;
+    if (t >= 64 && t < 64 + kStripes) {
+      s_cs[t - 64] = stripes[(t - 64) * kStripeStride];
+    } else if (t >= 96 && t < 96 + kStripes) {
+      s_ms[t - 96] = track ? stripes[kStripeMinBase + (t - 96) * kStripeStride] : 0xFFFFFFFFu;
+    } else if (t == 128) {
+      s_rs = *rs;
+    } else if (t == 129) {
+      s_q[0] = st->n_small;
+      s_q[1] = st->n_med;
+      s_q[2] = st->n_chunk;
+      s_q[3] = st->min_val;
+    }
+  }
   // the network's weights are loaded now and stored to LDS after the
   // reduction, whose latency hides theirs
   MlpStage<kFinishBlock> wst;
@@ -1310,6 +1330,7 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
     for (int k = 0; k < kC; ++k) s_c[wave_id()][k] = c[k];
   }
   __syncthreads();
+  if (s_rs.done) return;  // queued behind the traversal's end (uniform)
   if (threadIdx.x < kWave) {
     // wavefront 0 sums the wavefronts' sums, one per lane
     const uint32_t l = threadIdx.x;
@@ -1327,13 +1348,13 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
   if (threadIdx.x == 0) {
   uint32_t nn = 0;
 #pragma unroll
-  for (int k = 0; k < kStripes; ++k) nn += stripes[k * kStripeStride];
+  for (int k = 0; k < kStripes; ++k) nn += s_cs[k];
 #pragma unroll
   for (int k = 0; k < kStripes; ++k) stripes[k * kStripeStride] = 0;
-  d.n_small = st->n_small;
-  d.n_med = st->n_med;
-  d.n_chunk = st->n_chunk;
-  d.min_val = st->min_val;
+  d.n_small = s_q[0];
+  d.n_med = s_q[1];
+  d.n_chunk = s_q[2];
+  d.min_val = s_q[3];
   d.n_active = c[0];
   d.n_zero_deg = c[1];
   d.m_active = m;
@@ -1342,9 +1363,9 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
   d.m_def = md;
   // the record describes the frontier this iteration expanded: vertices
   // reached before it, and the first visits of the previous expansion
-  d.visited = rs->visited;
-  d.n_new = rs->pad;
-  d.iter = rs->iter + 1;
+  d.visited = s_rs.visited;
+  d.n_new = s_rs.pad;
+  d.iter = s_rs.iter + 1;
   d.choice = -1;
   rs->visited = d.visited + nn;
   rs->pad = nn;
@@ -1357,7 +1378,7 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
     mv = 0xFFFFFFFFu;
 #pragma unroll
     for (int k = 0; k < kStripes; ++k) {
-      const uint32_t x = stripes[kStripeMinBase + k * kStripeStride];
+      const uint32_t x = s_ms[k];
       mv = x < mv ? x : mv;
       stripes[kStripeMinBase + k * kStripeStride] = 0xFFFFFFFFu;
     }
@@ -1399,7 +1420,7 @@ __global__ __launch_bounds__(kFinishBlock) void k_fused_finish(const StatsPart* 
   next->choice = choice;
   *st = d;
   if (host_out) {
-    const uint32_t itv = record_tag(rs->epoch, d.iter);
+    const uint32_t itv = record_tag(s_rs.epoch, d.iter);
     d.iter = 0xFFFFFFFFu;
     *host_out = d;
     // the release store orders the record before `iter` at system scope
