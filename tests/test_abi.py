@@ -42,6 +42,20 @@ def test_options_defaults_match_reference_constants():
                                         ctypes.sizeof(_lib.PartitionInfo)]
 
 
+def test_tuning_table_names_real_options():
+    """Every option the tuning table stores (engine and batch entries) is a
+    field of cg_options, so the tuned runs apply all of it; the autotuner's
+    search spaces name only real fields too."""
+    from cggraph_amd import autotune
+    fields = {f for f, _ in _lib.Options._fields_}
+    table = autotune.load_table()
+    assert table, "tuning table missing"
+    for sig, rec in table.items():
+        assert set(rec["options"]) <= fields, (sig, set(rec["options"]) - fields)
+        _lib.default_options(**rec["options"])  # every value fits its field
+    assert set(autotune.ENGINE_SPACE) <= fields and set(autotune.BATCH_SPACE) <= fields
+
+
 def test_graph_validation_and_export():
     off = np.array([0, 2, 3, 3], np.uint32)
     dst = np.array([1, 2, 0], np.uint32)
