@@ -162,11 +162,14 @@ def main():
                     help="run 3 + 5 tuned BFS queries on R-MAT scale 18 / ef16 (for rocprofv3 kernel traces)")
     ap.add_argument("--profile-wcc", action="store_true",
                     help="run WCC on symmetrized R-MAT scale 24 / ef16, 2 + 5 times (for rocprofv3 kernel traces)")
+    ap.add_argument("--profile-cc", action="store_true",
+                    help="run directed CC (label propagation) on R-MAT scale 24 / ef16, 2 + 5 times "
+                         "(for rocprofv3 kernel traces)")
     args = ap.parse_args()
     if args.profile_run:
         profile_workload(args)
         return
-    if args.profile_bfs or args.profile_wcc:
+    if args.profile_bfs or args.profile_wcc or args.profile_cc:
         import cggraph_amd as cg
         from cggraph_amd import autotune
         if args.profile_bfs:
@@ -175,13 +178,17 @@ def main():
             opt = autotune.best_options(autotune.engine_signature("rmat", 0, info["V"], info["E"]))
             e = cg.Engine(g, 0, sched_mode=1, **opt)
             dt, st = timed_runs(e, info["max_outdeg_vertex"], 5, 3)
-        else:
+        elif args.profile_wcc:
             g = cg.Graph.rmat(24, 16, seed=1, weights=None, symmetrize=True)
             e = cg.Engine(g, 2, sched_mode=1)
             dt, st = timed_runs(e, 0, 5, 2)
+        else:
+            g = cg.Graph.rmat(24, 16, seed=1, weights=None)
+            e = cg.Engine(g, 2, sched_mode=1)
+            dt, st = timed_runs(e, 0, 5, 2)
         e.close()
-        print(json.dumps({"profile": "bfs_s18" if args.profile_bfs else "wcc_s24", "ms": round(dt * 1e3, 4),
-                          "iterations": st["iterations"]}), flush=True)
+        name = "bfs_s18" if args.profile_bfs else ("wcc_s24" if args.profile_wcc else "cc_s24_directed")
+        print(json.dumps({"profile": name, "ms": round(dt * 1e3, 4), "iterations": st["iterations"]}), flush=True)
         return
     if args.profile_batch or args.profile_pr:
         import cggraph_amd as cg
