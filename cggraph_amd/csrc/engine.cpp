@@ -568,6 +568,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
   // the fused compaction drops frontier vertices already expanded with their
   // present value (it needs the degree array)
   if (fused && dg.deg && e->xval.p) dg.xval = e->xval.p;
+  // label propagation on the fused pipeline pulls its first round from the
+  // in-edges (built once per graph, kept with it)
+  const bool cc_pull1 = fused && alg == CG_CC && !partial && g.E > 0;
+  if (cc_pull1) CG_TRY(build_transpose(&g));
   {
     TraverseInit ti{};
     ti.val = e->val.p;
@@ -604,6 +608,10 @@ int engine_run(cg_engine* e, uint32_t root, cg_run_stats* rs) {
     ti.acc = e->acc.p;
     ti.epoch = e->epoch;
     ti.xval = dg.xval;
+    if (cc_pull1) {
+      ti.in_off = g.d_in_off.p;
+      ti.in_src = g.d_in_src.p;
+    }
     CG_TRY(launch_init_traverse(ti, s));
   }
   bool early_timed[4] = {false, false, false, false};

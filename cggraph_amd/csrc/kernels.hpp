@@ -229,6 +229,10 @@ struct TraverseInit {
   unsigned long long* acc;  // the query's striped accumulators (kAccSlots), cleared
   uint32_t epoch;           // the run's tag (record_tag)
   uint32_t* xval = nullptr; // DevGraph::xval of the run, cleared (V entries)
+  // CC with the in-edges (sorted by source): the first round of labels is
+  // pulled here, and bm0 holds the vertices whose label fell
+  const uint64_t* in_off = nullptr;
+  const uint32_t* in_src = nullptr;
 };
 int launch_init_traverse(const TraverseInit& a, hipStream_t s);
 // BFS under the GPU schedule, two kernels per iteration (traverse.hip): the

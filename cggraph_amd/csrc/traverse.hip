@@ -274,6 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
   const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
   const bool cc = a.alg == CG_CC;
+  const bool pull1 = cc && a.in_off;  // CC: the first round of labels pulled below
   // values (and records) four at a time with 16-byte stores, then the rest
   const uint64_t n4 = a.V / 4;
   for (uint64_t i = t0; i < n4; i += nt) {
@@ -287,11 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
       x = make_uint4(here && r == 0 ? 0u : kUnreached, here && r == 1 ? 0u : kUnreached,
                      here && r == 2 ? 0u : kUnreached, here && r == 3 ? 0u : kUnreached);
     }
-    reinterpret_cast<uint4*>(a.val)[i] = x;
+    if (!pull1) reinterpret_cast<uint4*>(a.val)[i] = x;
     if (a.xval) reinterpret_cast<uint4*>(a.xval)[i] = make_uint4(0u, 0u, 0u, 0u);
   }
   for (uint64_t i = 4 * n4 + t0; i < a.V; i += nt) {
-    a.val[i] = cc ? (uint32_t)i : (i == a.root ? 0u : kUnreached);
+    if (!pull1) a.val[i] = cc ? (uint32_t)i : (i == a.root ? 0u : kUnreached);
     if (a.xval) a.xval[i] = 0;
   }
   for (uint64_t i = t0; i < a.bm_words; i += nt) {
@@ -303,8 +304,32 @@ __global__ __launch_bounds__(kBlock) void k_init_traverse(TraverseInit a) {
     } else if (i == (a.root >> 6)) {
       w = 1ull << (a.root & 63);
     }
-    a.bm0[i] = w;
+    if (!pull1) a.bm0[i] = w;
     a.bm1[i] = 0;
+  }
+  if (pull1) {
+    // Label propagation's first round as a pull: every vertex takes the
+    // smallest of its id and its in-neighbours' ids, and the in-edges are
+    // sorted by source, so that is its first in-edge's source.  The
+    // frontier of the next round is the vertices whose label fell (a vertex
+    // that kept its id has reached its out-neighbours through their pulls).
+    // A wavefront owns a bitmap word: lane l is vertex 64 w + l.
+    const uint64_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t w0 = t0 / kWave, nw = nt / kWave;
+    for (uint64_t w = w0; w < a.bm_words; w += nw) {
+      const uint64_t v = w * kWave + lane;
+      uint32_t label = (uint32_t)v;
+      if (v < a.V) {
+        const uint64_t lo = a.in_off[v], hi = a.in_off[v + 1];
+        if (hi > lo) {
+          const uint32_t s0 = a.in_src[lo];
+          label = s0 < label ? s0 : label;
+        }
+        a.val[v] = label;
+      }
+      const uint64_t fell = __ballot(v < a.V && label < (uint32_t)v);
+      if (lane == 0) a.bm0[w] = fell;
+    }
   }
   for (uint64_t i = t0; i < (uint64_t)kStripeWords; i += nt)
     a.stripes[i] = (a.minima_empty && i >= (uint64_t)kStripeMinBase) ? 0xFFFFFFFFu : 0u;
