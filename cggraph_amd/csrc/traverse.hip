@@ -1524,8 +1524,8 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
       }
       uint32_t total;
       const uint32_t excl = wave_excl_scan(deg, &total);
-      for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
-        const uint32_t j = j0 + lane;
+      // edge j's owner, found by binary search over the exclusive prefix
+      auto owner = [&](uint32_t j, uint64_t& e, uint32_t& osv) {
         uint32_t lo = 0;
 #pragma unroll
         for (uint32_t step = kWave / 2; step >= 1; step >>= 1) {
@@ -1533,9 +1533,38 @@ __device__ __forceinline__ uint32_t expand_items(const DevGraph& g, uint32_t* __
           if (pm <= j) lo += step;
         }
         const uint64_t os = __shfl(s, (int)lo, kWave);
-        const uint32_t osv = __shfl(sv, (int)lo, kWave);
+        osv = __shfl(sv, (int)lo, kWave);
         const uint32_t oex = __shfl(excl, (int)lo, kWave);
-        if (j < total) n_new += relax<ALG, EXACT, TRACK>(osv, os + (j - oex), g.dst, w, val, out, acc);
+        e = os + (j - oex);
+      };
+      // SSSP takes two rounds of 64 edges at a time: both edges' destination,
+      // weight and destination value are loaded before either is committed
+      // (measured neutral to slower for the two-load relaxations)
+      uint32_t j0 = 0;
+      if constexpr (ALG == ALG_SSSP) {
+        for (; j0 + kWave < total; j0 += 2 * kWave) {
+          const uint32_t jb = j0 + kWave + lane;  // j0 + lane < total here
+          uint64_t ea, eb;
+          uint32_t sva, svb;
+          owner(j0 + lane, ea, sva);
+          owner(jb, eb, svb);
+          const bool okb = jb < total;
+          const uint32_t da = g.dst[ea];
+          const uint32_t db = okb ? g.dst[eb] : 0u;
+          const uint32_t ca = candidate<ALG>(sva, ea, w);
+          const uint32_t cb = okb ? candidate<ALG>(svb, eb, w) : 0u;
+          const uint32_t xa = val[da];
+          const uint32_t xb = okb ? val[db] : 0u;
+          n_new += commit<ALG, EXACT, TRACK>(ca, xa, da, val, out, acc);
+          if (okb) n_new += commit<ALG, EXACT, TRACK>(cb, xb, db, val, out, acc);
+        }
+      }
+      for (; j0 < total; j0 += kWave) {
+        const uint32_t j = j0 + lane;
+        uint64_t e;
+        uint32_t osv;
+        owner(j, e, osv);
+        if (j < total) n_new += relax<ALG, EXACT, TRACK>(osv, e, g.dst, w, val, out, acc);
       }
     }
   }
