@@ -1936,9 +1936,21 @@ __global__ __launch_bounds__(kBlock) void k_finish_query(DevGraph g, const uint3
     }
     v0 = n4 * 4;
   }
-  for (uint64_t v = t0; v < g.V; v += nt) {
-    if (acc && v >= v0 && val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
-    if (new_of) out[v] = val[new_of[v]];
+  if (acc)
+    for (uint64_t v = v0 + t0; v < g.V; v += nt)
+      if (val[v] != kUnreached) sum += g.off[v + 1] - g.off[v];
+  if (new_of) {
+    // values back in the caller's ids: 16-byte loads of four ids and stores
+    // of four values, so four gathers are in flight per lane (both buffers
+    // are whole allocations, 16-byte aligned)
+    const uint64_t n4 = g.V / 4;
+    const uint4* o4 = reinterpret_cast<const uint4*>(new_of);
+    uint4* w4 = reinterpret_cast<uint4*>(out);
+    for (uint64_t i = t0; i < n4; i += nt) {
+      const uint4 o = o4[i];
+      w4[i] = make_uint4(val[o.x], val[o.y], val[o.z], val[o.w]);
+    }
+    for (uint64_t v = n4 * 4 + t0; v < g.V; v += nt) out[v] = val[new_of[v]];
   }
   if (!acc) return;  // uniform
   sum = wave_sum_u64(sum);
